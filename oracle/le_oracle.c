@@ -1,0 +1,557 @@
+/*
+ * le_oracle.c -- CPU oracle (test infrastructure only) restating the FP64
+ * spread/interpolate kernels of IBAMR.  See le_oracle.h for scope, pinning and
+ * the rules on who may use this file.  Build with -ffp-contract=off and no
+ * fast-math so that every operation is rounded as written.
+ *
+ * Reference sources (paths relative to the IBAMR checkout):
+ *   ibtk/src/lagrangian/fortran/lagrangian_interaction3d.f.m4
+ *     PIECEWISE_LINEAR interp :493-606, spread :617-730
+ *     IB_4      interp :1203-1334, spread :1344-1475
+ *     IB_6      interp :2178-2375, spread :2384-2582
+ *     BSPLINE_3 interp :2591-2693, spread :2703-2805, loop macros :24-86
+ *   ibtk/src/lagrangian/fortran/lagrangian_interaction2d.f.m4
+ *     PIECEWISE_LINEAR :469-700, IB_4 :1152-1392, IB_6 :1971-2299,
+ *     BSPLINE_3 :2300-2518, loop macros :17-86
+ *   ibtk/src/lagrangian/fortran/lagrangian_delta.f.m4:174-196, 243-260
+ *   ibtk/src/lagrangian/LEInteractor.cpp (registry :2035-2114, side wrappers
+ *     :2402-2489 and :3627-3714, raw index list :6088-6126)
+ *   ibtk/include/ibtk/private/IndexUtilities-inl.h:52-72
+ *   ibtk/src/lagrangian/LIndexSetData.cpp:55-150
+ */
+#include "le_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ registry */
+
+int orc_kernel_from_name(const char* name)
+{
+    if (!name) return -1;
+    if (strcmp(name, "PIECEWISE_LINEAR") == 0) return ORC_PIECEWISE_LINEAR;
+    if (strcmp(name, "IB_4") == 0) return ORC_IB_4;
+    if (strcmp(name, "IB_6") == 0) return ORC_IB_6;
+    if (strcmp(name, "BSPLINE_3") == 0) return ORC_BSPLINE_3;
+    return -1;
+}
+
+/* LEInteractor::getStencilSize (LEInteractor.cpp:2035-2105): BSPLINE_3 reports 4. */
+int orc_stencil_size(int kernel)
+{
+    switch (kernel)
+    {
+    case ORC_PIECEWISE_LINEAR: return 2;
+    case ORC_IB_4: return 4;
+    case ORC_IB_6: return 6;
+    case ORC_BSPLINE_3: return 4;
+    default: return -1;
+    }
+}
+
+/* LEInteractor::getMinimumGhostWidth (LEInteractor.cpp:2108-2114). */
+int orc_min_ghost_width(int kernel)
+{
+    const int s = orc_stencil_size(kernel);
+    return s < 0 ? -1 : (int)floor(0.5 * s) + 1;
+}
+
+/* Number of weights per coordinate the loops actually use. */
+int orc_kernel_width(int kernel)
+{
+    switch (kernel)
+    {
+    case ORC_PIECEWISE_LINEAR: return 2;
+    case ORC_IB_4: return 4;
+    case ORC_IB_6: return 6;
+    case ORC_BSPLINE_3: return 3;
+    default: return -1;
+    }
+}
+
+/* ------------------------------------------------------ scalar delta functions */
+
+/* lagrangian_delta.f.m4:174-196 */
+double orc_ib_4_delta(double r)
+{
+    if (r < 0.0) r = -r;
+    if (r < 1.0)
+    {
+        const double t2 = r * r;
+        const double t6 = sqrt(-4.0 * t2 + 4.0 * r + 1.0);
+        return -r / 4.0 + 3.0 / 8.0 + t6 / 8.0;
+    }
+    if (r < 2.0)
+    {
+        const double t2 = r * r;
+        const double t6 = sqrt(12.0 * r - 7.0 - 4.0 * t2);
+        return -r / 4.0 + 5.0 / 8.0 - t6 / 8.0;
+    }
+    return 0.0;
+}
+
+/* lagrangian_delta.f.m4:243-260 */
+double orc_bspline_3_delta(double x)
+{
+    const double modx = fabs(x);
+    const double r = modx + 1.5;
+    const double r2 = r * r;
+    if (modx <= 0.5) return 0.5 * (-2.0 * r2 + 6.0 * r - 3.0);
+    if (modx <= 1.5) return 0.5 * (r2 - 6.0 * r + 9.0);
+    return 0.0;
+}
+
+/* ------------------------------------------------------------- 1-D weights */
+
+static int nint_(double x) { return (int)lround(x); } /* Fortran NINT: half away from zero */
+
+/* IB_6 weights from r (lagrangian_interaction3d.f.m4:2243-2272).  K is the
+ * PARAMETER of :2227, evaluated in double with one rounding per operation,
+ * which is what compile-time folding produces. */
+static void ib6_weights(double r, double* w)
+{
+    const double K = (59.0 / 60.0) * (1.0 - sqrt(1.0 - (3220.0 / 3481.0)));
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    const double r4 = r2 * r2;
+    const double r6 = r3 * r3;
+    const double alpha = 28.0;
+    /* constant sub-expressions, folded as the Fortran compiler folds them */
+    const double c_b1 = (22.0 / 3.0) - 7.0 * K;
+    const double c_g1 = (((161.0 / 36.0) - (59.0 / 6.0) * K) + 5.0 * (K * K)) * (1.0 / 2.0);
+    const double c_g2 = (-(109.0 / 24.0) + 5.0 * K) * (1.0 / 3.0);
+    const double c_m2 = (1.0 / 12.0) * (3.0 * K - 1.0);
+    const double c_m1 = (1.0 / 6.0) * (4.0 - 3.0 * K);
+    const double beta = (((9.0 / 4.0) - (3.0 / 2.0) * (K + r2)) + c_b1 * r) - (7.0 / 3.0) * r3;
+    const double gamma = (1.0 / 4.0) * ((c_g1 * r2 + c_g2 * r4) + (5.0 / 18.0) * r6);
+    const double discr = beta * beta - (4.0 * alpha) * gamma;
+    const double sgn = ((3.0 / 2.0) - K) >= 0.0 ? 1.0 : -1.0; /* sign(1.d0, 3/2-K) */
+    const double pm3 = (-beta + sgn * sqrt(discr)) / (2.0 * alpha);
+    const double pm2 = ((((-(3.0 * pm3)) - (1.0 / 16.0)) + (1.0 / 8.0) * (K + r2)) + c_m2 * r) + (1.0 / 12.0) * r3;
+    const double pm1 = (((2.0 * pm3) + (1.0 / 4.0)) + c_m1 * r) - (1.0 / 6.0) * r3;
+    const double p = ((2.0 * pm3) + (5.0 / 8.0)) - (1.0 / 4.0) * (K + r2);
+    const double pp1 = (((-(3.0 * pm3)) + (1.0 / 4.0)) - c_m1 * r) + (1.0 / 6.0) * r3;
+    const double pp2 = (((pm3 - (1.0 / 16.0)) + (1.0 / 8.0) * (K + r2)) - c_m2 * r) - (1.0 / 12.0) * r3;
+    w[0] = pm3;
+    w[1] = pm2;
+    w[2] = pm1;
+    w[3] = p;
+    w[4] = pp1;
+    w[5] = pp2;
+}
+
+/* Returns the first stencil index relative to ilower and fills w.  Xs is the
+ * already-shifted coordinate X(d,s)+Xshift(d,l). */
+int orc_weights_1d(int kernel, double Xs, double x_lower, double dx, double* w)
+{
+    switch (kernel)
+    {
+    case ORC_IB_4:
+    {
+        /* lagrangian_interaction3d.f.m4:1265-1273 */
+        const double X_o_dx = (Xs - x_lower) / dx;
+        const int n = nint_(X_o_dx);
+        const int lo = n - 2;                              /* ic_lower - ilower */
+        const double r = X_o_dx - ((double)(lo + 1) + 0.5);
+        const double q = sqrt(1.0 + (4.0 * r) * (1.0 - r));
+        w[0] = 0.125 * ((3.0 - 2.0 * r) - q);
+        w[1] = 0.125 * ((3.0 - 2.0 * r) + q);
+        w[2] = 0.125 * ((1.0 + 2.0 * r) + q);
+        w[3] = 0.125 * ((1.0 + 2.0 * r) - q);
+        return lo;
+    }
+    case ORC_IB_6:
+    {
+        /* lagrangian_interaction3d.f.m4:2238-2241 */
+        const double X_o_dx = (Xs - x_lower) / dx;
+        const int n = nint_(X_o_dx);
+        const int lo = n - 3;
+        const double r = (1.0 - X_o_dx) + ((double)(lo + 2) + 0.5);
+        ib6_weights(r, w);
+        return lo;
+    }
+    case ORC_BSPLINE_3:
+    {
+        /* lagrangian_interaction3d.f.m4:2655-2676 (weights depend only on ic,
+         * so computing all three before clipping gives the same values) */
+        const int c = (int)floor((Xs - x_lower) / dx);
+        const int lo = c - 1;
+        for (int k = 0; k < 3; ++k)
+        {
+            const double X_cell = x_lower + ((double)(lo + k) + 0.5) * dx;
+            w[k] = orc_bspline_3_delta((Xs - X_cell) / dx);
+        }
+        return lo;
+    }
+    case ORC_PIECEWISE_LINEAR:
+    {
+        /* lagrangian_interaction3d.f.m4:680-699 */
+        const int c = nint_((Xs - x_lower) / dx - 0.5);
+        const double X_cell = x_lower + ((double)c + 0.5) * dx;
+        if (Xs < X_cell)
+        {
+            w[0] = (X_cell - Xs) / dx;
+            w[1] = 1.0 - w[0];
+            return c - 1;
+        }
+        w[0] = 1.0 + (X_cell - Xs) / dx;
+        w[1] = 1.0 - w[0];
+        return c;
+    }
+    default: return 0;
+    }
+}
+
+/* ------------------------------------------------------------ kernel loops */
+
+static int is_ib_family(int kernel) { return kernel == ORC_IB_4 || kernel == ORC_IB_6; }
+
+typedef struct
+{
+    int base[3];  /* absolute index of w[d][0] */
+    int lo[3];    /* clipped loop bounds (absolute) */
+    int hi[3];
+    double w[3][8];
+} stencil_t;
+
+static void make_stencil(int kernel, int ndim, const double* Xs, const double* dx, const double* x_lower,
+                         const int* ilower, const int* iupper, const int* nugc, stencil_t* st)
+{
+    const int W = orc_kernel_width(kernel);
+    for (int d = 0; d < ndim; ++d)
+    {
+        const int rel = orc_weights_1d(kernel, Xs[d], x_lower[d], dx[d], st->w[d]);
+        const int b = ilower[d] + rel;
+        const int glo = ilower[d] - nugc[d], ghi = iupper[d] + nugc[d];
+        st->base[d] = b;
+        st->lo[d] = b > glo ? b : glo;
+        st->hi[d] = (b + W - 1) < ghi ? (b + W - 1) : ghi;
+    }
+}
+
+void orc_interp(int kernel, int ndim, const double* dx, const double* x_lower, int depth, const int* ilower,
+                const int* iupper, const int* nugc, const double* u, const int* indices, const double* Xshift,
+                int nindices, const double* X, double* V)
+{
+    long n[3] = {1, 1, 1};
+    for (int d = 0; d < ndim; ++d) n[d] = iupper[d] - ilower[d] + 1 + 2 * nugc[d];
+    const long ncomp = n[0] * n[1] * n[2];
+    const int ib = is_ib_family(kernel);
+    for (int l = 0; l < nindices; ++l)
+    {
+        const int s = indices[l];
+        double Xs[3];
+        for (int d = 0; d < ndim; ++d) Xs[d] = X[ndim * s + d] + Xshift[ndim * l + d];
+        stencil_t st;
+        make_stencil(kernel, ndim, Xs, dx, x_lower, ilower, iupper, nugc, &st);
+        for (int c = 0; c < depth; ++c)
+        {
+            const double* uc = u + c * ncomp;
+            double v = 0.0;
+            if (ndim == 3)
+            {
+                for (int i2 = st.lo[2]; i2 <= st.hi[2]; ++i2)
+                    for (int i1 = st.lo[1]; i1 <= st.hi[1]; ++i1)
+                        for (int i0 = st.lo[0]; i0 <= st.hi[0]; ++i0)
+                        {
+                            const double w0 = st.w[0][i0 - st.base[0]];
+                            const double w1 = st.w[1][i1 - st.base[1]];
+                            const double w2 = st.w[2][i2 - st.base[2]];
+                            const double uv = uc[(i0 - (ilower[0] - nugc[0])) +
+                                                 n[0] * ((i1 - (ilower[1] - nugc[1])) +
+                                                         n[1] * (long)(i2 - (ilower[2] - nugc[2])))];
+                            /* IB: w(i0,i1,i2) = w0(i0)*(w1(i1)*w2(i2))  (:1288-1296)
+                             * BS: V + w0*w1*w2*u, left to right      (:24-37) */
+                            if (ib)
+                                v = v + (w0 * (w1 * w2)) * uv;
+                            else
+                                v = v + ((w0 * w1) * w2) * uv;
+                        }
+            }
+            else
+            {
+                for (int i1 = st.lo[1]; i1 <= st.hi[1]; ++i1)
+                    for (int i0 = st.lo[0]; i0 <= st.hi[0]; ++i0)
+                    {
+                        const double w0 = st.w[0][i0 - st.base[0]];
+                        const double w1 = st.w[1][i1 - st.base[1]];
+                        const double uv = uc[(i0 - (ilower[0] - nugc[0])) + n[0] * (long)(i1 - (ilower[1] - nugc[1]))];
+                        if (ib)
+                            v = v + (w0 * w1) * uv;
+                        else
+                            v = v + (w0 * w1) * uv;
+                    }
+            }
+            V[depth * s + c] = v;
+        }
+    }
+}
+
+void orc_spread(int kernel, int ndim, const double* dx, const double* x_lower, int depth, const int* indices,
+                const double* Xshift, int nindices, const double* X, const double* V, const int* ilower,
+                const int* iupper, const int* nugc, double* u)
+{
+    long n[3] = {1, 1, 1};
+    for (int d = 0; d < ndim; ++d) n[d] = iupper[d] - ilower[d] + 1 + 2 * nugc[d];
+    const long ncomp = n[0] * n[1] * n[2];
+    const int ib = is_ib_family(kernel);
+    const double vol = ndim == 3 ? (dx[0] * dx[1]) * dx[2] : dx[0] * dx[1];
+    for (int l = 0; l < nindices; ++l)
+    {
+        const int s = indices[l];
+        double Xs[3];
+        for (int d = 0; d < ndim; ++d) Xs[d] = X[ndim * s + d] + Xshift[ndim * l + d];
+        stencil_t st;
+        make_stencil(kernel, ndim, Xs, dx, x_lower, ilower, iupper, nugc, &st);
+        for (int c = 0; c < depth; ++c)
+        {
+            double* uc = u + c * ncomp;
+            const double Vv = V[depth * s + c];
+            if (ndim == 3)
+            {
+                for (int i2 = st.lo[2]; i2 <= st.hi[2]; ++i2)
+                    for (int i1 = st.lo[1]; i1 <= st.hi[1]; ++i1)
+                        for (int i0 = st.lo[0]; i0 <= st.hi[0]; ++i0)
+                        {
+                            const double w0 = st.w[0][i0 - st.base[0]];
+                            const double w1 = st.w[1][i1 - st.base[1]];
+                            const double w2 = st.w[2][i2 - st.base[2]];
+                            double* up = &uc[(i0 - (ilower[0] - nugc[0])) +
+                                             n[0] * ((i1 - (ilower[1] - nugc[1])) +
+                                                     n[1] * (long)(i2 - (ilower[2] - nugc[2])))];
+                            /* IB: wz = w2/(dx0*dx1*dx2); w = w0*(w1*wz); u += w*V  (:1432-1465)
+                             * BS: u + (w0*w1*w2*V)/(dx0*dx1*dx2)                 (:51-64) */
+                            if (ib)
+                                *up = *up + (w0 * (w1 * (w2 / vol))) * Vv;
+                            else
+                                *up = *up + (((w0 * w1) * w2) * Vv) / vol;
+                        }
+            }
+            else
+            {
+                for (int i1 = st.lo[1]; i1 <= st.hi[1]; ++i1)
+                    for (int i0 = st.lo[0]; i0 <= st.hi[0]; ++i0)
+                    {
+                        const double w0 = st.w[0][i0 - st.base[0]];
+                        const double w1 = st.w[1][i1 - st.base[1]];
+                        double* up = &uc[(i0 - (ilower[0] - nugc[0])) + n[0] * (long)(i1 - (ilower[1] - nugc[1]))];
+                        if (ib)
+                            *up = *up + (w0 * (w1 / vol)) * Vv;
+                        else
+                            *up = *up + ((w0 * w1) * Vv) / vol;
+                    }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ binning */
+
+void orc_get_cell_index(int ndim, const double* X, const double* x_lower, const double* x_upper, const double* dx,
+                        const int* ilower, const int* iupper, int* idx)
+{
+    for (int d = 0; d < ndim; ++d)
+    {
+        const double dX_lower = X[d] - x_lower[d], dX_upper = X[d] - x_upper[d];
+        if (fabs(dX_lower) <= fabs(dX_upper))
+            idx[d] = (int)((double)ilower[d] + floor(dX_lower / dx[d]));
+        else
+            idx[d] = (int)((double)iupper[d] + floor(dX_upper / dx[d]) + 1.0);
+    }
+}
+
+int orc_build_local_indices(int ndim, const int* box_lo, const int* box_hi, const double* x_lower,
+                            const double* x_upper, const double* dx, const int* ilower, const int* iupper,
+                            const double* X, int npoints, int* indices)
+{
+    int n = 0;
+    for (int k = 0; k < npoints; ++k)
+    {
+        int idx[3];
+        orc_get_cell_index(ndim, &X[ndim * k], x_lower, x_upper, dx, ilower, iupper, idx);
+        int inside = 1;
+        for (int d = 0; d < ndim; ++d) inside = inside && idx[d] >= box_lo[d] && idx[d] <= box_hi[d];
+        if (inside) indices[n++] = k;
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------ side wrappers */
+
+static void side_geometry(int ndim, int axis, const double* dx, const double* x_lower, const int* iupper,
+                          double* x_lower_axis, int* iupper_axis)
+{
+    for (int d = 0; d < ndim; ++d)
+    {
+        x_lower_axis[d] = x_lower[d];
+        iupper_axis[d] = iupper[d];
+    }
+    x_lower_axis[axis] -= 0.5 * dx[axis];   /* LEInteractor.cpp:2465 */
+    iupper_axis[axis] += 1;                 /* SideGeometry::toSideBox */
+}
+
+static int max_index(const int* indices, int n)
+{
+    int m = -1;
+    for (int l = 0; l < n; ++l)
+        if (indices[l] > m) m = indices[l];
+    return m;
+}
+
+void orc_interp_side(int kernel, int ndim, const double* dx, const double* x_lower, const int* ilower,
+                     const int* iupper, const int* gcw, const double* const* q_axis, const int* indices,
+                     const double* Xshift, int nindices, const double* X, double* Q)
+{
+    if (nindices == 0) return;
+    const int sz = max_index(indices, nindices) + 1;
+    double* buf = (double*)calloc((size_t)sz, sizeof(double));
+    for (int a = 0; a < ndim; ++a)
+    {
+        double xl[3];
+        int iu[3];
+        side_geometry(ndim, a, dx, x_lower, iupper, xl, iu);
+        orc_interp(kernel, ndim, dx, xl, 1, ilower, iu, gcw, q_axis[a], indices, Xshift, nindices, X, buf);
+        for (int l = 0; l < nindices; ++l) Q[ndim * indices[l] + a] = buf[indices[l]];
+    }
+    free(buf);
+}
+
+void orc_spread_side(int kernel, int ndim, const double* dx, const double* x_lower, const int* ilower,
+                     const int* iupper, const int* gcw, double* const* q_axis, const int* indices,
+                     const double* Xshift, int nindices, const double* X, const double* Q)
+{
+    if (nindices == 0) return;
+    const int sz = max_index(indices, nindices) + 1;
+    double* buf = (double*)calloc((size_t)sz, sizeof(double));
+    for (int a = 0; a < ndim; ++a)
+    {
+        double xl[3];
+        int iu[3];
+        side_geometry(ndim, a, dx, x_lower, iupper, xl, iu);
+        for (int l = 0; l < nindices; ++l) buf[indices[l]] = Q[ndim * indices[l] + a];
+        orc_spread(kernel, ndim, dx, xl, 1, indices, Xshift, nindices, X, buf, ilower, iu, gcw, q_axis[a]);
+    }
+    free(buf);
+}
+
+/* ---------------------------------------------------------- patch entries */
+
+typedef struct
+{
+    int key[7];
+    int point;
+    int off[3];
+} entry_t;
+
+static int cmp_entry(const void* a, const void* b)
+{
+    const entry_t* x = (const entry_t*)a;
+    const entry_t* y = (const entry_t*)b;
+    for (int k = 0; k < 7; ++k)
+    {
+        if (x->key[k] < y->key[k]) return -1;
+        if (x->key[k] > y->key[k]) return 1;
+    }
+    return 0;
+}
+
+static int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* ilower, const int* iupper,
+                            int idx_gcw, const int* periodic_shift, const int* domain_lower, const double* dx,
+                            const int* bin, int interior_only, int max_entries, int* indices, double* Xshift)
+{
+    int cap = npoints * (ndim == 3 ? 8 : 4) + 1;
+    entry_t* e = (entry_t*)malloc(sizeof(entry_t) * (size_t)cap);
+    int ne = 0;
+    for (int p = 0; p < npoints; ++p)
+    {
+        const int* c = &cell[ndim * p];
+        int olo[3] = {0, 0, 0}, ohi[3] = {0, 0, 0};
+        if (!interior_only)
+            for (int d = 0; d < ndim; ++d)
+                if (periodic_shift[d] > 0)
+                {
+                    olo[d] = -1;
+                    ohi[d] = 1;
+                }
+        for (int oz = olo[2]; oz <= ohi[2]; ++oz)
+            for (int oy = olo[1]; oy <= ohi[1]; ++oy)
+                for (int ox = olo[0]; ox <= ohi[0]; ++ox)
+                {
+                    const int o[3] = {ox, oy, oz};
+                    int ok = 1;
+                    for (int d = 0; d < ndim; ++d)
+                    {
+                        const int ci = c[d] + o[d] * periodic_shift[d];
+                        const int g = interior_only ? 0 : idx_gcw;
+                        ok = ok && ci >= ilower[d] - g && ci <= iupper[d] + g;
+                    }
+                    if (!ok) continue;
+                    entry_t* en = &e[ne++];
+                    int b[3] = {0, 0, 0};
+                    for (int d = 0; d < ndim; ++d) b[d] = floor_div(c[d] - domain_lower[d], bin[d]);
+                    if (ndim == 3)
+                    {
+                        en->key[0] = b[2];
+                        en->key[1] = o[2];
+                        en->key[2] = b[1];
+                        en->key[3] = o[1];
+                        en->key[4] = b[0];
+                        en->key[5] = o[0];
+                    }
+                    else
+                    {
+                        en->key[0] = 0;
+                        en->key[1] = 0;
+                        en->key[2] = b[1];
+                        en->key[3] = o[1];
+                        en->key[4] = b[0];
+                        en->key[5] = o[0];
+                    }
+                    en->key[6] = p;
+                    en->point = p;
+                    for (int d = 0; d < 3; ++d) en->off[d] = o[d];
+                }
+    }
+    qsort(e, (size_t)ne, sizeof(entry_t), cmp_entry);
+    if (ne <= max_entries)
+    {
+        for (int k = 0; k < ne; ++k)
+        {
+            indices[k] = e[k].point;
+            for (int d = 0; d < ndim; ++d)
+                Xshift[ndim * k + d] = (double)(e[k].off[d] * periodic_shift[d]) * dx[d]; /* LIndexSetData.cpp:116 */
+        }
+    }
+    free(e);
+    return ne;
+}
+
+/* ------------------------------------------------------- brute force check */
+
+double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const double* dx, const double* period,
+                                const double* X, const double* V, int npoints)
+{
+    double vol = 1.0;
+    for (int d = 0; d < ndim; ++d) vol *= dx[d];
+    double acc = 0.0;
+    for (int p = 0; p < npoints; ++p)
+    {
+        double wprod = 1.0;
+        for (int d = 0; d < ndim; ++d)
+        {
+            double wsum = 0.0;
+            const int nimg = period[d] > 0.0 ? 1 : 0;
+            for (int o = -nimg; o <= nimg; ++o)
+            {
+                const double r = (xg[d] - (X[ndim * p + d] + o * period[d])) / dx[d];
+                wsum += kernel == ORC_IB_4 ? orc_ib_4_delta(r) : orc_bspline_3_delta(r);
+            }
+            wprod *= wsum;
+        }
+        acc += wprod * V[p];
+    }
+    return acc / vol;
+}

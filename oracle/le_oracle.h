@@ -1,0 +1,118 @@
+/*
+ * le_oracle.h -- CPU oracle for the Eulerian-Lagrangian spread/interpolate path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of IBAMR's FP64 CPU
+ * algorithm (the m4-generated Fortran kernels behind IBTK::LEInteractor) used
+ * as the checker for the HIP library.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product path never does.
+ *
+ * Every function cites the reference file:line it follows.  Arithmetic is
+ * evaluated in the same operation order as the Fortran source (left to right,
+ * no FMA contraction: build with -ffp-contract=off), NINT is round-half-away
+ * (lround), r**n is expanded as gfortran expands small integer powers
+ * (r2=r*r, r3=r2*r, r4=r2*r2, r6=r3*r3).
+ *
+ * Parity pinning (see tests/test_oracle_*.py):
+ *   - interpolate, IB_4/IB_6/BSPLINE_3, cell data: golden vectors from
+ *     tests/interpolate/interpolate_01_{2d,3d}.<kernel>.output (regenerated
+ *     point sets with mt19937(42)).
+ *   - spread geometry on side data (x_lower - dx/2, side box, depth layout):
+ *     tests/IBTK/ghost_accumulation_01_3d.side.spread.a.output (PIECEWISE_LINEAR).
+ *   - 1-D weights: scalar delta functions (lagrangian_delta.f.m4:174-196,
+ *     243-260), partition of unity, moment and sum-of-squares identities.
+ *   - spread values: adjoint identity with interpolate, brute-force O(N*G)
+ *     evaluation, order independence to round-off.
+ *   - side-centred IB_4/IB_6/BSPLINE_3: no reference fixture exists; pinned by
+ *     the identities above and by the structural identity "side axis a = cell
+ *     kernel on the half-shifted grid".
+ */
+#ifndef LE_ORACLE_H
+#define LE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum orc_kernel
+{
+    ORC_PIECEWISE_LINEAR = 0,
+    ORC_IB_4 = 1,
+    ORC_IB_6 = 2,
+    ORC_BSPLINE_3 = 3
+};
+
+/* Registry (LEInteractor.cpp:2035-2114). Returns -1 for unknown names. */
+int orc_kernel_from_name(const char* name);
+int orc_stencil_size(int kernel);      /* getStencilSize: PWL 2, IB_4 4, IB_6 6, BSPLINE_3 4 */
+int orc_min_ghost_width(int kernel);   /* floor(stencil/2)+1 */
+
+/* Scalar delta functions (lagrangian_delta.f.m4:174-196 and :243-260). */
+double orc_ib_4_delta(double r);
+double orc_bspline_3_delta(double x);
+
+/* 1-D weights of one coordinate exactly as the kernel computes them.
+ * X_o_dx_num is (X+Xshift)-x_lower; returns the first stencil index relative
+ * to ilower (unclipped) and writes the stencil weights into w[] (length
+ * orc_kernel_width(kernel)). */
+int orc_kernel_width(int kernel);      /* PWL 2, IB_4 4, IB_6 6, BSPLINE_3 3 */
+int orc_weights_1d(int kernel, double Xs, double x_lower, double dx, double* w);
+
+/* Fortran-equivalent kernels.  Arrays follow the Fortran declarations:
+ *   u(ilower0-nugc0:iupper0+nugc0, ..., 0:depth-1)  column-major, depth slowest
+ *   X(0:ndim-1, 0:*), V(0:depth-1, 0:*), Xshift(0:ndim-1, 0:nindices-1)
+ * interp:  lagrangian_<k>_interp{2,3}d  (overwrites V(:,indices(l)))
+ * spread:  lagrangian_<k>_spread{2,3}d  (accumulates into u) */
+void orc_interp(int kernel, int ndim, const double* dx, const double* x_lower, int depth,
+                const int* ilower, const int* iupper, const int* nugc, const double* u,
+                const int* indices, const double* Xshift, int nindices, const double* X, double* V);
+void orc_spread(int kernel, int ndim, const double* dx, const double* x_lower, int depth,
+                const int* indices, const double* Xshift, int nindices, const double* X, const double* V,
+                const int* ilower, const int* iupper, const int* nugc, double* u);
+
+/* IndexUtilities::getCellIndex (IndexUtilities-inl.h:52-72). */
+void orc_get_cell_index(int ndim, const double* X, const double* x_lower, const double* x_upper,
+                        const double* dx, const int* ilower, const int* iupper, int* idx);
+
+/* Raw-array index list (LEInteractor.cpp:6088-6126): points whose cell lies in
+ * box [box_lo, box_hi]; returns the count, fills indices in point order. */
+int orc_build_local_indices(int ndim, const int* box_lo, const int* box_hi, const double* x_lower,
+                            const double* x_upper, const double* dx, const int* ilower, const int* iupper,
+                            const double* X, int npoints, int* indices);
+
+/* Side-centred wrappers (LEInteractor.cpp:2402-2489 interp, 3627-3714 spread).
+ * q_axis[a] points to the ghosted array of axis a over toSideBox(box, a).
+ * Q is AoS [n][ndim]; the kernel runs once per axis with depth 1. */
+void orc_interp_side(int kernel, int ndim, const double* dx, const double* x_lower, const int* ilower,
+                     const int* iupper, const int* gcw, const double* const* q_axis, const int* indices,
+                     const double* Xshift, int nindices, const double* X, double* Q);
+void orc_spread_side(int kernel, int ndim, const double* dx, const double* x_lower, const int* ilower,
+                     const int* iupper, const int* gcw, double* const* q_axis, const int* indices,
+                     const double* Xshift, int nindices, const double* X, const double* Q);
+
+/* Lagrangian index entries of one patch as LIndexSetData::cacheLocalIndices
+ * builds them (LIndexSetData.cpp:55-150) for a level whose points have cells
+ * cell[ndim*p+d] (level index space): every point whose cell, or whose periodic
+ * image cell cell+o*N (o in {-1,0,1} per periodic axis, N = periodic_shift),
+ * lies in the patch ghost box [ilower-idx_gcw, iupper+idx_gcw].  Xshift =
+ * (double)(o*N)*dx.  interior_only restricts to o=0 and cells in the patch box.
+ *
+ * Order: the reference iterates its node sets in SAMRAI IndexData order, which
+ * the reference sources do not fix; the oracle uses the canonical order
+ * (bz,oz,by,oy,bx,ox,point) with b = floor((cell - domain_lower)/bin) the
+ * physical bin of the point and o its image offset, which the GPU spread
+ * reproduces exactly.  Returns the number of entries (<= max_entries, or the
+ * required count if larger). */
+int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* ilower, const int* iupper,
+                            int idx_gcw, const int* periodic_shift, const int* domain_lower, const double* dx,
+                            const int* bin, int interior_only, int max_entries, int* indices, double* Xshift);
+
+/* Brute-force O(N*G) evaluation with the scalar delta functions: value of the
+ * spread of point values V (depth 1) at one grid location, periodic images
+ * included.  Independent of the kernel loops (used for IB_4 and BSPLINE_3). */
+double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const double* dx,
+                                const double* period, const double* X, const double* V, int npoints);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

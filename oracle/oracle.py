@@ -1,0 +1,242 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module; the product package never does.  See le_oracle.h for what each
+function restates (reference file:line) and how it is pinned.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+KERNELS = {"PIECEWISE_LINEAR": 0, "IB_4": 1, "IB_6": 2, "BSPLINE_3": 3}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_kernel_from_name.argtypes = [ctypes.c_char_p]
+        L.orc_stencil_size.argtypes = [ctypes.c_int]
+        L.orc_min_ghost_width.argtypes = [ctypes.c_int]
+        L.orc_kernel_width.argtypes = [ctypes.c_int]
+        L.orc_ib_4_delta.argtypes = [ctypes.c_double]
+        L.orc_ib_4_delta.restype = ctypes.c_double
+        L.orc_bspline_3_delta.argtypes = [ctypes.c_double]
+        L.orc_bspline_3_delta.restype = ctypes.c_double
+        L.orc_weights_1d.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_interp.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _dp,
+                                 _ip, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_spread.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int,
+                                 _dp, _dp, _ip, _ip, _ip, _dp]
+        L.orc_get_cell_index.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _ip]
+        L.orc_build_local_indices.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _ip, _ip, _dp,
+                                              ctypes.c_int, _ip]
+        L.orc_interp_side.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip, _ip,
+                                      ctypes.POINTER(_dp), _ip, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_spread_side.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip, _ip,
+                                      ctypes.POINTER(_dp), _ip, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_build_patch_entries.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip,
+                                              _ip, _dp, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.orc_bruteforce_spread_at.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                               ctypes.c_int]
+        L.orc_bruteforce_spread_at.restype = ctypes.c_double
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _dv(x, n=None):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    return a
+
+
+def _iv(x):
+    return np.ascontiguousarray(x, dtype=np.int32)
+
+
+def kernel_id(name):
+    return KERNELS[name] if isinstance(name, str) else int(name)
+
+
+def stencil_size(kernel):
+    return lib().orc_stencil_size(kernel_id(kernel))
+
+
+def min_ghost_width(kernel):
+    return lib().orc_min_ghost_width(kernel_id(kernel))
+
+
+def kernel_width(kernel):
+    return lib().orc_kernel_width(kernel_id(kernel))
+
+
+def ib_4_delta(r):
+    return lib().orc_ib_4_delta(float(r))
+
+
+def bspline_3_delta(x):
+    return lib().orc_bspline_3_delta(float(x))
+
+
+def weights_1d(kernel, Xs, x_lower, dx):
+    """Returns (first stencil index relative to ilower, weights)."""
+    k = kernel_id(kernel)
+    w = np.zeros(8)
+    lo = lib().orc_weights_1d(k, float(Xs), float(x_lower), float(dx), _d(w))
+    return lo, w[: kernel_width(k)].copy()
+
+
+def ghosted_shape(ilower, iupper, gcw, depth=1):
+    """numpy shape (depth, n2, n1, n0) of a column-major ghosted patch array."""
+    n = [iupper[d] - ilower[d] + 1 + 2 * gcw[d] for d in range(len(ilower))]
+    return (depth,) + tuple(reversed(n))
+
+
+def interp(kernel, dx, x_lower, ilower, iupper, nugc, u, X, indices=None, Xshift=None, depth=1):
+    """Fortran-equivalent interp: u is (depth, [n2,] n1, n0); X is (N, ndim). Returns V (N, depth)
+    with untouched rows left as NaN."""
+    ndim = len(ilower)
+    X = _dv(X).reshape(-1, ndim)
+    N = X.shape[0]
+    if indices is None:
+        indices = np.arange(N)
+    indices = _iv(indices)
+    if Xshift is None:
+        Xshift = np.zeros((len(indices), ndim))
+    Xshift = _dv(Xshift)
+    V = np.full((N, depth), np.nan)
+    u = _dv(u)
+    lib().orc_interp(kernel_id(kernel), ndim, _d(_dv(dx)), _d(_dv(x_lower)), depth, _i(_iv(ilower)),
+                     _i(_iv(iupper)), _i(_iv(nugc)), _d(u), _i(indices), _d(Xshift), len(indices), _d(X), _d(V))
+    return V
+
+
+def spread(kernel, dx, x_lower, ilower, iupper, nugc, u, X, V, indices=None, Xshift=None, depth=1):
+    """Fortran-equivalent spread: accumulates into u (modified in place, must be float64 C-contiguous)."""
+    ndim = len(ilower)
+    X = _dv(X).reshape(-1, ndim)
+    N = X.shape[0]
+    if indices is None:
+        indices = np.arange(N)
+    indices = _iv(indices)
+    if Xshift is None:
+        Xshift = np.zeros((len(indices), ndim))
+    Xshift = _dv(Xshift)
+    V = _dv(V).reshape(N, depth)
+    assert u.dtype == np.float64 and u.flags.c_contiguous
+    lib().orc_spread(kernel_id(kernel), ndim, _d(_dv(dx)), _d(_dv(x_lower)), depth, _i(indices), _d(Xshift),
+                     len(indices), _d(X), _d(V), _i(_iv(ilower)), _i(_iv(iupper)), _i(_iv(nugc)), _d(u))
+    return u
+
+
+def get_cell_index(X, x_lower, x_upper, dx, ilower, iupper):
+    X = _dv(X)
+    ndim = len(ilower)
+    X2 = X.reshape(-1, ndim)
+    out = np.zeros((X2.shape[0], ndim), dtype=np.int32)
+    L = lib()
+    xl, xu, h, il, iu = _dv(x_lower), _dv(x_upper), _dv(dx), _iv(ilower), _iv(iupper)
+    for k in range(X2.shape[0]):
+        row = np.ascontiguousarray(X2[k])
+        o = np.zeros(ndim, dtype=np.int32)
+        L.orc_get_cell_index(ndim, _d(row), _d(xl), _d(xu), _d(h), _i(il), _i(iu), _i(o))
+        out[k] = o
+    return out
+
+
+def build_local_indices(box_lo, box_hi, x_lower, x_upper, dx, ilower, iupper, X):
+    ndim = len(ilower)
+    X = _dv(X).reshape(-1, ndim)
+    idx = np.zeros(X.shape[0], dtype=np.int32)
+    n = lib().orc_build_local_indices(ndim, _i(_iv(box_lo)), _i(_iv(box_hi)), _d(_dv(x_lower)),
+                                      _d(_dv(x_upper)), _d(_dv(dx)), _i(_iv(ilower)), _i(_iv(iupper)), _d(X),
+                                      X.shape[0], _i(idx))
+    return idx[:n].copy()
+
+
+def side_axis_shape(ilower, iupper, gcw, axis):
+    iu = list(iupper)
+    iu[axis] += 1
+    return ghosted_shape(ilower, iu, gcw)[1:]
+
+
+def interp_side(kernel, dx, x_lower, ilower, iupper, gcw, q_axes, X, indices=None, Xshift=None):
+    ndim = len(ilower)
+    X = _dv(X).reshape(-1, ndim)
+    N = X.shape[0]
+    if indices is None:
+        indices = np.arange(N)
+    indices = _iv(indices)
+    if Xshift is None:
+        Xshift = np.zeros((len(indices), ndim))
+    Xshift = _dv(Xshift)
+    qs = [_dv(q) for q in q_axes]
+    arr = (_dp * ndim)(*[_d(q) for q in qs])
+    Q = np.full((N, ndim), np.nan)
+    lib().orc_interp_side(kernel_id(kernel), ndim, _d(_dv(dx)), _d(_dv(x_lower)), _i(_iv(ilower)),
+                          _i(_iv(iupper)), _i(_iv(gcw)), arr, _i(indices), _d(Xshift), len(indices), _d(X), _d(Q))
+    return Q
+
+
+def spread_side(kernel, dx, x_lower, ilower, iupper, gcw, q_axes, X, Q, indices=None, Xshift=None):
+    """Accumulates into the float64 C-contiguous arrays q_axes (in place)."""
+    ndim = len(ilower)
+    X = _dv(X).reshape(-1, ndim)
+    N = X.shape[0]
+    if indices is None:
+        indices = np.arange(N)
+    indices = _iv(indices)
+    if Xshift is None:
+        Xshift = np.zeros((len(indices), ndim))
+    Xshift = _dv(Xshift)
+    for q in q_axes:
+        assert q.dtype == np.float64 and q.flags.c_contiguous
+    arr = (_dp * ndim)(*[_d(q) for q in q_axes])
+    Q = _dv(Q).reshape(N, ndim)
+    lib().orc_spread_side(kernel_id(kernel), ndim, _d(_dv(dx)), _d(_dv(x_lower)), _i(_iv(ilower)),
+                          _i(_iv(iupper)), _i(_iv(gcw)), arr, _i(indices), _d(Xshift), len(indices), _d(X), _d(Q))
+    return q_axes
+
+
+def build_patch_entries(cell, ilower, iupper, idx_gcw, periodic_shift, domain_lower, dx, bin_size,
+                        interior_only=False):
+    ndim = len(ilower)
+    cell = _iv(cell).reshape(-1, ndim)
+    npts = cell.shape[0]
+    cap = npts * (2 ** ndim) + 1
+    idx = np.zeros(cap, dtype=np.int32)
+    sh = np.zeros((cap, ndim))
+    n = lib().orc_build_patch_entries(ndim, _i(cell), npts, _i(_iv(ilower)), _i(_iv(iupper)), int(idx_gcw),
+                                      _i(_iv(periodic_shift)), _i(_iv(domain_lower)), _d(_dv(dx)),
+                                      _i(_iv(bin_size)), int(bool(interior_only)), cap, _i(idx), _d(sh))
+    return idx[:n].copy(), sh[:n].copy()
+
+
+def bruteforce_spread_at(kernel, xg, dx, period, X, V):
+    ndim = len(dx)
+    X = _dv(X).reshape(-1, ndim)
+    return lib().orc_bruteforce_spread_at(kernel_id(kernel), ndim, _d(_dv(xg)), _d(_dv(dx)), _d(_dv(period)),
+                                          _d(X), _d(_dv(V)), X.shape[0])
