@@ -1,0 +1,266 @@
+"""ibamr_amd -- MI355X (gfx950) Eulerian-Lagrangian coupling for IBAMR-style
+immersed-boundary codes: regularized-delta spreading and interpolation
+(IB_4, IB_6, BSPLINE_3) on cell- and side-centred patch data.
+
+Python host layer over the C ABI of libibamr_amd.so (include/ibamr_amd.h).
+Device memory comes from torch tensors (plumbing only); every computation runs
+in the HIP library.  Importing works without a GPU; the library must have been
+built (python -m ibamr_amd.build), otherwise loading raises.
+
+Interface mirror of the reference (IBTK):
+  is_known_kernel / get_stencil_size / get_minimum_ghost_width
+      LEInteractor::isKnownKernel / getStencilSize / getMinimumGhostWidth
+      (ibtk/src/lagrangian/LEInteractor.cpp:2035-2114)
+  LevelPlan.spread / LevelPlan.interp
+      LDataManager::spread / LDataManager::interp for one level
+      (ibtk/src/lagrangian/LDataManager.cpp:552-667, 699-813), which call
+      LEInteractor::spread / interpolate per patch.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libibamr_amd.so")
+
+IBAMR_OK = 0
+IBAMR_ERR_BAD_KERNEL = -1
+IBAMR_ERR_GHOST_WIDTH = -2
+IBAMR_ERR_DEPTH = -3
+IBAMR_ERR_INVALID = -4
+IBAMR_ERR_OUT_OF_DOMAIN = -5
+IBAMR_ERR_CAPACITY = -6
+IBAMR_ERR_DEVICE = -7
+IBAMR_ERR_NOT_BINNED = -8
+
+CELL, SIDE = 0, 1
+FP64, FP32C = 0, 1
+ACCUMULATE, OVERWRITE = 0, 1
+
+# Every symbol include/ibamr_amd.h declares (checked by the CPU test suite).
+C_API = [
+    "ibamr_is_known_kernel", "ibamr_get_stencil_size", "ibamr_get_minimum_ghost_width",
+    "ibamr_plan_create", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
+    "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
+    "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_info", "ibamr_last_error",
+]
+
+
+class IBAMRError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("ibamr_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class PatchGeom(ctypes.Structure):
+    _fields_ = [("ilower", ctypes.c_int * 3), ("iupper", ctypes.c_int * 3),
+                ("x_lower", ctypes.c_double * 3), ("x_upper", ctypes.c_double * 3)]
+
+
+class LevelGeom(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int), ("dx", ctypes.c_double * 3),
+                ("domain_lower", ctypes.c_int * 3), ("domain_upper", ctypes.c_int * 3),
+                ("domain_x_lower", ctypes.c_double * 3), ("domain_x_upper", ctypes.c_double * 3),
+                ("periodic_shift", ctypes.c_int * 3), ("npatches", ctypes.c_int),
+                ("patches", ctypes.POINTER(PatchGeom))]
+
+
+class PlanConfig(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_char_p), ("centering", ctypes.c_int), ("precision", ctypes.c_int),
+                ("depth", ctypes.c_int), ("data_gcw", ctypes.c_int * 3), ("idx_gcw", ctypes.c_int),
+                ("max_points", ctypes.c_int), ("drift_cells", ctypes.c_int), ("tile", ctypes.c_int)]
+
+
+_LIB = None
+
+
+def lib():
+    """Loads libibamr_amd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libibamr_amd.so is not built (%s); run python -m ibamr_amd.build" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.ibamr_last_error.restype = ctypes.c_char_p
+    L.ibamr_is_known_kernel.argtypes = [ctypes.c_char_p]
+    L.ibamr_get_stencil_size.argtypes = [ctypes.c_char_p]
+    L.ibamr_get_minimum_ghost_width.argtypes = [ctypes.c_char_p]
+    L.ibamr_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(LevelGeom), ctypes.POINTER(PlanConfig)]
+    L.ibamr_plan_destroy.argtypes = [vp]
+    L.ibamr_plan_bind_data.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.ibamr_plan_bin.argtypes = [vp, vp, ctypes.c_int, vp]
+    L.ibamr_plan_bin_async.argtypes = [vp, vp, ctypes.c_int, vp]
+    L.ibamr_plan_check.argtypes = [vp, vp]
+    L.ibamr_plan_spread.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
+    L.ibamr_plan_interp.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+    L.ibamr_plan_fill_periodic_ghosts.argtypes = [vp, ctypes.c_int, vp]
+    L.ibamr_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int)]
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != IBAMR_OK:
+        raise IBAMRError(rc, lib().ibamr_last_error().decode())
+    return rc
+
+
+def is_known_kernel(name):
+    return bool(lib().ibamr_is_known_kernel(name.encode()))
+
+
+def get_stencil_size(name):
+    v = lib().ibamr_get_stencil_size(name.encode())
+    if v < 0:
+        _check(v)
+    return v
+
+
+def get_minimum_ghost_width(name):
+    v = lib().ibamr_get_minimum_ghost_width(name.encode())
+    if v < 0:
+        _check(v)
+    return v
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Patch:
+    """A patch box on a level (cell indices) and its physical extent."""
+
+    def __init__(self, ilower, iupper, x_lower, x_upper):
+        self.ilower, self.iupper = list(ilower), list(iupper)
+        self.x_lower, self.x_upper = list(x_lower), list(x_upper)
+
+
+class Level:
+    """Level geometry: dx, domain box and extent, periodic shift, patches."""
+
+    def __init__(self, dx, domain_lower, domain_upper, domain_x_lower, domain_x_upper, periodic_shift, patches):
+        self.ndim = len(dx)
+        self.dx = list(dx)
+        self.domain_lower, self.domain_upper = list(domain_lower), list(domain_upper)
+        self.domain_x_lower, self.domain_x_upper = list(domain_x_lower), list(domain_x_upper)
+        self.periodic_shift = list(periodic_shift)
+        self.patches = list(patches)
+
+    @staticmethod
+    def uniform_periodic(n, ndim=3, length=1.0):
+        """One patch covering a periodic box [0, length)^ndim with n cells per axis."""
+        h = length / n
+        p = Patch([0] * ndim, [n - 1] * ndim, [0.0] * ndim, [length] * ndim)
+        return Level([h] * ndim, [0] * ndim, [n - 1] * ndim, [0.0] * ndim, [length] * ndim, [n] * ndim, [p])
+
+    def to_c(self):
+        pg = (PatchGeom * len(self.patches))()
+        for i, p in enumerate(self.patches):
+            for d in range(self.ndim):
+                pg[i].ilower[d] = p.ilower[d]
+                pg[i].iupper[d] = p.iupper[d]
+                pg[i].x_lower[d] = p.x_lower[d]
+                pg[i].x_upper[d] = p.x_upper[d]
+        L = LevelGeom()
+        L.ndim = self.ndim
+        for d in range(self.ndim):
+            L.dx[d] = self.dx[d]
+            L.domain_lower[d] = self.domain_lower[d]
+            L.domain_upper[d] = self.domain_upper[d]
+            L.domain_x_lower[d] = self.domain_x_lower[d]
+            L.domain_x_upper[d] = self.domain_x_upper[d]
+            L.periodic_shift[d] = self.periodic_shift[d]
+        L.npatches = len(self.patches)
+        L.patches = ctypes.cast(pg, ctypes.POINTER(PatchGeom))
+        return L, pg
+
+
+def array_shape(level, patch_index, gcw, centering, axis=0, depth=1):
+    """Shape (torch/numpy, C order) of one ghosted Eulerian array of a patch:
+    (depth, n2, n1, n0) for cell data, (n2, n1, n0) of axis `axis` for side data."""
+    p = level.patches[patch_index]
+    n = [p.iupper[d] - p.ilower[d] + 1 + 2 * gcw[d] + (1 if (centering == SIDE and d == axis) else 0)
+         for d in range(level.ndim)]
+    shp = tuple(reversed(n))
+    return shp if centering == SIDE else (depth,) + shp
+
+
+class LevelPlan:
+    """Spread/interp plan for one level (the LDataManager::spread/interp level loop)."""
+
+    def __init__(self, level, kernel="IB_4", centering=SIDE, precision=FP64, depth=1, data_gcw=None,
+                 idx_gcw=0, max_points=0, drift_cells=0):
+        self.level = level
+        self.kernel = kernel
+        self.centering = centering
+        self.precision = precision
+        self.depth = depth
+        g = get_minimum_ghost_width(kernel) if is_known_kernel(kernel) else 1
+        self.data_gcw = list(data_gcw) if data_gcw is not None else [g] * level.ndim
+        Lc, self._pg = level.to_c()
+        cfg = PlanConfig()
+        cfg.kernel = kernel.encode()
+        cfg.centering = centering
+        cfg.precision = precision
+        cfg.depth = depth
+        for d in range(level.ndim):
+            cfg.data_gcw[d] = self.data_gcw[d]
+        cfg.idx_gcw = idx_gcw
+        cfg.max_points = max_points
+        cfg.drift_cells = drift_cells
+        cfg.tile = 0
+        h = ctypes.c_void_p()
+        _check(lib().ibamr_plan_create(ctypes.byref(h), ctypes.byref(Lc), ctypes.byref(cfg)))
+        self._h = h
+        self._bound = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ibamr_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def narr(self):
+        return self.level.ndim if self.centering == SIDE else 1
+
+    def info(self):
+        nt, ts, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().ibamr_plan_info(self._h, ctypes.byref(nt), ctypes.byref(ts), ctypes.byref(nb)))
+        return {"ntiles": nt.value, "tile": ts.value, "nbins": nb.value}
+
+    def bind(self, slot, arrays):
+        """arrays: flat list [patch0 axis0, patch0 axis1, ..., patch1 axis0, ...] of device tensors."""
+        ptrs = (ctypes.c_void_p * len(arrays))(*[a.data_ptr() for a in arrays])
+        _check(lib().ibamr_plan_bind_data(self._h, slot, ptrs))
+        self._bound[slot] = list(arrays)  # keep alive
+
+    def bin(self, X, stream=None, check=True):
+        n = X.shape[0]
+        fn = lib().ibamr_plan_bin if check else lib().ibamr_plan_bin_async
+        _check(fn(self._h, _ptr(X), n, _stream_ptr(stream)))
+
+    def check(self, stream=None):
+        _check(lib().ibamr_plan_check(self._h, _stream_ptr(stream)))
+
+    def spread(self, slot, F, X, mode=ACCUMULATE, stream=None):
+        _check(lib().ibamr_plan_spread(self._h, slot, _ptr(F), _ptr(X), mode, _stream_ptr(stream)))
+
+    def interp(self, slot, U, X, stream=None):
+        _check(lib().ibamr_plan_interp(self._h, slot, _ptr(U), _ptr(X), _stream_ptr(stream)))
+
+    def fill_periodic_ghosts(self, slot, stream=None):
+        _check(lib().ibamr_plan_fill_periodic_ghosts(self._h, slot, _stream_ptr(stream)))

@@ -1,0 +1,1163 @@
+// le_plan.hip -- level plans: point binning, deterministic gather spread,
+// tensor-product interpolation and periodic ghost fill on gfx950.
+//
+// Reference path (IBAMR): LDataManager::spread/interp (LDataManager.cpp:552-667,
+// 699-813) -> LEInteractor::spread/interpolate (side wrappers LEInteractor.cpp:
+// 2402-2489, 3627-3714) -> lagrangian_<kernel>_{spread,interp}{2,3}d
+// (lagrangian_interaction{2,3}d.f.m4).
+//
+// Design (MI355X-first, not a translation of the per-point Fortran loops):
+//  * binning: IndexUtilities::getCellIndex per point (FP64, bit-exact), then a
+//    stable radix sort of points by spread bin (a cube of cells); CSR offsets.
+//  * spread: owner-computes gather.  One 64-lane wavefront owns one tile of
+//    grid values of one array (side axis or depth component) of one patch, in
+//    LDS.  It visits the bins whose points can reach the tile in a fixed order
+//    (bin z, image z, bin y, image y, bin x, image x) and the points of each
+//    bin in index order, and adds each point's clipped stencil to the tile.
+//    A tile is touched by one wavefront only, so every grid value sums its
+//    contributions sequentially in that fixed order: no atomics, bitwise
+//    reproducible, and identical to the reference loop run over the same
+//    entry order.  Periodic images (LIndexSetData.cpp:80-117) are generated on
+//    the fly from the bin offsets.
+//  * interp: one thread per point in bin order (neighbouring threads read
+//    neighbouring grid values), all axes per thread, overwriting U.
+#include "ibamr_amd.h"
+#include "le_weights.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace ibamr
+{
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                          \
+    do                                                                                        \
+    {                                                                                         \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(IBAMR_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+static constexpr int WAVE = 64;
+static constexpr int NSLOT = 64;
+static constexpr int MAX_SLOTS = 8;
+
+struct PatchDev
+{
+    int ilo[3];
+    int ihi[3];
+    double xlo[3];
+};
+
+struct TileDesc
+{
+    int patch;
+    int comp;   // side: axis; cell: depth component
+    int t[3];   // tile coordinates inside the array's ghost box
+};
+
+static int kernel_id(const char* name)
+{
+    if (!name) return -1;
+    if (!std::strcmp(name, "IB_4")) return K_IB4;
+    if (!std::strcmp(name, "IB_6")) return K_IB6;
+    if (!std::strcmp(name, "BSPLINE_3")) return K_BS3;
+    if (!std::strcmp(name, "PIECEWISE_LINEAR")) return K_PWL;
+    return -1;
+}
+
+static int stencil_size(int k)
+{
+    switch (k)
+    {
+    case K_PWL: return 2;
+    case K_IB4: return 4;
+    case K_IB6: return 6;
+    case K_BS3: return 4;  // LEInteractor.cpp:2057-2058
+    default: return -1;
+    }
+}
+
+static int min_ghost(int k) { return stencil_size(k) < 0 ? -1 : stencil_size(k) / 2 + 1; }
+
+__host__ __device__ inline int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+} // namespace ibamr
+
+using namespace ibamr;
+
+struct ibamr_plan
+{
+    int ndim = 3;
+    int kernel = K_IB4;
+    int centering = IBAMR_SIDE;
+    int precision = IBAMR_FP64;
+    int depth = 1;
+    int narr = 3;           // arrays per patch
+    int fdepth = 3;         // F/U components per point
+    int data_gcw[3] = {0, 0, 0};
+    int idx_gcw = 0;
+    int reach = 0;
+    int tile = 8;
+    int bin = 8;
+    int drift = 0;
+    double dx[3] = {1, 1, 1};
+    int dom_lo[3] = {0, 0, 0}, dom_hi[3] = {0, 0, 0};
+    double dom_xlo[3] = {0, 0, 0}, dom_xhi[3] = {0, 0, 0};
+    int pshift[3] = {0, 0, 0};
+    int nb[3] = {1, 1, 1};
+    int nbins = 1;
+    int npatches = 0;
+    std::vector<PatchDev> patches_h;
+    int max_points = 0;
+    int npoints = -1;       // points binned (-1: not binned)
+
+    // device buffers
+    PatchDev* patches_d = nullptr;
+    TileDesc* tiles_d = nullptr;
+    int ntiles = 0;
+    int* bin_patch_start_d = nullptr;  // [nbins+1]
+    int* bin_patches_d = nullptr;
+    void** arrays_d[MAX_SLOTS] = {};
+    int* keys_d = nullptr;
+    int* keys_sorted_d = nullptr;
+    int* idx_d = nullptr;
+    int* perm_d = nullptr;
+    int4* cell_d = nullptr;
+    int4* cell_sorted_d = nullptr;
+    int* bin_start_d = nullptr;        // [nbins+2]
+    void* sort_tmp_d = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int sort_bits = 1;
+};
+
+// ----------------------------------------------------------------------------- binning
+
+namespace
+{
+struct BinArgs
+{
+    const double* X;
+    int n;
+    double xlo[3], xhi[3], dx[3];
+    int dom_lo[3], dom_hi[3];
+    int nb[3];
+    int bin;
+    int nbins;
+    int* keys;
+    int* idx;
+    int4* cell;
+};
+
+// IndexUtilities::getCellIndex (IndexUtilities-inl.h:52-72) on the level domain
+// (level overload :226-243), then the spread bin of the cell.
+template <int NDIM>
+__global__ void k_bin_keys(BinArgs a)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    int c[3] = {0, 0, 0};
+    bool inside = true;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d)
+    {
+        const double X = a.X[NDIM * p + d];
+        const double dXl = X - a.xlo[d], dXu = X - a.xhi[d];
+        double v;
+        if (fabs(dXl) <= fabs(dXu))
+            v = (double)a.dom_lo[d] + floor(dXl / a.dx[d]);
+        else
+            v = ((double)a.dom_hi[d] + floor(dXu / a.dx[d])) + 1.0;
+        // NaN or far outside: not inside the domain
+        inside = inside && (v >= (double)a.dom_lo[d]) && (v <= (double)a.dom_hi[d]);
+        c[d] = inside ? (int)v : a.dom_lo[d];
+    }
+    int key = a.nbins;  // sentinel: sorts after every bin
+    if (inside)
+    {
+        int b[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) b[d] = (c[d] - a.dom_lo[d]) / a.bin;
+        key = NDIM == 3 ? (b[2] * a.nb[1] + b[1]) * a.nb[0] + b[0] : b[1] * a.nb[0] + b[0];
+    }
+    a.keys[p] = key;
+    a.idx[p] = p;
+    a.cell[p] = make_int4(c[0], c[1], c[2], 0);
+}
+
+// bin_start[b] = first sorted position with key >= b, for b = 0..nbins+1
+__global__ void k_bin_bounds(const int* __restrict__ keys, int n, int nbins, int* __restrict__ start)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const int prev = i == 0 ? -1 : keys[i - 1];
+    const int cur = i == n ? nbins + 1 : keys[i];
+    for (int b = prev + 1; b <= cur; ++b) start[b] = i;
+}
+
+__global__ void k_gather_cells(const int* __restrict__ perm, const int4* __restrict__ cell, int n,
+                               int4* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = cell[perm[i]];
+}
+
+// ----------------------------------------------------------------------------- spread
+
+template <typename T>
+struct SpreadArgs
+{
+    const TileDesc* tiles;
+    const PatchDev* patches;
+    T* const* arrays;
+    const double* X;
+    const T* F;
+    const int* perm;
+    const int4* cells;
+    const int* bin_start;
+    int nb[3];
+    int bin;
+    int dom_lo[3], dom_hi[3];
+    int pshift[3];
+    double dx[3];
+    double vol;
+    int fdepth;
+    int narr;
+    int gcw[3];
+    int idx_gcw;
+    int reach;
+    int side;
+    int mode;
+};
+
+template <int NDIM, int TILE>
+struct TileShape
+{
+    static constexpr int N = NDIM == 3 ? TILE * TILE * TILE : TILE * TILE;
+};
+
+template <typename T>
+struct Acc
+{
+};
+
+// Kahan-compensated add in FP32; plain add in FP64 (bitwise with the reference loop).
+__device__ __forceinline__ void acc_add(double* s, double* /*c*/, int i, double v) { s[i] = s[i] + v; }
+__device__ __forceinline__ void acc_add(float* s, float* c, int i, float v)
+{
+    const float y = v - c[i];
+    const float t = s[i] + y;
+    c[i] = (t - s[i]) - y;
+    s[i] = t;
+}
+
+template <int NDIM, int KER, typename T, int TILE>
+__global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
+{
+    using KT = KernelTraits<KER>;
+    constexpr int W = KT::W;
+    constexpr int TN = TileShape<NDIM, TILE>::N;
+    constexpr bool FP64 = sizeof(T) == 8;
+    __shared__ T s_acc[TN];
+    __shared__ T s_cmp[FP64 ? 1 : TN];
+    __shared__ T s_w[NDIM][W][NSLOT];
+    __shared__ T s_F[NSLOT];
+    __shared__ int s_base[NDIM][NSLOT];
+    __shared__ int s_lo[NDIM][NSLOT];
+    __shared__ int s_n[NDIM][NSLOT];
+    __shared__ int s_act[NSLOT];
+
+    const int lane = threadIdx.x;
+    const TileDesc td = a.tiles[blockIdx.x];
+    const PatchDev P = a.patches[td.patch];
+    const int axis = a.side ? td.comp : -1;
+
+    // array box (ghosted, side box along the axis) and this tile's index range
+    int alo[3] = {0, 0, 0}, ahi[3] = {0, 0, 0}, an[3] = {1, 1, 1};
+    int flo[3] = {0, 0, 0}, fhi[3] = {0, 0, 0}, fn[3] = {1, 1, 1};
+    double xl[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d)
+    {
+        alo[d] = P.ilo[d] - a.gcw[d];
+        ahi[d] = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0);
+        an[d] = ahi[d] - alo[d] + 1;
+        flo[d] = alo[d] + td.t[d] * TILE;
+        fhi[d] = min(flo[d] + TILE - 1, ahi[d]);
+        fn[d] = fhi[d] - flo[d] + 1;
+        xl[d] = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);  // LEInteractor.cpp:3689
+    }
+    T* __restrict__ arr = a.arrays[td.patch * a.narr + (a.side ? td.comp : 0)];
+    const size_t comp_off = a.side ? 0 : (size_t)td.comp * an[0] * an[1] * (NDIM == 3 ? an[2] : 1);
+    const int fcomp = td.comp;
+    const int tcount = fn[0] * fn[1] * (NDIM == 3 ? fn[2] : 1);
+
+    // Cells whose points can reach this tile, inside the patch's Lagrangian ghost box.
+    int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+    bool any = true;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d)
+    {
+        clo[d] = max(flo[d] - a.reach, P.ilo[d] - a.idx_gcw);
+        chi[d] = min(fhi[d] + a.reach, P.ihi[d] + a.idx_gcw);
+        any = any && clo[d] <= chi[d];
+    }
+    // Per-axis (bin, image offset) lists in increasing (bin, offset) order.
+    int lb[3][16], lo_[3][16], ln[3] = {0, 0, 0};
+    if (any)
+    {
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d)
+        {
+            const int N = a.pshift[d];
+            const int omax = N > 0 ? 1 : 0;
+            for (int o = -omax; o <= omax; ++o)
+            {
+                const int pl = max(clo[d] - o * N, a.dom_lo[d]);
+                const int ph = min(chi[d] - o * N, a.dom_hi[d]);
+                if (pl > ph) continue;
+                const int bl = floor_div(pl - a.dom_lo[d], a.bin);
+                const int bh = floor_div(ph - a.dom_lo[d], a.bin);
+                for (int b = bl; b <= bh && ln[d] < 16; ++b)
+                {
+                    // insertion in (b, o) order
+                    int k = ln[d]++;
+                    while (k > 0 && (lb[d][k - 1] > b || (lb[d][k - 1] == b && lo_[d][k - 1] > o)))
+                    {
+                        lb[d][k] = lb[d][k - 1];
+                        lo_[d][k] = lo_[d][k - 1];
+                        --k;
+                    }
+                    lb[d][k] = b;
+                    lo_[d][k] = o;
+                }
+            }
+            any = any && ln[d] > 0;
+        }
+    }
+    if (NDIM == 2) ln[2] = 1, lb[2][0] = 0, lo_[2][0] = 0;
+
+    // Total points in the visited bins (uniform across the wavefront).
+    int total = 0;
+    if (any)
+    {
+        for (int kz = 0; kz < ln[2]; ++kz)
+            for (int ky = 0; ky < ln[1]; ++ky)
+                for (int kx = 0; kx < ln[0]; ++kx)
+                {
+                    const int b = NDIM == 3 ? (lb[2][kz] * a.nb[1] + lb[1][ky]) * a.nb[0] + lb[0][kx]
+                                            : lb[1][ky] * a.nb[0] + lb[0][kx];
+                    total += a.bin_start[b + 1] - a.bin_start[b];
+                }
+    }
+    const bool overwrite = a.mode == IBAMR_SPREAD_OVERWRITE;
+    if (total == 0 && !overwrite) return;
+
+    // Load (accumulate) or clear (overwrite) the tile; LDS layout is TILE-strided.
+    for (int e = lane; e < TN; e += WAVE)
+    {
+        T v = T(0);
+        const int i0 = e % TILE;
+        const int i1 = (e / TILE) % TILE;
+        const int i2 = NDIM == 3 ? e / (TILE * TILE) : 0;
+        if (!overwrite && i0 < fn[0] && i1 < fn[1] && i2 < fn[2])
+        {
+            const size_t g = (size_t)(flo[0] + i0 - alo[0]) +
+                             (size_t)an[0] * ((size_t)(flo[1] + i1 - alo[1]) +
+                                              (NDIM == 3 ? (size_t)an[1] * (size_t)(flo[2] + i2 - alo[2]) : 0));
+            v = arr[comp_off + g];
+        }
+        s_acc[e] = v;
+        if (!FP64) s_cmp[e] = T(0);
+    }
+    __syncthreads();
+
+    const T inv_vol = T(1.0 / a.vol);
+    for (int kz = 0; kz < ln[2] && total > 0; ++kz)
+        for (int ky = 0; ky < ln[1]; ++ky)
+            for (int kx = 0; kx < ln[0]; ++kx)
+            {
+                const int b = NDIM == 3 ? (lb[2][kz] * a.nb[1] + lb[1][ky]) * a.nb[0] + lb[0][kx]
+                                        : lb[1][ky] * a.nb[0] + lb[0][kx];
+                const int o[3] = {lo_[0][kx], lo_[1][ky], lo_[2][kz]};
+                const int s0 = a.bin_start[b], s1 = a.bin_start[b + 1];
+                for (int base = s0; base < s1; base += NSLOT)
+                {
+                    // ---- stage up to 64 points: weights and clipped, tile-intersected ranges
+                    const int k = base + lane;
+                    int act = 0;
+                    if (k < s1)
+                    {
+                        const int4 c4 = a.cells[k];
+                        const int cc[3] = {c4.x, c4.y, c4.z};
+                        act = 1;
+#pragma unroll
+                        for (int d = 0; d < NDIM; ++d)
+                        {
+                            const int ci = cc[d] + o[d] * a.pshift[d];
+                            act = act && ci >= clo[d] && ci <= chi[d];
+                        }
+                        if (act)
+                        {
+                            const int p = a.perm[k];
+#pragma unroll
+                            for (int d = 0; d < NDIM; ++d)
+                            {
+                                // X + Xshift with Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
+                                const double Xs = a.X[NDIM * p + d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
+                                T w[W];
+                                const int rel = weights_1d<KER>(Xs, xl[d], a.dx[d], w);
+                                const int bse = P.ilo[d] + rel;
+                                // clip to the array's ghost box (istart/istop, f.m4:1452-1457), then to the tile
+                                const int l = max(max(bse, alo[d]), flo[d]);
+                                const int h = min(min(bse + W - 1, ahi[d]), fhi[d]);
+                                act = act && l <= h;
+                                s_base[d][lane] = bse;
+                                s_lo[d][lane] = l;
+                                s_n[d][lane] = h - l + 1;
+#pragma unroll
+                                for (int j = 0; j < W; ++j) s_w[d][j][lane] = w[j];
+                            }
+                            const T Fv = a.F[(size_t)a.fdepth * p + fcomp];
+                            s_F[lane] = FP64 ? Fv : Fv * inv_vol;
+                        }
+                    }
+                    s_act[lane] = act;
+                    __syncthreads();
+                    // ---- apply the staged points one after another (single wavefront: program order)
+                    const int nst = min(NSLOT, s1 - base);
+                    for (int j = 0; j < nst; ++j)
+                    {
+                        if (!s_act[j]) continue;
+                        const int n0 = s_n[0][j], n1 = s_n[1][j];
+                        const int n2 = NDIM == 3 ? s_n[2][j] : 1;
+                        const int cnt = n0 * n1 * n2;
+                        const T Fv = s_F[j];
+                        for (int e = lane; e < cnt; e += WAVE)
+                        {
+                            const int e0 = e % n0;
+                            const int e1 = (e / n0) % n1;
+                            const int e2 = e / (n0 * n1);
+                            const int i0 = s_lo[0][j] + e0, i1 = s_lo[1][j] + e1;
+                            const T w0 = s_w[0][i0 - s_base[0][j]][j];
+                            const T w1 = s_w[1][i1 - s_base[1][j]][j];
+                            int li = (i0 - flo[0]) + TILE * (i1 - flo[1]);
+                            T c;
+                            if (NDIM == 3)
+                            {
+                                const int i2 = s_lo[2][j] + e2;
+                                const T w2 = s_w[2][i2 - s_base[2][j]][j];
+                                li += TILE * TILE * (i2 - flo[2]);
+                                if (FP64)
+                                {
+                                    const T vol = T(a.vol);
+                                    // IB: (w0*(w1*(w2/vol)))*V (f.m4:1432-1463); BS: (((w0*w1)*w2)*V)/vol (:51-64)
+                                    c = KT::IB ? (w0 * (w1 * (w2 / vol))) * Fv : (((w0 * w1) * w2) * Fv) / vol;
+                                }
+                                else
+                                    c = (w0 * (w1 * w2)) * Fv;
+                            }
+                            else
+                            {
+                                if (FP64)
+                                {
+                                    const T vol = T(a.vol);
+                                    c = KT::IB ? (w0 * (w1 / vol)) * Fv : ((w0 * w1) * Fv) / vol;
+                                }
+                                else
+                                    c = (w0 * w1) * Fv;
+                            }
+                            // remap the compact tile index to the TILE-strided LDS layout
+                            acc_add(s_acc, s_cmp, li, c);
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+
+    // Write the tile back (strided LDS layout: TILE per row).
+    for (int e = lane; e < tcount; e += WAVE)
+    {
+        const int i0 = e % fn[0];
+        const int i1 = (e / fn[0]) % fn[1];
+        const int i2 = NDIM == 3 ? e / (fn[0] * fn[1]) : 0;
+        const int li = i0 + TILE * (i1 + TILE * i2);
+        const size_t g = (size_t)(flo[0] + i0 - alo[0]) +
+                         (size_t)an[0] * ((size_t)(flo[1] + i1 - alo[1]) +
+                                          (NDIM == 3 ? (size_t)an[1] * (size_t)(flo[2] + i2 - alo[2]) : 0));
+        T v = s_acc[li];
+        if (!FP64) v = v - s_cmp[li];
+        arr[comp_off + g] = v;
+    }
+}
+
+// ----------------------------------------------------------------------------- interp
+
+template <typename T>
+struct InterpArgs
+{
+    const PatchDev* patches;
+    int npatches;
+    const int* bin_patch_start;
+    const int* bin_patches;
+    T* const* arrays;
+    const double* X;
+    T* U;
+    const int* perm;
+    const int4* cells;
+    const int* keys_sorted;
+    int n;
+    double dx[3];
+    int gcw[3];
+    int narr;
+    int fdepth;
+    int depth;
+    int side;
+};
+
+// Compensated (FP32) or plain (FP64, same order as the reference) sum.
+template <typename T>
+struct Sum;
+template <>
+struct Sum<double>
+{
+    double s = 0.0;
+    __device__ void add(double v) { s = s + v; }
+    __device__ double get() const { return s; }
+};
+template <>
+struct Sum<float>
+{
+    float s = 0.0f, c = 0.0f;
+    __device__ void add(float v)
+    {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ float get() const { return s - c; }
+};
+
+template <int NDIM, int KER, typename T>
+__global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a)
+{
+    using KT = KernelTraits<KER>;
+    constexpr int W = KT::W;
+    constexpr bool FP64 = sizeof(T) == 8;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const int key = a.keys_sorted[k];
+    const int4 c4 = a.cells[k];
+    const int cc[3] = {c4.x, c4.y, c4.z};
+    // owning patch: the one whose interior holds the cell (LDataManager.cpp:748-753)
+    int pid = -1;
+    if (key >= 0)
+    {
+        const int b0 = a.bin_patch_start[key], b1 = a.bin_patch_start[key + 1];
+        for (int j = b0; j < b1 && pid < 0; ++j)
+        {
+            const PatchDev& Q = a.patches[a.bin_patches[j]];
+            bool in = true;
+#pragma unroll
+            for (int d = 0; d < NDIM; ++d) in = in && cc[d] >= Q.ilo[d] && cc[d] <= Q.ihi[d];
+            if (in) pid = a.bin_patches[j];
+        }
+    }
+    if (pid < 0) return;
+    const PatchDev P = a.patches[pid];
+    const int p = a.perm[k];
+    double Xp[3];
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) Xp[d] = a.X[NDIM * p + d];
+
+    const int narrays = a.side ? NDIM : 1;
+    for (int ax = 0; ax < narrays; ++ax)
+    {
+        const int axis = a.side ? ax : -1;
+        int alo[3] = {0, 0, 0}, ahi[3] = {0, 0, 0}, an[3] = {1, 1, 1};
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, bse[3] = {0, 0, 0};
+        T w[3][W];
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d)
+        {
+            alo[d] = P.ilo[d] - a.gcw[d];
+            ahi[d] = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0);
+            an[d] = ahi[d] - alo[d] + 1;
+            const double xl = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);  // LEInteractor.cpp:2465
+            const int rel = weights_1d<KER>(Xp[d], xl, a.dx[d], w[d]);
+            bse[d] = P.ilo[d] + rel;
+            lo[d] = max(bse[d], alo[d]);
+            hi[d] = min(bse[d] + W - 1, ahi[d]);
+        }
+        const T* __restrict__ arr = a.arrays[pid * a.narr + (a.side ? ax : 0)];
+        const size_t ncomp = (size_t)an[0] * an[1] * (NDIM == 3 ? an[2] : 1);
+        const int ncomps = a.side ? 1 : a.depth;
+        for (int cd = 0; cd < ncomps; ++cd)
+        {
+            const T* __restrict__ u = arr + cd * ncomp;
+            Sum<T> acc;
+            if (NDIM == 3)
+            {
+                for (int i2 = lo[2]; i2 <= hi[2]; ++i2)
+                {
+                    const T w2 = w[2][i2 - bse[2]];
+                    for (int i1 = lo[1]; i1 <= hi[1]; ++i1)
+                    {
+                        const T w1 = w[1][i1 - bse[1]];
+                        const T* row = u + (size_t)(i1 - alo[1]) * an[0] + (size_t)(i2 - alo[2]) * an[0] * an[1] - alo[0];
+                        if (FP64)
+                        {
+                            for (int i0 = lo[0]; i0 <= hi[0]; ++i0)
+                            {
+                                const T w0 = w[0][i0 - bse[0]];
+                                // IB: V + (w0*(w1*w2))*u (f.m4:1288-1328); BS: V + ((w0*w1)*w2)*u (:24-37)
+                                acc.add(KT::IB ? (w0 * (w1 * w2)) * row[i0] : ((w0 * w1) * w2) * row[i0]);
+                            }
+                        }
+                        else
+                        {
+                            // FP32: one partial sum per grid row (W terms), compensated across rows
+                            const T w12 = w1 * w2;
+                            T part = T(0);
+                            for (int i0 = lo[0]; i0 <= hi[0]; ++i0) part = fmaf(w[0][i0 - bse[0]], row[i0], part);
+                            acc.add(w12 * part);
+                        }
+                    }
+                }
+            }
+            else
+            {
+                for (int i1 = lo[1]; i1 <= hi[1]; ++i1)
+                {
+                    const T w1 = w[1][i1 - bse[1]];
+                    const T* row = u + (size_t)(i1 - alo[1]) * an[0] - alo[0];
+                    if (FP64)
+                    {
+                        for (int i0 = lo[0]; i0 <= hi[0]; ++i0) acc.add((w[0][i0 - bse[0]] * w1) * row[i0]);
+                    }
+                    else
+                    {
+                        T part = T(0);
+                        for (int i0 = lo[0]; i0 <= hi[0]; ++i0) part = fmaf(w[0][i0 - bse[0]], row[i0], part);
+                        acc.add(w1 * part);
+                    }
+                }
+            }
+            a.U[(size_t)a.fdepth * p + (a.side ? ax : cd)] = acc.get();
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------- periodic ghost fill
+
+template <typename T>
+__global__ void k_fill_periodic(T* __restrict__ arr, int n0, int n1, int n2, int g0, int g1, int g2, int N0, int N1,
+                                int N2, long total)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int i0 = (int)(e % n0);
+    const int i1 = (int)((e / n0) % n1);
+    const int i2 = (int)(e / ((long)n0 * n1));
+    // periodic image inside the first N entries of the interior along every axis
+    const int j0 = ((i0 - g0) % N0 + N0) % N0 + g0;
+    const int j1 = ((i1 - g1) % N1 + N1) % N1 + g1;
+    const int j2 = ((i2 - g2) % N2 + N2) % N2 + g2;
+    if (j0 == i0 && j1 == i1 && j2 == i2) return;
+    arr[e] = arr[(long)j0 + (long)n0 * (j1 + (long)n1 * j2)];
+}
+
+} // namespace
+
+// ----------------------------------------------------------------------------- host side
+
+namespace
+{
+template <typename F>
+int dispatch(int ndim, int kernel, int precision, F&& f)
+{
+    auto by_kernel = [&](auto nd, auto prec) -> int {
+        switch (kernel)
+        {
+        case K_IB4: return f(nd, std::integral_constant<int, K_IB4>{}, prec);
+        case K_IB6: return f(nd, std::integral_constant<int, K_IB6>{}, prec);
+        case K_BS3: return f(nd, std::integral_constant<int, K_BS3>{}, prec);
+        default: return fail(IBAMR_ERR_BAD_KERNEL, "kernel has no GPU spread/interp");
+        }
+    };
+    auto by_prec = [&](auto nd) -> int {
+        if (precision == IBAMR_FP64) return by_kernel(nd, (double)0);
+        return by_kernel(nd, (float)0);
+    };
+    if (ndim == 3) return by_prec(std::integral_constant<int, 3>{});
+    return by_prec(std::integral_constant<int, 2>{});
+}
+
+template <typename T>
+void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double* X, int mode, SpreadArgs<T>& a)
+{
+    a.tiles = P->tiles_d;
+    a.patches = P->patches_d;
+    a.arrays = reinterpret_cast<T* const*>(P->arrays_d[slot]);
+    a.X = X;
+    a.F = static_cast<const T*>(F);
+    a.perm = P->perm_d;
+    a.cells = P->cell_sorted_d;
+    a.bin_start = P->bin_start_d;
+    for (int d = 0; d < 3; ++d)
+    {
+        a.nb[d] = P->nb[d];
+        a.dom_lo[d] = P->dom_lo[d];
+        a.dom_hi[d] = P->dom_hi[d];
+        a.pshift[d] = P->pshift[d];
+        a.dx[d] = P->dx[d];
+        a.gcw[d] = P->data_gcw[d];
+    }
+    a.bin = P->bin;
+    a.vol = P->ndim == 3 ? (P->dx[0] * P->dx[1]) * P->dx[2] : P->dx[0] * P->dx[1];
+    a.fdepth = P->fdepth;
+    a.narr = P->narr;
+    a.idx_gcw = P->idx_gcw;
+    a.reach = P->reach;
+    a.side = P->centering == IBAMR_SIDE;
+    a.mode = mode;
+}
+} // namespace
+
+extern "C" {
+
+const char* ibamr_last_error(void) { return ibamr::g_err.c_str(); }
+
+int ibamr_is_known_kernel(const char* k) { return kernel_id(k) >= 0 ? 1 : 0; }
+
+int ibamr_get_stencil_size(const char* k)
+{
+    const int id = kernel_id(k);
+    return id < 0 ? fail(IBAMR_ERR_BAD_KERNEL, std::string("unknown kernel function ") + (k ? k : "(null)"))
+                  : stencil_size(id);
+}
+
+int ibamr_get_minimum_ghost_width(const char* k)
+{
+    const int id = kernel_id(k);
+    return id < 0 ? fail(IBAMR_ERR_BAD_KERNEL, std::string("unknown kernel function ") + (k ? k : "(null)"))
+                  : min_ghost(id);
+}
+
+int ibamr_plan_destroy(ibamr_plan* P)
+{
+    if (!P) return IBAMR_OK;
+    hipFree(P->patches_d);
+    hipFree(P->tiles_d);
+    hipFree(P->bin_patch_start_d);
+    hipFree(P->bin_patches_d);
+    for (int s = 0; s < MAX_SLOTS; ++s) hipFree(P->arrays_d[s]);
+    hipFree(P->keys_d);
+    hipFree(P->keys_sorted_d);
+    hipFree(P->idx_d);
+    hipFree(P->perm_d);
+    hipFree(P->cell_d);
+    hipFree(P->cell_sorted_d);
+    hipFree(P->bin_start_d);
+    hipFree(P->sort_tmp_d);
+    delete P;
+    return IBAMR_OK;
+}
+
+int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_plan_config* C)
+{
+    if (!out || !L || !C) return fail(IBAMR_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const int k = kernel_id(C->kernel);
+    if (k < 0) return fail(IBAMR_ERR_BAD_KERNEL, std::string("unknown kernel function ") + (C->kernel ? C->kernel : "(null)"));
+    if (k == K_PWL) return fail(IBAMR_ERR_BAD_KERNEL, "PIECEWISE_LINEAR has no GPU spread/interp");
+    if (L->ndim != 2 && L->ndim != 3) return fail(IBAMR_ERR_INVALID, "ndim must be 2 or 3");
+    if (C->centering != IBAMR_CELL && C->centering != IBAMR_SIDE) return fail(IBAMR_ERR_INVALID, "bad centering");
+    if (C->precision != IBAMR_FP64 && C->precision != IBAMR_FP32C) return fail(IBAMR_ERR_INVALID, "bad precision");
+    if (C->centering == IBAMR_SIDE && C->depth != 1)
+        return fail(IBAMR_ERR_DEPTH, "side-centered spreading/interpolation requires vector-valued data (q depth 1, Q depth NDIM)");
+    if (C->depth < 1) return fail(IBAMR_ERR_DEPTH, "depth must be >= 1");
+    if (L->npatches < 1 || !L->patches) return fail(IBAMR_ERR_INVALID, "level has no patches");
+    if (C->max_points < 0) return fail(IBAMR_ERR_INVALID, "max_points < 0");
+    const int ndim = L->ndim;
+    const int mg = min_ghost(k);
+    int gmin = 1 << 30;
+    for (int d = 0; d < ndim; ++d) gmin = std::min(gmin, C->data_gcw[d]);
+    if (gmin < mg)
+        return fail(IBAMR_ERR_GHOST_WIDTH, "insufficient ghost cells: kernel " + std::string(C->kernel) + " needs " +
+                                               std::to_string(mg) + ", data ghost width " + std::to_string(gmin));
+    const int igcw = C->idx_gcw > 0 ? C->idx_gcw : mg;
+    if (igcw < mg) return fail(IBAMR_ERR_GHOST_WIDTH, "Lagrangian index ghost width below the kernel minimum");
+    for (int d = 0; d < ndim; ++d)
+    {
+        if (!(L->dx[d] > 0.0)) return fail(IBAMR_ERR_INVALID, "dx must be positive");
+        if (L->domain_upper[d] < L->domain_lower[d]) return fail(IBAMR_ERR_INVALID, "empty domain box");
+        if (L->periodic_shift[d] < 0) return fail(IBAMR_ERR_INVALID, "negative periodic shift");
+        if (L->periodic_shift[d] > 0 && L->periodic_shift[d] < igcw + C->drift_cells)
+            return fail(IBAMR_ERR_INVALID, "periodic extent smaller than the ghost width");
+    }
+
+    ibamr_plan* P = new ibamr_plan();
+    P->ndim = ndim;
+    P->kernel = k;
+    P->centering = C->centering;
+    P->precision = C->precision;
+    P->depth = C->depth;
+    P->narr = C->centering == IBAMR_SIDE ? ndim : 1;
+    P->fdepth = C->centering == IBAMR_SIDE ? ndim : C->depth;
+    P->idx_gcw = igcw;
+    P->drift = std::max(0, C->drift_cells);
+    P->reach = mg + P->drift;
+    P->tile = ndim == 3 ? 8 : 16;
+    if (C->tile != 0 && C->tile != P->tile)
+    {
+        delete P;
+        return fail(IBAMR_ERR_INVALID, "unsupported tile edge");
+    }
+    P->bin = P->tile;
+    // per-axis (bin, image) lists of a tile hold at most 16 entries
+    if (3 * ((P->tile + 2 * P->reach - 1) / P->bin + 2) > 16)
+    {
+        delete P;
+        return fail(IBAMR_ERR_INVALID, "drift_cells too large for the tile size");
+    }
+    for (int d = 0; d < 3; ++d)
+    {
+        const bool on = d < ndim;
+        P->data_gcw[d] = on ? C->data_gcw[d] : 0;
+        P->dx[d] = on ? L->dx[d] : 1.0;
+        P->dom_lo[d] = on ? L->domain_lower[d] : 0;
+        P->dom_hi[d] = on ? L->domain_upper[d] : 0;
+        P->dom_xlo[d] = on ? L->domain_x_lower[d] : 0.0;
+        P->dom_xhi[d] = on ? L->domain_x_upper[d] : 1.0;
+        P->pshift[d] = on ? L->periodic_shift[d] : 0;
+        P->nb[d] = on ? (P->dom_hi[d] - P->dom_lo[d] + P->bin) / P->bin : 1;
+    }
+    const long nbins_l = (long)P->nb[0] * P->nb[1] * P->nb[2];
+    if (nbins_l > (1L << 30))
+    {
+        delete P;
+        return fail(IBAMR_ERR_INVALID, "level too large for 32-bit bin keys");
+    }
+    P->nbins = (int)nbins_l;
+    P->npatches = L->npatches;
+    P->max_points = C->max_points;
+
+    // patches, and the patches overlapping each bin (for point -> patch lookup in interp)
+    std::vector<std::vector<int>> per_bin(P->nbins);
+    for (int i = 0; i < L->npatches; ++i)
+    {
+        const ibamr_patch_geom& g = L->patches[i];
+        PatchDev pd{};
+        for (int d = 0; d < 3; ++d)
+        {
+            const bool on = d < ndim;
+            pd.ilo[d] = on ? g.ilower[d] : 0;
+            pd.ihi[d] = on ? g.iupper[d] : 0;
+            pd.xlo[d] = on ? g.x_lower[d] : 0.0;
+            if (on && (g.iupper[d] < g.ilower[d] || g.ilower[d] < P->dom_lo[d] || g.iupper[d] > P->dom_hi[d]))
+            {
+                delete P;
+                return fail(IBAMR_ERR_INVALID, "patch box empty or outside the domain");
+            }
+        }
+        P->patches_h.push_back(pd);
+        int bl[3] = {0, 0, 0}, bh[3] = {0, 0, 0};
+        for (int d = 0; d < ndim; ++d)
+        {
+            bl[d] = (pd.ilo[d] - P->dom_lo[d]) / P->bin;
+            bh[d] = (pd.ihi[d] - P->dom_lo[d]) / P->bin;
+        }
+        for (int bz = bl[2]; bz <= bh[2]; ++bz)
+            for (int by = bl[1]; by <= bh[1]; ++by)
+                for (int bx = bl[0]; bx <= bh[0]; ++bx) per_bin[(bz * P->nb[1] + by) * P->nb[0] + bx].push_back(i);
+    }
+    std::vector<int> bps(P->nbins + 2, 0), bpl;
+    for (int b = 0; b < P->nbins; ++b)
+    {
+        bps[b] = (int)bpl.size();
+        for (int i : per_bin[b]) bpl.push_back(i);
+    }
+    bps[P->nbins] = (int)bpl.size();
+    bps[P->nbins + 1] = (int)bpl.size();
+    if (bpl.empty()) bpl.push_back(0);
+
+    // work list: every tile of every array of every patch
+    std::vector<TileDesc> tiles;
+    const int ncomp_tiles = C->centering == IBAMR_SIDE ? ndim : C->depth;
+    for (int i = 0; i < L->npatches; ++i)
+    {
+        const PatchDev& pd = P->patches_h[i];
+        for (int c = 0; c < ncomp_tiles; ++c)
+        {
+            const int axis = C->centering == IBAMR_SIDE ? c : -1;
+            int nt[3] = {1, 1, 1};
+            for (int d = 0; d < ndim; ++d)
+            {
+                const int n = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
+                nt[d] = (n + P->tile - 1) / P->tile;
+            }
+            for (int tz = 0; tz < nt[2]; ++tz)
+                for (int ty = 0; ty < nt[1]; ++ty)
+                    for (int tx = 0; tx < nt[0]; ++tx) tiles.push_back(TileDesc{i, c, {tx, ty, tz}});
+        }
+    }
+    P->ntiles = (int)tiles.size();
+
+    auto cleanup_fail = [&](int code) {
+        std::string m = g_err;
+        ibamr_plan_destroy(P);
+        g_err = m;
+        return code;
+    };
+#define PLAN_HIP(expr)                                                                                  \
+    do                                                                                                  \
+    {                                                                                                   \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+        {                                                                                               \
+            fail(IBAMR_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));                    \
+            return cleanup_fail(IBAMR_ERR_DEVICE);                                                      \
+        }                                                                                               \
+    } while (0)
+
+    PLAN_HIP(hipMalloc(&P->patches_d, sizeof(PatchDev) * P->patches_h.size()));
+    PLAN_HIP(hipMemcpy(P->patches_d, P->patches_h.data(), sizeof(PatchDev) * P->patches_h.size(), hipMemcpyHostToDevice));
+    PLAN_HIP(hipMalloc(&P->tiles_d, sizeof(TileDesc) * std::max<size_t>(1, tiles.size())));
+    if (!tiles.empty())
+        PLAN_HIP(hipMemcpy(P->tiles_d, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
+    PLAN_HIP(hipMalloc(&P->bin_patch_start_d, sizeof(int) * bps.size()));
+    PLAN_HIP(hipMemcpy(P->bin_patch_start_d, bps.data(), sizeof(int) * bps.size(), hipMemcpyHostToDevice));
+    PLAN_HIP(hipMalloc(&P->bin_patches_d, sizeof(int) * bpl.size()));
+    PLAN_HIP(hipMemcpy(P->bin_patches_d, bpl.data(), sizeof(int) * bpl.size(), hipMemcpyHostToDevice));
+    const size_t np = (size_t)std::max(1, C->max_points);
+    PLAN_HIP(hipMalloc(&P->keys_d, sizeof(int) * np));
+    PLAN_HIP(hipMalloc(&P->keys_sorted_d, sizeof(int) * np));
+    PLAN_HIP(hipMalloc(&P->idx_d, sizeof(int) * np));
+    PLAN_HIP(hipMalloc(&P->perm_d, sizeof(int) * np));
+    PLAN_HIP(hipMalloc(&P->cell_d, sizeof(int4) * np));
+    PLAN_HIP(hipMalloc(&P->cell_sorted_d, sizeof(int4) * np));
+    PLAN_HIP(hipMalloc(&P->bin_start_d, sizeof(int) * (P->nbins + 2)));
+    int bits = 1;
+    while ((1L << bits) <= (long)P->nbins) ++bits;  // keys 0..nbins
+    P->sort_bits = bits;
+    PLAN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, P->sort_tmp_bytes, reinterpret_cast<unsigned*>(P->keys_d),
+                                                reinterpret_cast<unsigned*>(P->keys_sorted_d), P->idx_d, P->perm_d,
+                                                (int)np, 0, bits));
+    PLAN_HIP(hipMalloc(&P->sort_tmp_d, std::max<size_t>(16, P->sort_tmp_bytes)));
+#undef PLAN_HIP
+    *out = P;
+    return IBAMR_OK;
+}
+
+int ibamr_plan_info(const ibamr_plan* P, int* ntiles, int* tile, int* nbins)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    if (ntiles) *ntiles = P->ntiles;
+    if (tile) *tile = P->tile;
+    if (nbins) *nbins = P->nbins;
+    return IBAMR_OK;
+}
+
+int ibamr_plan_bind_data(ibamr_plan* P, int slot, void* const* arrays)
+{
+    if (!P || !arrays) return fail(IBAMR_ERR_INVALID, "null argument");
+    if (slot < 0 || slot >= MAX_SLOTS) return fail(IBAMR_ERR_INVALID, "slot out of range");
+    const size_t n = (size_t)P->npatches * P->narr;
+    for (size_t i = 0; i < n; ++i)
+        if (!arrays[i]) return fail(IBAMR_ERR_INVALID, "null array pointer");
+    if (!P->arrays_d[slot]) HIP_OK(hipMalloc(&P->arrays_d[slot], sizeof(void*) * n));
+    HIP_OK(hipMemcpy(P->arrays_d[slot], arrays, sizeof(void*) * n, hipMemcpyHostToDevice));
+    return IBAMR_OK;
+}
+
+int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stream)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    if (npoints < 0) return fail(IBAMR_ERR_INVALID, "npoints < 0");
+    if (npoints > P->max_points) return fail(IBAMR_ERR_CAPACITY, "more points than the plan capacity");
+    if (npoints > 0 && !X) return fail(IBAMR_ERR_INVALID, "null X");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    P->npoints = npoints;
+    if (npoints == 0)
+    {
+        HIP_OK(hipMemsetAsync(P->bin_start_d, 0, sizeof(int) * (P->nbins + 2), s));
+        return IBAMR_OK;
+    }
+    BinArgs a{};
+    a.X = X;
+    a.n = npoints;
+    for (int d = 0; d < 3; ++d)
+    {
+        a.xlo[d] = P->dom_xlo[d];
+        a.xhi[d] = P->dom_xhi[d];
+        a.dx[d] = P->dx[d];
+        a.dom_lo[d] = P->dom_lo[d];
+        a.dom_hi[d] = P->dom_hi[d];
+        a.nb[d] = P->nb[d];
+    }
+    a.bin = P->bin;
+    a.nbins = P->nbins;
+    a.keys = P->keys_d;
+    a.idx = P->idx_d;
+    a.cell = P->cell_d;
+    const int B = 256;
+    const int G = (npoints + B - 1) / B;
+    if (P->ndim == 3)
+        hipLaunchKernelGGL(k_bin_keys<3>, dim3(G), dim3(B), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_bin_keys<2>, dim3(G), dim3(B), 0, s, a);
+    HIP_OK(hipGetLastError());
+    size_t tb = P->sort_tmp_bytes;
+    HIP_OK(hipcub::DeviceRadixSort::SortPairs(P->sort_tmp_d, tb, reinterpret_cast<unsigned*>(P->keys_d),
+                                              reinterpret_cast<unsigned*>(P->keys_sorted_d), P->idx_d, P->perm_d,
+                                              npoints, 0, P->sort_bits, s));
+    hipLaunchKernelGGL(k_bin_bounds, dim3((npoints + 1 + B - 1) / B), dim3(B), 0, s, P->keys_sorted_d, npoints,
+                       P->nbins, P->bin_start_d);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d);
+    HIP_OK(hipGetLastError());
+    return IBAMR_OK;
+}
+
+int ibamr_plan_check(ibamr_plan* P, void* stream)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int first_sentinel = 0;
+    HIP_OK(hipMemcpyAsync(&first_sentinel, P->bin_start_d + P->nbins, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (first_sentinel != P->npoints)
+        return fail(IBAMR_ERR_OUT_OF_DOMAIN, std::to_string(P->npoints - first_sentinel) +
+                                                 " point(s) outside the level domain (or NaN positions)");
+    return IBAMR_OK;
+}
+
+int ibamr_plan_bin(ibamr_plan* P, const double* X, int npoints, void* stream)
+{
+    const int rc = ibamr_plan_bin_async(P, X, npoints, stream);
+    if (rc != IBAMR_OK) return rc;
+    return ibamr_plan_check(P, stream);
+}
+
+int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, int mode, void* stream)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    if (slot < 0 || slot >= MAX_SLOTS || !P->arrays_d[slot]) return fail(IBAMR_ERR_INVALID, "slot not bound");
+    if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
+    if (mode != IBAMR_SPREAD_ACCUMULATE && mode != IBAMR_SPREAD_OVERWRITE) return fail(IBAMR_ERR_INVALID, "bad mode");
+    if (P->npoints > 0 && (!F || !X)) return fail(IBAMR_ERR_INVALID, "null F or X");
+    if (P->ntiles == 0) return IBAMR_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return dispatch(P->ndim, P->kernel, P->precision, [&](auto nd, auto ker, auto prec) -> int {
+        using T = decltype(prec);
+        constexpr int NDIM = decltype(nd)::value;
+        constexpr int KER = decltype(ker)::value;
+        constexpr int TILE = NDIM == 3 ? 8 : 16;
+        SpreadArgs<T> a{};
+        fill_spread_args<T>(P, slot, F, X, mode, a);
+        hipLaunchKernelGGL((k_spread_tiles<NDIM, KER, T, TILE>), dim3(P->ntiles), dim3(WAVE), 0, s, a);
+        HIP_OK(hipGetLastError());
+        return IBAMR_OK;
+    });
+}
+
+int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* stream)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    if (slot < 0 || slot >= MAX_SLOTS || !P->arrays_d[slot]) return fail(IBAMR_ERR_INVALID, "slot not bound");
+    if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
+    if (P->npoints == 0) return IBAMR_OK;
+    if (!U || !X) return fail(IBAMR_ERR_INVALID, "null U or X");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return dispatch(P->ndim, P->kernel, P->precision, [&](auto nd, auto ker, auto prec) -> int {
+        using T = decltype(prec);
+        constexpr int NDIM = decltype(nd)::value;
+        constexpr int KER = decltype(ker)::value;
+        InterpArgs<T> a{};
+        a.patches = P->patches_d;
+        a.npatches = P->npatches;
+        a.bin_patch_start = P->bin_patch_start_d;
+        a.bin_patches = P->bin_patches_d;
+        a.arrays = reinterpret_cast<T* const*>(P->arrays_d[slot]);
+        a.X = X;
+        a.U = static_cast<T*>(U);
+        a.perm = P->perm_d;
+        a.cells = P->cell_sorted_d;
+        a.keys_sorted = P->keys_sorted_d;
+        a.n = P->npoints;
+        for (int d = 0; d < 3; ++d)
+        {
+            a.dx[d] = P->dx[d];
+            a.gcw[d] = P->data_gcw[d];
+        }
+        a.narr = P->narr;
+        a.fdepth = P->fdepth;
+        a.depth = P->depth;
+        a.side = P->centering == IBAMR_SIDE;
+        const int B = 256;
+        hipLaunchKernelGGL((k_interp<NDIM, KER, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, a);
+        HIP_OK(hipGetLastError());
+        return IBAMR_OK;
+    });
+}
+
+int ibamr_plan_fill_periodic_ghosts(ibamr_plan* P, int slot, void* stream)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    if (slot < 0 || slot >= MAX_SLOTS || !P->arrays_d[slot]) return fail(IBAMR_ERR_INVALID, "slot not bound");
+    if (P->npatches != 1) return fail(IBAMR_ERR_INVALID, "periodic ghost fill needs a single-patch level");
+    const PatchDev& pd = P->patches_h[0];
+    for (int d = 0; d < P->ndim; ++d)
+    {
+        if (P->pshift[d] != pd.ihi[d] - pd.ilo[d] + 1 || pd.ilo[d] != P->dom_lo[d])
+            return fail(IBAMR_ERR_INVALID, "periodic ghost fill needs a fully periodic domain covered by the patch");
+    }
+    std::vector<void*> ptrs((size_t)P->narr);
+    HIP_OK(hipMemcpy(ptrs.data(), P->arrays_d[slot], sizeof(void*) * P->narr, hipMemcpyDeviceToHost));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int ncomps = P->centering == IBAMR_SIDE ? 1 : P->depth;
+    for (int a = 0; a < P->narr; ++a)
+    {
+        const int axis = P->centering == IBAMR_SIDE ? a : -1;
+        int n[3] = {1, 1, 1}, g[3] = {0, 0, 0}, N[3] = {1, 1, 1};
+        for (int d = 0; d < P->ndim; ++d)
+        {
+            n[d] = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
+            g[d] = P->data_gcw[d];
+            N[d] = pd.ihi[d] - pd.ilo[d] + 1;
+        }
+        const long total = (long)n[0] * n[1] * n[2];
+        for (int c = 0; c < ncomps; ++c)
+        {
+            const int B = 256;
+            const long G = (total + B - 1) / B;
+            if (P->precision == IBAMR_FP64)
+                hipLaunchKernelGGL(k_fill_periodic<double>, dim3((unsigned)G), dim3(B), 0, s,
+                                   static_cast<double*>(ptrs[a]) + c * total, n[0], n[1], n[2], g[0], g[1], g[2],
+                                   N[0], N[1], N[2], total);
+            else
+                hipLaunchKernelGGL(k_fill_periodic<float>, dim3((unsigned)G), dim3(B), 0, s,
+                                   static_cast<float*>(ptrs[a]) + c * total, n[0], n[1], n[2], g[0], g[1], g[2],
+                                   N[0], N[1], N[2], total);
+            HIP_OK(hipGetLastError());
+        }
+    }
+    return IBAMR_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,156 @@
+/*
+ * ibamr_amd.h -- C ABI of the MI355X (gfx950) Eulerian-Lagrangian coupling
+ * library: regularized-delta spreading and interpolation (IB_4, IB_6,
+ * BSPLINE_3) on cell- and side-centred patch data.
+ *
+ * All pointers passed to the plan entry points are device pointers; the
+ * caller owns them.  Plain C types only: streams are passed as void*
+ * (a hipStream_t, NULL = default stream).  Every entry point returns an
+ * ibamr_status; ibamr_last_error() describes the last failure of the calling
+ * thread.  No exceptions cross this boundary.
+ *
+ * Data layouts follow the reference exactly, so SAMRAI/PETSc arrays can be
+ * handed over without repacking:
+ *   - Eulerian data: SAMRAI ArrayData over the ghost box, column-major
+ *     (x fastest), depth slowest.  Side data: one array per axis over
+ *     toSideBox(ghost box, axis) (upper index + 1 along the axis).
+ *   - Lagrangian data: LData local form, AoS: X[n][ndim] (always double),
+ *     F/U[n][depth] (side data: depth = ndim).
+ *   - FP64 plans use double grids and F/U; FP32C plans use float grids and
+ *     F/U with compensated accumulation; positions and stencil selection stay
+ *     FP64 in both.
+ */
+#ifndef IBAMR_AMD_H
+#define IBAMR_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ibamr_status
+{
+    IBAMR_OK = 0,
+    IBAMR_ERR_BAD_KERNEL = -1,      /* unknown kernel name (LEInteractor.cpp:1923-2016 TBOX_ERROR) */
+    IBAMR_ERR_GHOST_WIDTH = -2,     /* ghost width below getMinimumGhostWidth (LEInteractor.cpp:4491-4498) */
+    IBAMR_ERR_DEPTH = -3,           /* side data needs Q_depth == NDIM, q depth 1 (LEInteractor.cpp:2197-2201) */
+    IBAMR_ERR_INVALID = -4,         /* invalid descriptor or argument */
+    IBAMR_ERR_OUT_OF_DOMAIN = -5,   /* a point's cell lies outside the level domain */
+    IBAMR_ERR_CAPACITY = -6,        /* more points than the plan was created for */
+    IBAMR_ERR_DEVICE = -7,          /* HIP runtime error */
+    IBAMR_ERR_NOT_BINNED = -8       /* spread/interp before ibamr_plan_bin */
+} ibamr_status;
+
+enum
+{
+    IBAMR_CELL = 0,
+    IBAMR_SIDE = 1
+};
+enum
+{
+    IBAMR_FP64 = 0,  /* plain FP64, results identical to the reference arithmetic */
+    IBAMR_FP32C = 1  /* FP32 grid/force data, FP32 weights, Kahan-compensated sums */
+};
+enum
+{
+    IBAMR_SPREAD_ACCUMULATE = 0, /* u += S F (LEInteractor::spread) */
+    IBAMR_SPREAD_OVERWRITE = 1   /* u = S F over the whole ghost box (zero + spread) */
+};
+
+/* Kernel registry, LEInteractor::isKnownKernel / getStencilSize /
+ * getMinimumGhostWidth (LEInteractor.cpp:2035-2114).  Names: "IB_4", "IB_6",
+ * "BSPLINE_3" (spread/interp on the GPU) and "PIECEWISE_LINEAR" (registry only). */
+int ibamr_is_known_kernel(const char* kernel_fcn);
+int ibamr_get_stencil_size(const char* kernel_fcn);
+int ibamr_get_minimum_ghost_width(const char* kernel_fcn);
+
+/* One patch of a level: SAMRAI Box (cell indices, level index space) and
+ * CartesianPatchGeometry x_lower / x_upper. */
+typedef struct ibamr_patch_geom
+{
+    int ilower[3];
+    int iupper[3];
+    double x_lower[3];
+    double x_upper[3];
+} ibamr_patch_geom;
+
+/* A patch level (LDataManager::spread/interp loop over one level,
+ * LDataManager.cpp:552-667 and 699-813).  The domain box and x range feed the
+ * level form of IndexUtilities::getCellIndex (IndexUtilities-inl.h:226-243);
+ * periodic_shift is CartesianGridGeometry::getPeriodicShift(ratio) (0 = not
+ * periodic along that axis). */
+typedef struct ibamr_level_geom
+{
+    int ndim;
+    double dx[3];
+    int domain_lower[3];
+    int domain_upper[3];
+    double domain_x_lower[3];
+    double domain_x_upper[3];
+    int periodic_shift[3];
+    int npatches;
+    const ibamr_patch_geom* patches;
+} ibamr_level_geom;
+
+typedef struct ibamr_plan_config
+{
+    const char* kernel;   /* "IB_4" | "IB_6" | "BSPLINE_3" */
+    int centering;        /* IBAMR_CELL | IBAMR_SIDE */
+    int precision;        /* IBAMR_FP64 | IBAMR_FP32C */
+    int depth;            /* cell data depth; side data: must be 1 */
+    int data_gcw[3];      /* ghost cell width of the Eulerian arrays */
+    int idx_gcw;          /* Lagrangian index ghost width (LDataManager ghost width); 0 = kernel minimum */
+    int max_points;       /* capacity of ibamr_plan_bin */
+    int drift_cells;      /* how far (cells) points may move between bin and spread */
+    int tile;             /* spread tile edge in cells (0 = default: 8 in 3-D, 16 in 2-D) */
+} ibamr_plan_config;
+
+typedef struct ibamr_plan ibamr_plan;
+
+/* Creates a plan for one level: validates the kernel and ghost widths, builds
+ * the (patch, tile) work list and allocates device scratch for max_points. */
+int ibamr_plan_create(ibamr_plan** plan, const ibamr_level_geom* level, const ibamr_plan_config* config);
+int ibamr_plan_destroy(ibamr_plan* plan);
+
+/* Binds Eulerian arrays to a slot (0..7): arrays[p*narr + a], narr = ndim for
+ * side data (a = axis) and 1 for cell data.  Copies the pointer table to the
+ * device (synchronous; do it once per regrid, not per step). */
+int ibamr_plan_bind_data(ibamr_plan* plan, int slot, void* const* arrays);
+
+/* Point binning: cell of every point by IndexUtilities::getCellIndex (level
+ * form, bit-exact FP64), then a stable sort by spread bin.  This is the GPU
+ * counterpart of the LNodeSetData association made at regrid
+ * (LDataManager.cpp:1564-1651).  ibamr_plan_bin synchronizes the stream and
+ * reports points outside the domain; ibamr_plan_bin_async does not (use
+ * ibamr_plan_check afterwards), so it can be captured in a HIP graph. */
+int ibamr_plan_bin(ibamr_plan* plan, const double* X, int npoints, void* stream);
+int ibamr_plan_bin_async(ibamr_plan* plan, const double* X, int npoints, void* stream);
+int ibamr_plan_check(ibamr_plan* plan, void* stream);
+
+/* LDataManager::spread for one level (LDataManager.cpp:623-658): for every
+ * patch, LEInteractor::spread over the Lagrangian index ghost box, with
+ * periodic images, into the arrays bound to `slot`.  Deterministic: every
+ * grid value sums its contributions in one fixed order (bin, image, point),
+ * with no atomics, so results are bitwise reproducible.  F: [n][ndim] (side)
+ * or [n][depth] (cell); X: current positions. */
+int ibamr_plan_spread(ibamr_plan* plan, int slot, const void* F, const double* X, int mode, void* stream);
+
+/* LDataManager::interp for one level (LDataManager.cpp:748-802): each point
+ * is interpolated from the patch whose interior holds its cell; ghost values
+ * must be filled beforehand (see ibamr_plan_fill_periodic_ghosts).  U is
+ * overwritten for those points. */
+int ibamr_plan_interp(ibamr_plan* plan, int slot, void* U, const double* X, void* stream);
+
+/* Fills the ghost cells of the arrays bound to `slot` from the periodic
+ * interior for a single-patch level that spans a fully periodic domain (the
+ * RefineSchedule::fillData step before interp, LDataManager.cpp:742-745). */
+int ibamr_plan_fill_periodic_ghosts(ibamr_plan* plan, int slot, void* stream);
+
+/* Plan facts for reports: number of spread tiles, tile edge, bins. */
+int ibamr_plan_info(const ibamr_plan* plan, int* ntiles, int* tile, int* nbins);
+
+const char* ibamr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+FP64 plans must match the oracle bitwise:
+  * interp: every point, every axis/component (same per-point arithmetic);
+  * spread: every grid value of every ghosted array, starting from the same
+    non-zero field, against the oracle run over the patch's entry list
+    (points + periodic images, LIndexSetData.cpp:55-150) in the canonical
+    (bin, image, point) order the GPU uses.
+FP32C plans: relative L-infinity error against the FP64 oracle <= 1e-6 (the
+bar of the precision form).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+import ibamr_amd as ia  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+TOL32 = 1e-6
+
+
+def _points(ndim, n, npts, seed, crowd=True):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(0.0, 1.0, (npts, ndim))
+    if crowd:
+        h = 1.0 / n
+        q = npts // 4
+        X[:q] = rng.uniform(0.0, 2.5 * h, (q, ndim))              # near the periodic corner
+        X[q:2 * q] = 1.0 - rng.uniform(1e-12, 2.5 * h, (q, ndim))
+        X[2 * q:2 * q + 8] = np.array([[0.5] * ndim]) + (np.arange(8)[:, None] % 2) * h * 0.5  # cell faces/centres
+    return X
+
+
+def _periodic_fill_np(u, gw, n):
+    out = u.copy()
+    for ax in range(out.ndim - 1, -1, -1):
+        if out.shape[ax] < n:  # depth axis
+            continue
+        idx = (np.arange(out.shape[ax]) - gw) % n + gw
+        out = np.take(out, idx, axis=ax)
+    return out
+
+
+def _arrays(level, plan, rng, dtype=np.float64, fill="random"):
+    out = []
+    for p in range(len(level.patches)):
+        for a in range(plan.narr):
+            shp = ia.array_shape(level, p, plan.data_gcw, plan.centering, axis=a, depth=plan.depth)
+            v = rng.uniform(-1.0, 1.0, shp) if fill == "random" else np.zeros(shp)
+            out.append(np.ascontiguousarray(v.astype(dtype)))
+    return out
+
+
+def _tile(ndim):
+    return 8 if ndim == 3 else 16
+
+
+def _oracle_spread(orc, level, plan, arrays_np, X, F, kernel):
+    """LDataManager::spread over all patches of the level with the oracle."""
+    ndim = level.ndim
+    gw = plan.data_gcw
+    igcw = ia.get_minimum_ghost_width(kernel)
+    cells = orc.get_cell_index(X, level.domain_x_lower, level.domain_x_upper, level.dx, level.domain_lower,
+                               level.domain_upper)
+    out = [a.copy() for a in arrays_np]
+    for pi, P in enumerate(level.patches):
+        idx, sh = orc.build_patch_entries(cells, P.ilower, P.iupper, igcw, level.periodic_shift, level.domain_lower,
+                                          level.dx, [_tile(ndim)] * ndim)
+        if plan.centering == ia.SIDE:
+            q = out[pi * ndim:(pi + 1) * ndim]
+            orc.spread_side(kernel, level.dx, P.x_lower, P.ilower, P.iupper, gw, q, X, F, indices=idx, Xshift=sh)
+        else:
+            orc.spread(kernel, level.dx, P.x_lower, P.ilower, P.iupper, gw, out[pi], X, F, indices=idx, Xshift=sh,
+                       depth=plan.depth)
+    return out
+
+
+def _oracle_interp(orc, level, plan, arrays_np, X, kernel):
+    ndim = level.ndim
+    cells = orc.get_cell_index(X, level.domain_x_lower, level.domain_x_upper, level.dx, level.domain_lower,
+                               level.domain_upper)
+    fdepth = ndim if plan.centering == ia.SIDE else plan.depth
+    U = np.full((X.shape[0], fdepth), np.nan)
+    for pi, P in enumerate(level.patches):
+        idx, sh = orc.build_patch_entries(cells, P.ilower, P.iupper, 0, level.periodic_shift, level.domain_lower,
+                                          level.dx, [_tile(ndim)] * ndim, interior_only=True)
+        if len(idx) == 0:
+            continue
+        if plan.centering == ia.SIDE:
+            q = arrays_np[pi * ndim:(pi + 1) * ndim]
+            V = orc.interp_side(kernel, level.dx, P.x_lower, P.ilower, P.iupper, plan.data_gcw, q, X, indices=idx,
+                                Xshift=sh)
+        else:
+            V = orc.interp(kernel, level.dx, P.x_lower, P.ilower, P.iupper, plan.data_gcw, arrays_np[pi], X,
+                           indices=idx, Xshift=sh, depth=plan.depth)
+        U[idx] = V[idx]
+    return U
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+CASES = [
+    # ndim, n, kernel, centering, depth, npts
+    (3, 16, "IB_4", ia.SIDE, 1, 600),
+    (3, 16, "IB_6", ia.SIDE, 1, 600),
+    (3, 16, "BSPLINE_3", ia.SIDE, 1, 600),
+    (3, 12, "IB_6", ia.CELL, 2, 400),
+    (3, 8, "IB_4", ia.CELL, 1, 300),
+    (2, 32, "IB_4", ia.SIDE, 1, 500),
+    (2, 32, "IB_6", ia.SIDE, 1, 500),
+    (2, 24, "BSPLINE_3", ia.CELL, 3, 400),
+]
+
+
+def _ids(c):
+    return "%dd-n%d-%s-%s-d%d" % (c[0], c[1], c[2], "side" if c[3] == ia.SIDE else "cell", c[4])
+
+
+@pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
+def test_fp64_periodic_bitwise(orc, case):
+    ndim, n, kernel, centering, depth, npts = case
+    print("case", _ids(case), flush=True)
+    rng = np.random.default_rng(hash(_ids(case)) % 2**32)
+    level = ia.Level.uniform_periodic(n, ndim)
+    plan = ia.LevelPlan(level, kernel, centering, ia.FP64, depth=depth, max_points=npts)
+    X = _points(ndim, n, npts, 7)
+    fdepth = ndim if centering == ia.SIDE else depth
+    F = rng.standard_normal((npts, fdepth))
+    Xd, Fd = _dev(X), _dev(F)
+    plan.bin(Xd)
+    torch.cuda.synchronize()
+
+    # --- spread (accumulate into a non-zero field)
+    init = _arrays(level, plan, rng)
+    dev_arrays = [_dev(a) for a in init]
+    plan.bind(0, dev_arrays)
+    plan.spread(0, Fd, Xd, mode=ia.ACCUMULATE)
+    torch.cuda.synchronize()
+    ref = _oracle_spread(orc, level, plan, init, X, F, kernel)
+    for got, want in zip(dev_arrays, ref):
+        g = got.cpu().numpy()
+        assert np.array_equal(g, want), np.max(np.abs(g - want))
+
+    # --- spread twice from the same field: bitwise reproducible
+    again = [_dev(a) for a in init]
+    plan.bind(1, again)
+    plan.spread(1, Fd, Xd, mode=ia.ACCUMULATE)
+    torch.cuda.synchronize()
+    for a, b in zip(dev_arrays, again):
+        assert torch.equal(a, b)
+
+    # --- overwrite mode = zero field + spread
+    ow = [_dev(a) for a in init]
+    plan.bind(2, ow)
+    plan.spread(2, Fd, Xd, mode=ia.OVERWRITE)
+    zero = [np.zeros_like(a) for a in init]
+    ref0 = _oracle_spread(orc, level, plan, zero, X, F, kernel)
+    torch.cuda.synchronize()
+    for got, want in zip(ow, ref0):
+        assert np.array_equal(got.cpu().numpy(), want)
+
+    # --- periodic ghost fill on the GPU, then interp
+    u_np = []
+    for a in init:
+        gw = plan.data_gcw[0]
+        u_np.append(_periodic_fill_np(a, gw, n))
+    u_dev = [_dev(a) for a in init]
+    plan.bind(3, u_dev)
+    plan.fill_periodic_ghosts(3)
+    torch.cuda.synchronize()
+    for got, want in zip(u_dev, u_np):
+        assert np.array_equal(got.cpu().numpy(), want)
+    U = torch.full((npts, fdepth), float("nan"), dtype=torch.float64, device=DEV)
+    plan.interp(3, U, Xd)
+    torch.cuda.synchronize()
+    Uref = _oracle_interp(orc, level, plan, u_np, X, kernel)
+    assert np.array_equal(U.cpu().numpy(), Uref), np.nanmax(np.abs(U.cpu().numpy() - Uref))
+
+
+@pytest.mark.parametrize("case", CASES[:3] + CASES[5:7], ids=[_ids(c) for c in CASES[:3] + CASES[5:7]])
+def test_fp32c_error(orc, case):
+    ndim, n, kernel, centering, depth, npts = case
+    print("case", _ids(case), flush=True)
+    rng = np.random.default_rng(11)
+    level = ia.Level.uniform_periodic(n, ndim)
+    plan = ia.LevelPlan(level, kernel, centering, ia.FP32C, depth=depth, max_points=npts)
+    X = _points(ndim, n, npts, 5)
+    fdepth = ndim
+    F = rng.standard_normal((npts, fdepth))
+    Xd, Fd = _dev(X), _dev(F.astype(np.float32))
+    plan.bin(Xd)
+    zero = _arrays(level, plan, rng, fill="zero")
+    f_dev = [_dev(a.astype(np.float32)) for a in zero]
+    plan.bind(0, f_dev)
+    plan.spread(0, Fd, Xd, mode=ia.OVERWRITE)
+    ref = _oracle_spread(orc, level, plan, zero, X, F.astype(np.float32).astype(np.float64), kernel)
+    torch.cuda.synchronize()
+    for got, want in zip(f_dev, ref):
+        err = np.max(np.abs(got.cpu().numpy().astype(np.float64) - want)) / np.max(np.abs(want))
+        assert err <= TOL32, err
+    u = [_periodic_fill_np(a, plan.data_gcw[0], n).astype(np.float32) for a in _arrays(level, plan, rng)]
+    u_dev = [_dev(a) for a in u]
+    plan.bind(1, u_dev)
+    U = torch.zeros((npts, fdepth), dtype=torch.float32, device=DEV)
+    plan.interp(1, U, Xd)
+    Uref = _oracle_interp(orc, level, plan, [a.astype(np.float64) for a in u], X, kernel)
+    torch.cuda.synchronize()
+    err = np.max(np.abs(U.cpu().numpy().astype(np.float64) - Uref)) / np.max(np.abs(Uref))
+    assert err <= TOL32, err
+
+
+def test_multi_patch_level_fp64(orc):
+    """A non-periodic level of three patches (one of them touching a neighbour):
+    spread over each patch's index ghost box (points of neighbouring patches
+    included, LDataManager.cpp:623-658), interp from the owning patch."""
+    ndim, n, kernel = 3, 32, "IB_4"
+    h = 1.0 / n
+    boxes = [([4, 4, 4], [11, 19, 11]), ([12, 4, 4], [27, 11, 13]), ([16, 20, 18], [23, 27, 29])]
+    patches = [ia.Patch(lo, hi, [l * h for l in lo], [(u + 1) * h for u in hi]) for lo, hi in boxes]
+    level = ia.Level([h] * 3, [0] * 3, [n - 1] * 3, [0.0] * 3, [1.0] * 3, [0, 0, 0], patches)
+    rng = np.random.default_rng(3)
+    pts = []
+    for lo, hi in boxes:
+        m = 150
+        pts.append(np.stack([rng.uniform(lo[d] * h, (hi[d] + 1) * h, m) for d in range(3)], axis=1))
+    X = np.concatenate(pts)
+    npts = X.shape[0]
+    plan = ia.LevelPlan(level, kernel, ia.SIDE, ia.FP64, max_points=npts)
+    F = rng.standard_normal((npts, 3))
+    Xd, Fd = _dev(X), _dev(F)
+    plan.bin(Xd)
+    init = _arrays(level, plan, rng)
+    dv = [_dev(a) for a in init]
+    plan.bind(0, dv)
+    plan.spread(0, Fd, Xd, mode=ia.ACCUMULATE)
+    ref = _oracle_spread(orc, level, plan, init, X, F, kernel)
+    torch.cuda.synchronize()
+    for got, want in zip(dv, ref):
+        assert np.array_equal(got.cpu().numpy(), want)
+    uv = [_dev(a) for a in init]
+    plan.bind(1, uv)
+    U = torch.full((npts, 3), float("nan"), dtype=torch.float64, device=DEV)
+    plan.interp(1, U, Xd)
+    Uref = _oracle_interp(orc, level, plan, init, X, kernel)
+    torch.cuda.synchronize()
+    assert np.array_equal(U.cpu().numpy(), Uref)
+
+
+def test_out_of_domain_reported(orc):
+    level = ia.Level.uniform_periodic(8, 3)
+    plan = ia.LevelPlan(level, "IB_4", ia.SIDE, ia.FP64, max_points=4)
+    X = _dev(np.array([[0.5, 0.5, 0.5], [1.5, 0.5, 0.5], [0.1, np.nan, 0.2], [0.2, 0.2, 0.2]]))
+    with pytest.raises(ia.IBAMRError) as e:
+        plan.bin(X)
+    assert e.value.code == ia.IBAMR_ERR_OUT_OF_DOMAIN
