@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Throughput of one spread + one interpolate of Lagrangian points on one MI355X.
+
+Workload (default): 3-D periodic unit cube, 512^3 cells, side-centred (MAC)
+grid, IB_6 kernel, 8,388,608 points: half uniform in the domain, half in a
+spherical shell of radius 0.3 and thickness 2h about the centre.  Grid and
+force/velocity data FP32 with compensated sums (positions and stencil
+selection FP64).  One step = LDataManager::spread into a zeroed field (one
+launch) + LDataManager::interp from a ghost-filled field (one launch), points
+binned beforehand (binning belongs to regrid in the reference; its time is
+reported separately).  The plain FP64 form runs the same workload for the
+FP32/FP64 ratio and for the error of the FP32 form.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+
+Prints one JSON line.  The CPU baseline is the C oracle (the reference's FP64
+loops restated) timed on a bounded sample of the same points on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Lagrangian points spread+interpolated/s (FP32-batched vs FP64) @<=1e-6 err; HBM GB/s"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--points", type=int, default=1 << 23)
+    ap.add_argument("--kernel", default="IB_6")
+    ap.add_argument("--cpu-sample", type=int, default=400000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def make_points(torch, n, N, seed):
+    """Half uniform in [0,1)^3, half in the shell |x - c| in [0.3 - h, 0.3 + h)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    h = 1.0 / N
+    nu = n // 2
+    ns = n - nu
+    Xu = torch.rand((nu, 3), generator=g, device="cuda", dtype=torch.float64)
+    d = torch.randn((ns, 3), generator=g, device="cuda", dtype=torch.float64)
+    d = d / d.norm(dim=1, keepdim=True)
+    r = 0.3 - h + 2 * h * torch.rand((ns, 1), generator=g, device="cuda", dtype=torch.float64)
+    Xs = 0.5 + r * d
+    X = torch.cat([Xu, Xs]).contiguous()
+    X = torch.remainder(X, 1.0)
+    F = torch.randn((n, 3), generator=g, device="cuda", dtype=torch.float64)
+    return X, F
+
+
+def touched_faces(torch, X, N, kernel):
+    """T: distinct (axis, face) grid values in the union of all point stencils
+    (periodic DOFs), counted exactly on the GPU."""
+    W = {"IB_4": 4, "IB_6": 6, "BSPLINE_3": 3}[kernel]
+    h = 1.0 / N
+    total = 0
+    for a in range(3):
+        starts = []
+        for d in range(3):
+            xl = -0.5 * h if d == a else 0.0
+            t = (X[:, d] - xl) / h
+            if kernel == "BSPLINE_3":
+                s = torch.floor(t) - 1
+            else:
+                s = torch.floor(t + 0.5) - (W // 2)   # NINT for t >= 0
+            starts.append(s.to(torch.int64))
+        mask = torch.zeros(N * N * N, dtype=torch.bool, device="cuda")
+        for k2 in range(W):
+            i2 = torch.remainder(starts[2] + k2, N)
+            for k1 in range(W):
+                i1 = torch.remainder(starts[1] + k1, N)
+                base = (i2 * N + i1) * N
+                for k0 in range(W):
+                    mask[base + torch.remainder(starts[0] + k0, N)] = True
+        total += int(mask.sum().item())
+        del mask
+    return total
+
+
+def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
+    dt = torch.float64 if precision == ia.FP64 else torch.float32
+    level = ia.Level.uniform_periodic(N, 3)
+    n = X.shape[0]
+    plan = ia.LevelPlan(level, kernel, ia.SIDE, precision, max_points=n)
+    g = plan.data_gcw
+    f = [torch.zeros(ia.array_shape(level, 0, g, ia.SIDE, axis=a), dtype=dt, device="cuda") for a in range(3)]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1234)
+    u = [(2 * torch.rand(ia.array_shape(level, 0, g, ia.SIDE, axis=a), generator=gen, device="cuda",
+                         dtype=torch.float64) - 1).to(dt) for a in range(3)]
+    plan.bind(0, f)
+    plan.bind(1, u)
+    s = torch.cuda.current_stream()
+    plan.fill_periodic_ghosts(1, stream=s)
+    Fd = F.to(dt).contiguous()
+    U = torch.zeros((n, 3), dtype=dt, device="cuda")
+    eb0, eb1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    eb0.record(s)
+    plan.bin(X, stream=s, check=False)
+    eb1.record(s)
+    plan.check(stream=s)
+    torch.cuda.synchronize()
+    bin_ms = eb0.elapsed_time(eb1)
+
+    def step(ev=None):
+        if ev:
+            ev[0].record(s)
+        plan.spread(0, Fd, X, mode=ia.OVERWRITE, stream=s)
+        if ev:
+            ev[1].record(s)
+        plan.interp(1, U, X, stream=s)
+        if ev:
+            ev[2].record(s)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sp = [e[0].elapsed_time(e[1]) for e in evs]
+    it = [e[1].elapsed_time(e[2]) for e in evs]
+    return {"plan": plan, "f": f, "u": u, "U": U, "wall_s": wall, "spread_ms": sum(sp) / len(sp),
+            "interp_ms": sum(it) / len(it), "bin_ms": bin_ms, "info": plan.info()}
+
+
+def cpu_baseline(X_np, F_np, N, kernel, sample, seed=5):
+    """The oracle's FP64 spread + interp (the reference CPU loops, one core) on a
+    random sample of the workload's points, on the full grid."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    rng = np.random.default_rng(seed)
+    sel = rng.choice(X_np.shape[0], size=min(sample, X_np.shape[0]), replace=False)
+    X = np.ascontiguousarray(X_np[sel])
+    F = np.ascontiguousarray(F_np[sel])
+    h = 1.0 / N
+    g = O.min_ghost_width(kernel)
+    il, iu = [0] * 3, [N - 1] * 3
+    f = [np.empty(O.side_axis_shape(il, iu, [g] * 3, a)) for a in range(3)]
+    for a in f:
+        a.fill(0.0)
+    cell = O.get_cell_index(X, [0.0] * 3, [1.0] * 3, [h] * 3, il, iu)
+    idx, sh = O.build_patch_entries(cell, il, iu, g, [N] * 3, il, [h] * 3, [8] * 3)
+    iidx, ish = O.build_patch_entries(cell, il, iu, 0, [N] * 3, il, [h] * 3, [8] * 3, interior_only=True)
+    t0 = time.perf_counter()
+    O.spread_side(kernel, [h] * 3, [0.0] * 3, il, iu, [g] * 3, f, X, F, indices=idx, Xshift=sh)
+    O.interp_side(kernel, [h] * 3, [0.0] * 3, il, iu, [g] * 3, f, X, indices=iidx, Xshift=ish)
+    t = time.perf_counter() - t0
+    return {"value": X.shape[0] / t, "unit": "points/s", "cores": 1, "kind": "port",
+            "sample": "%d of the %d workload points (uniform random subset), full %d^3 side grid, FP64 spread+interp, %.1f s"
+                      % (X.shape[0], X_np.shape[0], N, t)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import ibamr_amd as ia
+
+    ia.lib()
+    N, n, kernel = args.grid, args.points, args.kernel
+    X, F64 = make_points(torch, n, N, seed=2026)
+    print("bench: %d^3 %s side, %d points; FP32C form" % (N, kernel, n), file=sys.stderr, flush=True)
+    r32 = run_form(torch, ia, ia.FP32C, X, F64, N, kernel, args.steps, args.warmup)
+    ms32 = r32["spread_ms"] + r32["interp_ms"]
+    T = touched_faces(torch, X, N, kernel)
+    d, sx, s32 = 3, 8, 4
+    B_spread = n * (sx * d + s32 * d) + s32 * T
+    B_interp = n * (sx * d + s32 * d) + s32 * T
+    dom = "spread" if r32["spread_ms"] >= r32["interp_ms"] else "interp"
+    B_dom = B_spread if dom == "spread" else B_interp
+    t_dom = r32["spread_ms"] if dom == "spread" else r32["interp_ms"]
+    achieved = B_dom / (t_dom * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": n / (ms32 * 1e-3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "3-D periodic %d^3 side-centred %s, %d points (half uniform, half in a 2h shell "
+                               "r=0.3), FP32 compensated spread+interp" % (N, kernel, n),
+                   "grid": N, "points": n, "kernel": kernel, "centering": "side", "precision": "fp32c",
+                   "tile": r32["info"]["tile"], "spread_tiles": r32["info"]["ntiles"]},
+        "roofline": {"bound": "hbm", "kernel": "k_spread_tiles" if dom == "spread" else "k_interp",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": B_dom},
+        "detail": {"spread_ms": r32["spread_ms"], "interp_ms": r32["interp_ms"], "bin_ms": r32["bin_ms"],
+                   "touched_faces": T, "bytes_step": B_spread + B_interp,
+                   "step_GBs": (B_spread + B_interp) / (ms32 * 1e-3) / 1e9},
+    }
+    if not args.no_fp64:
+        print("bench: FP64 form", file=sys.stderr, flush=True)
+        r64 = run_form(torch, ia, ia.FP64, X, F64, N, kernel, max(2, args.steps // 2), 1)
+        ms64 = r64["spread_ms"] + r64["interp_ms"]
+        err_f = max(float((r32["f"][a].double() - r64["f"][a]).abs().max() / r64["f"][a].abs().max()) for a in range(3))
+        err_u = float((r32["U"].double() - r64["U"]).abs().max() / r64["U"].abs().max())
+        out["detail"].update({"fp64_points_per_s": n / (ms64 * 1e-3), "fp64_spread_ms": r64["spread_ms"],
+                              "fp64_interp_ms": r64["interp_ms"], "fp32c_over_fp64": ms64 / ms32,
+                              "fp32c_rel_linf_err_spread": err_f, "fp32c_rel_linf_err_interp": err_u})
+        del r64
+    if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(X.cpu().numpy(), F64.cpu().numpy(), N, kernel, args.cpu_sample)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

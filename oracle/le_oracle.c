@@ -361,6 +361,13 @@ void orc_get_cell_index(int ndim, const double* X, const double* x_lower, const 
     }
 }
 
+void orc_get_cell_indices(int ndim, const double* X, int n, const double* x_lower, const double* x_upper,
+                          const double* dx, const int* ilower, const int* iupper, int* idx)
+{
+    for (int k = 0; k < n; ++k)
+        orc_get_cell_index(ndim, &X[(long)ndim * k], x_lower, x_upper, dx, ilower, iupper, &idx[(long)ndim * k]);
+}
+
 int orc_build_local_indices(int ndim, const int* box_lo, const int* box_hi, const double* x_lower,
                             const double* x_upper, const double* dx, const int* ilower, const int* iupper,
                             const double* X, int npoints, int* indices)

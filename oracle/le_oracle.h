@@ -73,6 +73,10 @@ void orc_spread(int kernel, int ndim, const double* dx, const double* x_lower, i
 void orc_get_cell_index(int ndim, const double* X, const double* x_lower, const double* x_upper,
                         const double* dx, const int* ilower, const int* iupper, int* idx);
 
+/* Batch form of orc_get_cell_index over n points (X is [n][ndim], idx [n][ndim]). */
+void orc_get_cell_indices(int ndim, const double* X, int n, const double* x_lower, const double* x_upper,
+                          const double* dx, const int* ilower, const int* iupper, int* idx);
+
 /* Raw-array index list (LEInteractor.cpp:6088-6126): points whose cell lies in
  * box [box_lo, box_hi]; returns the count, fills indices in point order. */
 int orc_build_local_indices(int ndim, const int* box_lo, const int* box_hi, const double* x_lower,

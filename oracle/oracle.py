@@ -45,6 +45,7 @@ def lib():
         L.orc_spread.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int,
                                  _dp, _dp, _ip, _ip, _ip, _dp]
         L.orc_get_cell_index.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _ip]
+        L.orc_get_cell_indices.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _ip, _ip, _ip]
         L.orc_build_local_indices.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _ip, _ip, _dp,
                                               ctypes.c_int, _ip]
         L.orc_interp_side.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip, _ip,
@@ -153,17 +154,11 @@ def spread(kernel, dx, x_lower, ilower, iupper, nugc, u, X, V, indices=None, Xsh
 
 
 def get_cell_index(X, x_lower, x_upper, dx, ilower, iupper):
-    X = _dv(X)
     ndim = len(ilower)
-    X2 = X.reshape(-1, ndim)
+    X2 = _dv(X).reshape(-1, ndim)
     out = np.zeros((X2.shape[0], ndim), dtype=np.int32)
-    L = lib()
-    xl, xu, h, il, iu = _dv(x_lower), _dv(x_upper), _dv(dx), _iv(ilower), _iv(iupper)
-    for k in range(X2.shape[0]):
-        row = np.ascontiguousarray(X2[k])
-        o = np.zeros(ndim, dtype=np.int32)
-        L.orc_get_cell_index(ndim, _d(row), _d(xl), _d(xu), _d(h), _i(il), _i(iu), _i(o))
-        out[k] = o
+    lib().orc_get_cell_indices(ndim, _d(X2), X2.shape[0], _d(_dv(x_lower)), _d(_dv(x_upper)), _d(_dv(dx)),
+                               _i(_iv(ilower)), _i(_iv(iupper)), _i(out))
     return out
 
 
