@@ -1,0 +1,611 @@
+// le_kernels3d.h -- 3-D spread and interpolation kernels for gfx950.
+// Included by le_plan.hip (shares SpreadArgs/InterpArgs/PatchDev/TileDesc).
+//
+// Spread (k_spread3): owner-computes, register-resident.  One wavefront owns a
+// column tile of one array: 8 x 8 (x, y) grid lines, one per lane, times
+// S3_ZC consecutive z values held in registers.  Points that can reach the
+// tile are staged 64 at a time: each lane computes one point's weights (FP64
+// index selection, weights in T) and writes the x and y weights, aligned to
+// the tile's lanes and zero-padded, plus the z weights to LDS.  The staged
+// points are then applied grouped by their first z index: a ballot per z start
+// yields the points with that start, so every register index is a compile-time
+// constant (no dynamic VGPR indexing).  Per point and lane the work is two LDS
+// reads, one product and W fused multiply-adds.  No atomics: each grid value
+// lives in one lane's register and receives its contributions in a fixed
+// order, so results are bitwise reproducible.
+//   FP32C: contributions go to a per-chunk FP32 partial, folded into a
+//          Kahan-compensated total after every chunk.
+//   FP64:  the reference operation order per contribution
+//          (IB: (w0*(w1*(w2/vol)))*F, BSPLINE: (((w0*w1)*w2)*F)/vol) and
+//          sequential adds, over chunks cut per bin, so the result equals the
+//          reference loop run over the oracle's z-bucketed entry order.
+//
+// Interpolation (k_interp3): one 256-thread workgroup per superbin (2x2x2
+// bins = 16^3 cells).  For each array it finds the exact stencil extent of its
+// points, stages that block of the ghosted array into LDS with coalesced
+// loads, then each thread sums the W^3 stencil of its points from LDS.
+#pragma once
+
+namespace
+{
+constexpr int S3_ZC = 32;    // z values per spread tile (registers per lane)
+constexpr int S3_PAD = 25;   // staged weight row: W-1 zeros, 8 lanes, W-1 zeros (odd stride)
+constexpr int S3_MAXCOMB = 64;
+
+template <int KER, typename T, bool MERGE>
+__global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
+{
+    using KT = KernelTraits<KER>;
+    constexpr int W = KT::W;
+    constexpr bool FP64 = sizeof(T) == 8;
+    constexpr int ZC = S3_ZC;
+    __shared__ T s_rx[64 * S3_PAD];
+    __shared__ T s_ry[64 * S3_PAD];
+    __shared__ T s_wz[64 * 8];
+    __shared__ T s_F[64];
+    __shared__ int s_dx0[64];
+    __shared__ int s_dy0[64];
+    __shared__ int s_lb[3][16];
+    __shared__ int s_lo[3][16];
+    __shared__ int s_ln[3];
+    __shared__ int s_cs[S3_MAXCOMB];    // per combination: first sorted point
+    __shared__ int s_cl[S3_MAXCOMB];    //                  number of points
+    __shared__ int s_co[S3_MAXCOMB];    //                  image offset code
+    __shared__ int s_cp[S3_MAXCOMB + 1];// prefix of lengths
+
+    const int lane = threadIdx.x;
+    const int x = lane & 7, y = lane >> 3;
+    const TileDesc td = a.tiles[blockIdx.x];
+    const PatchDev P = a.patches[td.patch];
+    const int axis = a.side ? td.comp : -1;
+    const int tsz[3] = {8, 8, ZC};
+    int alo[3], ahi[3], an[3], flo[3], fhi[3];
+    double xl[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        alo[d] = P.ilo[d] - a.gcw[d];
+        ahi[d] = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0);
+        an[d] = ahi[d] - alo[d] + 1;
+        flo[d] = alo[d] + td.t[d] * tsz[d];
+        fhi[d] = min(flo[d] + tsz[d] - 1, ahi[d]);
+        xl[d] = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);  // LEInteractor.cpp:3689
+    }
+    T* __restrict__ arr = a.arrays[td.patch * a.narr + (a.side ? td.comp : 0)];
+    const size_t plane = (size_t)an[0] * an[1];
+    if (!a.side) arr += (size_t)td.comp * plane * an[2];
+    const int fcomp = td.comp;
+    const int gx = flo[0] + x, gy = flo[1] + y;
+    const bool col_in = gx <= fhi[0] && gy <= fhi[1];
+    const size_t col_off = (size_t)(gx - alo[0]) + (size_t)an[0] * (size_t)(gy - alo[1]);
+
+    // cells whose points can reach the tile, inside the patch's Lagrangian ghost box
+    int clo[3], chi[3];
+    bool any = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        clo[d] = max(flo[d] - a.reach, P.ilo[d] - a.idx_gcw);
+        chi[d] = min(fhi[d] + a.reach, P.ihi[d] + a.idx_gcw);
+        any = any && clo[d] <= chi[d];
+    }
+    // per-axis (bin, image offset) lists in increasing (bin, offset) order
+    if (lane < 3)
+    {
+        const int d = lane;
+        int n = 0;
+        if (any)
+        {
+            const int N = a.pshift[d];
+            const int omax = N > 0 ? 1 : 0;
+            for (int o = -omax; o <= omax; ++o)
+            {
+                const int pl = max(clo[d] - o * N, a.dom_lo[d]);
+                const int ph = min(chi[d] - o * N, a.dom_hi[d]);
+                if (pl > ph) continue;
+                const int bl = floor_div(pl - a.dom_lo[d], a.bin);
+                const int bh = floor_div(ph - a.dom_lo[d], a.bin);
+                for (int b = bl; b <= bh && n < 16; ++b)
+                {
+                    int k = n++;
+                    while (k > 0 && (s_lb[d][k - 1] > b || (s_lb[d][k - 1] == b && s_lo[d][k - 1] > o)))
+                    {
+                        s_lb[d][k] = s_lb[d][k - 1];
+                        s_lo[d][k] = s_lo[d][k - 1];
+                        --k;
+                    }
+                    s_lb[d][k] = b;
+                    s_lo[d][k] = o;
+                }
+            }
+        }
+        s_ln[d] = n;
+    }
+    __syncthreads();
+    const int nx = s_ln[0], ny = s_ln[1], nz = s_ln[2];
+    const int ncomb = nx * ny * nz;
+
+    // registers: tile values (and the FP32 partial / compensation)
+    T acc[ZC];
+    T part[FP64 ? 1 : ZC];
+    T comp[FP64 ? 1 : ZC];
+    const bool overwrite = a.mode == IBAMR_SPREAD_OVERWRITE;
+#pragma unroll
+    for (int z = 0; z < ZC; ++z)
+    {
+        acc[z] = T(0);
+        if (!FP64)
+        {
+            part[z] = T(0);
+            comp[z] = T(0);
+        }
+    }
+
+    // total number of points in the visited bins; an empty tile keeps its values
+    int total = 0;
+    for (int c = lane; c < ncomb; c += 64)
+    {
+        const int kx = c % nx, ky = (c / nx) % ny, kz = c / (nx * ny);
+        const int m = bin_mem(3, s_lb[0][kx], s_lb[1][ky], s_lb[2][kz], a.nb, a.nsb);
+        total += a.bin_start[m + 1] - a.bin_start[m];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off);
+    if (total == 0 && !overwrite) return;
+    if (!overwrite && col_in)
+    {
+#pragma unroll
+        for (int z = 0; z < ZC; ++z)
+            if (flo[2] + z <= fhi[2]) acc[z] = arr[col_off + plane * (size_t)(flo[2] + z - alo[2])];
+    }
+
+    const T inv_vol = T(1.0 / a.vol);
+    const T vol = T(a.vol);
+    for (int cb = 0; cb < ncomb && total > 0; cb += S3_MAXCOMB)
+    {
+        // ---- this group of combinations: ranges and their prefix (in canonical order)
+        {
+            const int c = cb + lane;
+            int st = 0, ln = 0, oc = 0;
+            if (lane < S3_MAXCOMB && c < ncomb)
+            {
+                const int kx = c % nx, ky = (c / nx) % ny, kz = c / (nx * ny);
+                const int m = bin_mem(3, s_lb[0][kx], s_lb[1][ky], s_lb[2][kz], a.nb, a.nsb);
+                st = a.bin_start[m];
+                ln = a.bin_start[m + 1] - st;
+                oc = (s_lo[0][kx] + 1) + 3 * (s_lo[1][ky] + 1) + 9 * (s_lo[2][kz] + 1);
+            }
+            // inclusive scan of ln across the wavefront
+            int inc = ln;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1)
+            {
+                const int t = __shfl_up(inc, off);
+                if (lane >= off) inc += t;
+            }
+            if (lane < S3_MAXCOMB)
+            {
+                s_cs[lane] = st;
+                s_cl[lane] = ln;
+                s_co[lane] = oc;
+                s_cp[lane + 1] = inc;
+            }
+            if (lane == 0) s_cp[0] = 0;
+        }
+        __syncthreads();
+        const int ngc = min(S3_MAXCOMB, ncomb - cb);
+        const int gtotal = s_cp[ngc];
+
+        // chunks: MERGE packs any 64 consecutive points of the group; otherwise a
+        // chunk never crosses a combination (64-point blocks of each bin)
+        int r = 0;          // current combination (non-merge)
+        int rpos = 0;       // position inside it (non-merge)
+        int q0 = 0;         // position in the concatenation (merge)
+        while (MERGE ? (q0 < gtotal) : (r < ngc))
+        {
+            if (!MERGE && rpos >= s_cl[r])
+            {
+                ++r;
+                rpos = 0;
+                continue;
+            }
+            // ---- lane's point
+            int k = -1, oc = 13;
+            if (MERGE)
+            {
+                const int q = q0 + lane;
+                if (q < gtotal)
+                {
+                    int lo = 0, hi = ngc - 1;  // last combination with prefix <= q
+                    while (lo < hi)
+                    {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_cp[mid] <= q)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    k = s_cs[lo] + (q - s_cp[lo]);
+                    oc = s_co[lo];
+                }
+            }
+            else
+            {
+                if (rpos + lane < s_cl[r])
+                {
+                    k = s_cs[r] + rpos + lane;
+                    oc = s_co[r];
+                }
+            }
+            int my_zz = 1 << 20;
+            if (k >= 0)
+            {
+                const int o[3] = {oc % 3 - 1, (oc / 3) % 3 - 1, oc / 9 - 1};
+                const int4 c4 = a.cells[k];
+                const int cc[3] = {c4.x, c4.y, c4.z};
+                bool ok = true;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    const int ci = cc[d] + o[d] * a.pshift[d];
+                    ok = ok && ci >= clo[d] && ci <= chi[d];
+                }
+                if (ok)
+                {
+                    T w[3][W];
+                    int bse[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        // X + Xshift, Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
+                        const double Xs = a.X[3 * (size_t)k + d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
+                        bse[d] = P.ilo[d] + weights_1d<KER>(Xs, xl[d], a.dx[d], w[d]);
+                    }
+                    const int dx0 = bse[0] - flo[0], dy0 = bse[1] - flo[1], zz = bse[2] - flo[2];
+                    if (dx0 > -W && dx0 < 8 && dy0 > -W && dy0 < 8 && zz > -W && zz < ZC)
+                    {
+                        const T Fv = a.F[(size_t)a.fdepth * k + fcomp];
+                        T* rx = s_rx + lane * S3_PAD;
+                        T* ry = s_ry + lane * S3_PAD;
+                        // only positions W-1 .. W+6 (the 8 lanes) are ever read
+#pragma unroll
+                        for (int i = W - 1; i < W + 7; ++i)
+                        {
+                            rx[i] = T(0);
+                            ry[i] = T(0);
+                        }
+                        const T fx = FP64 ? T(1) : (KT::IB ? Fv : Fv * inv_vol);
+#pragma unroll
+                        for (int j = 0; j < W; ++j)
+                        {
+                            rx[dx0 + W - 1 + j] = FP64 ? w[0][j] : w[0][j] * fx;
+                            ry[dy0 + W - 1 + j] = w[1][j];
+                            // FP64 IB folds 1/vol into w2 exactly as the reference (f.m4:1435)
+                            s_wz[lane * 8 + j] = (FP64 && KT::IB) ? w[2][j] / vol
+                                                                  : ((!FP64 && KT::IB) ? w[2][j] * inv_vol : w[2][j]);
+                        }
+                        s_F[lane] = Fv;
+                        s_dx0[lane] = dx0;
+                        s_dy0[lane] = dy0;
+                        my_zz = zz;
+                    }
+                }
+            }
+            __syncthreads();
+            const bool chunk_any = __ballot(my_zz < (1 << 20)) != 0;
+            if (chunk_any)
+            {
+#pragma unroll
+                for (int zz = -(W - 1); zz < ZC; ++zz)
+                {
+                    unsigned long long m = __ballot(my_zz == zz);
+                    while (m)
+                    {
+                        const int j = __builtin_ctzll(m);
+                        m &= m - 1;
+                        const T wx = s_rx[j * S3_PAD + x + W - 1];
+                        const T wy = s_ry[j * S3_PAD + y + W - 1];
+                        T wz[W];
+#pragma unroll
+                        for (int i = 0; i < W; ++i) wz[i] = s_wz[j * 8 + i];
+                        if (!FP64)
+                        {
+                            const T b = wx * wy;
+#pragma unroll
+                            for (int i = 0; i < W; ++i)
+                                if (zz + i >= 0 && zz + i < ZC) part[zz + i] = fmaf(b, wz[i], part[zz + i]);
+                        }
+                        else
+                        {
+                            const bool inside = (unsigned)(x - s_dx0[j]) < (unsigned)W &&
+                                                (unsigned)(y - s_dy0[j]) < (unsigned)W;
+                            if (inside)
+                            {
+                                const T Fv = s_F[j];
+#pragma unroll
+                                for (int i = 0; i < W; ++i)
+                                    if (zz + i >= 0 && zz + i < ZC)
+                                    {
+                                        const T c = KT::IB ? (wx * (wy * wz[i])) * Fv : (((wx * wy) * wz[i]) * Fv) / vol;
+                                        acc[zz + i] = acc[zz + i] + c;
+                                    }
+                            }
+                        }
+                    }
+                }
+                if (!FP64)
+                {
+                    // fold the chunk's partial sums into the compensated totals
+#pragma unroll
+                    for (int z = 0; z < ZC; ++z)
+                    {
+                        const T yv = part[z] - comp[z];
+                        const T t = acc[z] + yv;
+                        comp[z] = (t - acc[z]) - yv;
+                        acc[z] = t;
+                        part[z] = T(0);
+                    }
+                }
+            }
+            __syncthreads();
+            if (MERGE)
+                q0 += 64;
+            else
+                rpos += 64;
+        }
+    }
+
+    if (col_in)
+    {
+#pragma unroll
+        for (int z = 0; z < ZC; ++z)
+            if (flo[2] + z <= fhi[2])
+            {
+                T v = acc[z];
+                if (!FP64) v = v - comp[z];
+                arr[col_off + plane * (size_t)(flo[2] + z - alo[2])] = v;
+            }
+    }
+}
+
+// ----------------------------------------------------------------------------- interpolation
+
+constexpr int I3_THREADS = 256;
+
+template <int KER, typename T>
+__global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds_elems)
+{
+    using KT = KernelTraits<KER>;
+    constexpr int W = KT::W;
+    constexpr bool FP64 = sizeof(T) == 8;
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    T* s_u = reinterpret_cast<T*>(s_raw);
+    __shared__ int s_lo[3], s_hi[3], s_pid, s_flag;
+
+    const int sb = blockIdx.x;
+    const int s0 = a.bin_start[sb * 8], s1 = a.bin_start[sb * 8 + 8];
+    if (s0 == s1) return;
+    const int tid = threadIdx.x;
+
+    const int narrays = a.side ? 3 : 1;
+    for (int ax = 0; ax < narrays; ++ax)
+    {
+        const int axis = a.side ? ax : -1;
+        if (tid < 3)
+        {
+            s_lo[tid] = 1 << 30;
+            s_hi[tid] = -(1 << 30);
+        }
+        if (tid == 0)
+        {
+            s_pid = -1;
+            s_flag = 0;
+        }
+        __syncthreads();
+        // pass 1: owning patch and stencil bases of every point -> block extent
+        // (thread-local min/max, then one LDS atomic per wavefront)
+        int lmin[3] = {1 << 30, 1 << 30, 1 << 30}, lmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        int lpid = -1, lflag = 0;
+        for (int k = s0 + tid; k < s1; k += I3_THREADS)
+        {
+            const int key = a.keys_sorted[k];
+            const int4 c4 = a.cells[k];
+            const int cc[3] = {c4.x, c4.y, c4.z};
+            int pid = -1;
+            for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && pid < 0; ++j)
+            {
+                const PatchDev& Q = a.patches[a.bin_patches[j]];
+                if (cc[0] >= Q.ilo[0] && cc[0] <= Q.ihi[0] && cc[1] >= Q.ilo[1] && cc[1] <= Q.ihi[1] &&
+                    cc[2] >= Q.ilo[2] && cc[2] <= Q.ihi[2])
+                    pid = a.bin_patches[j];
+            }
+            if (pid < 0) continue;
+            if (lpid >= 0 && lpid != pid) lflag |= 1;
+            lpid = pid;
+            const PatchDev P = a.patches[pid];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                const double xl = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);
+                T w[W];
+                const int b = P.ilo[d] + weights_1d<KER>(a.X[3 * (size_t)k + d], xl, a.dx[d], w);
+                const int alo = P.ilo[d] - a.gcw[d];
+                const int ahi = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0);
+                if (b < alo || b + W - 1 > ahi) lflag |= 2;  // needs clipping
+                lmin[d] = min(lmin[d], b);
+                lmax[d] = max(lmax[d], b);
+            }
+        }
+        {
+            // wavefront reductions; patch id: any two different ids set the flag
+            int pmin = lpid < 0 ? (1 << 30) : lpid, pmax = lpid;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+            {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    lmin[d] = min(lmin[d], __shfl_xor(lmin[d], off));
+                    lmax[d] = max(lmax[d], __shfl_xor(lmax[d], off));
+                }
+                pmin = min(pmin, __shfl_xor(pmin, off));
+                pmax = max(pmax, __shfl_xor(pmax, off));
+                lflag |= __shfl_xor(lflag, off);
+            }
+            if ((tid & 63) == 0)
+            {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    atomicMin(&s_lo[d], lmin[d]);
+                    atomicMax(&s_hi[d], lmax[d]);
+                }
+                if (pmax >= 0)
+                {
+                    if (pmin != pmax) lflag |= 1;
+                    const int prev = atomicCAS(&s_pid, -1, pmin);
+                    if (prev != -1 && prev != pmin) lflag |= 1;
+                }
+                if (lflag) atomicOr(&s_flag, lflag);
+            }
+        }
+        __syncthreads();
+        const int pid = s_pid;
+        if (pid < 0) return;  // no point of this superbin lies in a patch interior
+        const PatchDev P = a.patches[pid];
+        int alo[3], an[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            alo[d] = P.ilo[d] - a.gcw[d];
+            an[d] = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0) - alo[d] + 1;
+        }
+        const int e0 = s_hi[0] - s_lo[0] + W, e1 = s_hi[1] - s_lo[1] + W, e2 = s_hi[2] - s_lo[2] + W;
+        const bool staged = s_flag == 0 && e0 * e1 * e2 <= lds_elems;
+        const size_t plane = (size_t)an[0] * an[1];
+        const int ncomps = a.side ? 1 : a.depth;
+        for (int cd = 0; cd < ncomps; ++cd)
+        {
+            const T* __restrict__ u = a.arrays[pid * a.narr + (a.side ? ax : 0)] + (size_t)cd * plane * an[2];
+            if (staged)
+            {
+                const int lo0 = s_lo[0], lo1 = s_lo[1], lo2 = s_lo[2];
+                const int nrow = e1 * e2;
+                for (int idx = tid; idx < e0 * nrow; idx += I3_THREADS)
+                {
+                    const int i0 = idx % e0;
+                    const int r = idx / e0;
+                    const int i1 = r % e1, i2 = r / e1;
+                    s_u[idx] = u[(size_t)(lo0 + i0 - alo[0]) + (size_t)an[0] * (lo1 + i1 - alo[1]) +
+                                 plane * (size_t)(lo2 + i2 - alo[2])];
+                }
+                __syncthreads();
+            }
+            for (int k = s0 + tid; k < s1; k += I3_THREADS)
+            {
+                const int4 c4 = a.cells[k];
+                // the point's own patch (the superbin may straddle patches: then no staging)
+                int qid = pid;
+                if (!staged)
+                {
+                    const int key = a.keys_sorted[k];
+                    qid = -1;
+                    for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && qid < 0; ++j)
+                    {
+                        const PatchDev& Q = a.patches[a.bin_patches[j]];
+                        if (c4.x >= Q.ilo[0] && c4.x <= Q.ihi[0] && c4.y >= Q.ilo[1] && c4.y <= Q.ihi[1] &&
+                            c4.z >= Q.ilo[2] && c4.z <= Q.ihi[2])
+                            qid = a.bin_patches[j];
+                    }
+                    if (qid < 0) continue;
+                }
+                const PatchDev Pq = a.patches[qid];
+                if (c4.x < Pq.ilo[0] || c4.x > Pq.ihi[0] || c4.y < Pq.ilo[1] || c4.y > Pq.ihi[1] ||
+                    c4.z < Pq.ilo[2] || c4.z > Pq.ihi[2])
+                    continue;
+                const int p = a.perm[k];
+                T w[3][W];
+                int bse[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    const double xl = Pq.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);
+                    bse[d] = Pq.ilo[d] + weights_1d<KER>(a.X[3 * (size_t)k + d], xl, a.dx[d], w[d]);
+                }
+                Sum<T> acc;
+                if (staged)
+                {
+                    const T* base = s_u + (bse[0] - s_lo[0]) + e0 * ((bse[1] - s_lo[1]) + e1 * (bse[2] - s_lo[2]));
+#pragma unroll
+                    for (int i2 = 0; i2 < W; ++i2)
+#pragma unroll
+                        for (int i1 = 0; i1 < W; ++i1)
+                        {
+                            const T* row = base + e0 * (i1 + e1 * i2);
+                            if (FP64)
+                            {
+#pragma unroll
+                                for (int i0 = 0; i0 < W; ++i0)
+                                    acc.add(KT::IB ? (w[0][i0] * (w[1][i1] * w[2][i2])) * row[i0]
+                                                   : ((w[0][i0] * w[1][i1]) * w[2][i2]) * row[i0]);
+                            }
+                            else
+                            {
+                                T s = T(0);
+#pragma unroll
+                                for (int i0 = 0; i0 < W; ++i0) s = fmaf(w[0][i0], row[i0], s);
+                                acc.add((w[1][i1] * w[2][i2]) * s);
+                            }
+                        }
+                }
+                else
+                {
+                    // clipped stencil straight from the point's ghosted array (f.m4:1309-1314)
+                    int lo[3], hi[3], qlo[3], qn[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        qlo[d] = Pq.ilo[d] - a.gcw[d];
+                        qn[d] = Pq.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0) - qlo[d] + 1;
+                        lo[d] = max(bse[d], qlo[d]) - bse[d];
+                        hi[d] = min(bse[d] + W - 1, qlo[d] + qn[d] - 1) - bse[d];
+                    }
+                    const size_t qplane = (size_t)qn[0] * qn[1];
+                    const T* __restrict__ uq =
+                        a.arrays[qid * a.narr + (a.side ? ax : 0)] + (size_t)cd * qplane * qn[2];
+#pragma unroll
+                    for (int i2 = 0; i2 < W; ++i2)
+                    {
+                        if (i2 < lo[2] || i2 > hi[2]) continue;
+#pragma unroll
+                        for (int i1 = 0; i1 < W; ++i1)
+                        {
+                            if (i1 < lo[1] || i1 > hi[1]) continue;
+                            const T* row = uq + (size_t)(bse[0] - qlo[0]) + (size_t)qn[0] * (bse[1] + i1 - qlo[1]) +
+                                           qplane * (size_t)(bse[2] + i2 - qlo[2]);
+                            if (FP64)
+                            {
+#pragma unroll
+                                for (int i0 = 0; i0 < W; ++i0)
+                                    if (i0 >= lo[0] && i0 <= hi[0])
+                                        acc.add(KT::IB ? (w[0][i0] * (w[1][i1] * w[2][i2])) * row[i0]
+                                                       : ((w[0][i0] * w[1][i1]) * w[2][i2]) * row[i0]);
+                            }
+                            else
+                            {
+                                T s = T(0);
+#pragma unroll
+                                for (int i0 = 0; i0 < W; ++i0)
+                                    if (i0 >= lo[0] && i0 <= hi[0]) s = fmaf(w[0][i0], row[i0], s);
+                                acc.add((w[1][i1] * w[2][i2]) * s);
+                            }
+                        }
+                    }
+                }
+                a.U[(size_t)a.fdepth * p + (a.side ? ax : cd)] = acc.get();
+            }
+            __syncthreads();
+        }
+    }
+}
+} // namespace

@@ -94,6 +94,17 @@ static int min_ghost(int k) { return stencil_size(k) < 0 ? -1 : stencil_size(k) 
 
 __host__ __device__ inline int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
+// Memory slot of a bin.  3-D: bins are grouped into superbins of 2x2x2 bins
+// whose points are contiguous (the interpolation stages one superbin block);
+// 2-D: row-major.  The spread visits bins in logical (z, y, x) order
+// whatever their memory slots.
+__host__ __device__ inline int bin_mem(int ndim, int bx, int by, int bz, const int* nb, const int* nsb)
+{
+    if (ndim == 2) return by * nb[0] + bx;
+    const int sb = ((bz >> 1) * nsb[1] + (by >> 1)) * nsb[0] + (bx >> 1);
+    return sb * 8 + (((bz & 1) << 2) | ((by & 1) << 1) | (bx & 1));
+}
+
 } // namespace ibamr
 
 using namespace ibamr;
@@ -111,6 +122,7 @@ struct ibamr_plan
     int idx_gcw = 0;
     int reach = 0;
     int tile = 8;
+    int tsz[3] = {8, 8, 8};   // spread tile extent per axis
     int bin = 8;
     int drift = 0;
     double dx[3] = {1, 1, 1};
@@ -118,7 +130,9 @@ struct ibamr_plan
     double dom_xlo[3] = {0, 0, 0}, dom_xhi[3] = {0, 0, 0};
     int pshift[3] = {0, 0, 0};
     int nb[3] = {1, 1, 1};
+    int nsb[3] = {1, 1, 1};
     int nbins = 1;
+    int nmem = 1;           // bin memory slots (>= nbins)
     int npatches = 0;
     std::vector<PatchDev> patches_h;
     int max_points = 0;
@@ -128,7 +142,7 @@ struct ibamr_plan
     PatchDev* patches_d = nullptr;
     TileDesc* tiles_d = nullptr;
     int ntiles = 0;
-    int* bin_patch_start_d = nullptr;  // [nbins+1]
+    int* bin_patch_start_d = nullptr;  // [nmem+2]
     int* bin_patches_d = nullptr;
     void** arrays_d[MAX_SLOTS] = {};
     int* keys_d = nullptr;
@@ -137,7 +151,9 @@ struct ibamr_plan
     int* perm_d = nullptr;
     int4* cell_d = nullptr;
     int4* cell_sorted_d = nullptr;
-    int* bin_start_d = nullptr;        // [nbins+2]
+    double* Xs_d = nullptr;            // positions in bin order
+    void* Fs_d = nullptr;              // forces in bin order
+    int* bin_start_d = nullptr;        // [nmem+2]
     void* sort_tmp_d = nullptr;
     size_t sort_tmp_bytes = 0;
     int sort_bits = 1;
@@ -154,8 +170,9 @@ struct BinArgs
     double xlo[3], xhi[3], dx[3];
     int dom_lo[3], dom_hi[3];
     int nb[3];
+    int nsb[3];
     int bin;
-    int nbins;
+    int nmem;
     int* keys;
     int* idx;
     int4* cell;
@@ -184,26 +201,26 @@ __global__ void k_bin_keys(BinArgs a)
         inside = inside && (v >= (double)a.dom_lo[d]) && (v <= (double)a.dom_hi[d]);
         c[d] = inside ? (int)v : a.dom_lo[d];
     }
-    int key = a.nbins;  // sentinel: sorts after every bin
+    int key = a.nmem;  // sentinel: sorts after every bin
     if (inside)
     {
         int b[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < NDIM; ++d) b[d] = (c[d] - a.dom_lo[d]) / a.bin;
-        key = NDIM == 3 ? (b[2] * a.nb[1] + b[1]) * a.nb[0] + b[0] : b[1] * a.nb[0] + b[0];
+        key = bin_mem(NDIM, b[0], b[1], b[2], a.nb, a.nsb);
     }
     a.keys[p] = key;
     a.idx[p] = p;
     a.cell[p] = make_int4(c[0], c[1], c[2], 0);
 }
 
-// bin_start[b] = first sorted position with key >= b, for b = 0..nbins+1
-__global__ void k_bin_bounds(const int* __restrict__ keys, int n, int nbins, int* __restrict__ start)
+// bin_start[b] = first sorted position with key >= b, for b = 0..nmem+1
+__global__ void k_bin_bounds(const int* __restrict__ keys, int n, int nmem, int* __restrict__ start)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
     const int prev = i == 0 ? -1 : keys[i - 1];
-    const int cur = i == n ? nbins + 1 : keys[i];
+    const int cur = i == n ? nmem + 1 : keys[i];
     for (int b = prev + 1; b <= cur; ++b) start[b] = i;
 }
 
@@ -214,6 +231,19 @@ __global__ void k_gather_cells(const int* __restrict__ perm, const int4* __restr
     if (i < n) out[i] = cell[perm[i]];
 }
 
+// Point data in bin order: every later kernel reads slot k coalesced.
+template <typename T>
+__global__ void k_permute_points(const int* __restrict__ perm, int n, int ndim, const double* __restrict__ X,
+                                 double* __restrict__ Xs, int fdepth, const T* __restrict__ F, T* __restrict__ Fs)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t p = (size_t)perm[k];
+    for (int d = 0; d < ndim; ++d) Xs[(size_t)ndim * k + d] = X[ndim * p + d];
+    if (F)
+        for (int c = 0; c < fdepth; ++c) Fs[(size_t)fdepth * k + c] = F[fdepth * p + c];
+}
+
 // ----------------------------------------------------------------------------- spread
 
 template <typename T>
@@ -222,12 +252,12 @@ struct SpreadArgs
     const TileDesc* tiles;
     const PatchDev* patches;
     T* const* arrays;
-    const double* X;
-    const T* F;
-    const int* perm;
+    const double* X;   // positions in bin order (X[perm[k]] at slot k)
+    const T* F;        // forces in bin order
     const int4* cells;
     const int* bin_start;
     int nb[3];
+    int nsb[3];
     int bin;
     int dom_lo[3], dom_hi[3];
     int pshift[3];
@@ -357,8 +387,7 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
             for (int ky = 0; ky < ln[1]; ++ky)
                 for (int kx = 0; kx < ln[0]; ++kx)
                 {
-                    const int b = NDIM == 3 ? (lb[2][kz] * a.nb[1] + lb[1][ky]) * a.nb[0] + lb[0][kx]
-                                            : lb[1][ky] * a.nb[0] + lb[0][kx];
+                    const int b = bin_mem(NDIM, lb[0][kx], lb[1][ky], lb[2][kz], a.nb, a.nsb);
                     total += a.bin_start[b + 1] - a.bin_start[b];
                 }
     }
@@ -389,8 +418,7 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
         for (int ky = 0; ky < ln[1]; ++ky)
             for (int kx = 0; kx < ln[0]; ++kx)
             {
-                const int b = NDIM == 3 ? (lb[2][kz] * a.nb[1] + lb[1][ky]) * a.nb[0] + lb[0][kx]
-                                        : lb[1][ky] * a.nb[0] + lb[0][kx];
+                const int b = bin_mem(NDIM, lb[0][kx], lb[1][ky], lb[2][kz], a.nb, a.nsb);
                 const int o[3] = {lo_[0][kx], lo_[1][ky], lo_[2][kz]};
                 const int s0 = a.bin_start[b], s1 = a.bin_start[b + 1];
                 for (int base = s0; base < s1; base += NSLOT)
@@ -411,12 +439,11 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
                         }
                         if (act)
                         {
-                            const int p = a.perm[k];
 #pragma unroll
                             for (int d = 0; d < NDIM; ++d)
                             {
                                 // X + Xshift with Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
-                                const double Xs = a.X[NDIM * p + d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
+                                const double Xs = a.X[NDIM * (size_t)k + d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
                                 T w[W];
                                 const int rel = weights_1d<KER>(Xs, xl[d], a.dx[d], w);
                                 const int bse = P.ilo[d] + rel;
@@ -430,7 +457,7 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
 #pragma unroll
                                 for (int j = 0; j < W; ++j) s_w[d][j][lane] = w[j];
                             }
-                            const T Fv = a.F[(size_t)a.fdepth * p + fcomp];
+                            const T Fv = a.F[(size_t)a.fdepth * k + fcomp];
                             s_F[lane] = FP64 ? Fv : Fv * inv_vol;
                         }
                     }
@@ -513,11 +540,12 @@ struct InterpArgs
     const int* bin_patch_start;
     const int* bin_patches;
     T* const* arrays;
-    const double* X;
+    const double* X;   // positions in bin order
     T* U;
     const int* perm;
     const int4* cells;
     const int* keys_sorted;
+    const int* bin_start;
     int n;
     double dx[3];
     int gcw[3];
@@ -581,7 +609,7 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a)
     const int p = a.perm[k];
     double Xp[3];
 #pragma unroll
-    for (int d = 0; d < NDIM; ++d) Xp[d] = a.X[NDIM * p + d];
+    for (int d = 0; d < NDIM; ++d) Xp[d] = a.X[NDIM * (size_t)k + d];
 
     const int narrays = a.side ? NDIM : 1;
     for (int ax = 0; ax < narrays; ++ax)
@@ -682,6 +710,8 @@ __global__ void k_fill_periodic(T* __restrict__ arr, int n0, int n1, int n2, int
 
 } // namespace
 
+#include "le_kernels3d.h"
+
 // ----------------------------------------------------------------------------- host side
 
 namespace
@@ -714,12 +744,12 @@ void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double
     a.arrays = reinterpret_cast<T* const*>(P->arrays_d[slot]);
     a.X = X;
     a.F = static_cast<const T*>(F);
-    a.perm = P->perm_d;
     a.cells = P->cell_sorted_d;
     a.bin_start = P->bin_start_d;
     for (int d = 0; d < 3; ++d)
     {
         a.nb[d] = P->nb[d];
+        a.nsb[d] = P->nsb[d];
         a.dom_lo[d] = P->dom_lo[d];
         a.dom_hi[d] = P->dom_hi[d];
         a.pshift[d] = P->pshift[d];
@@ -734,6 +764,14 @@ void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double
     a.reach = P->reach;
     a.side = P->centering == IBAMR_SIDE;
     a.mode = mode;
+}
+} // namespace
+
+namespace
+{
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
 }
 } // namespace
 
@@ -760,19 +798,21 @@ int ibamr_get_minimum_ghost_width(const char* k)
 int ibamr_plan_destroy(ibamr_plan* P)
 {
     if (!P) return IBAMR_OK;
-    hipFree(P->patches_d);
-    hipFree(P->tiles_d);
-    hipFree(P->bin_patch_start_d);
-    hipFree(P->bin_patches_d);
-    for (int s = 0; s < MAX_SLOTS; ++s) hipFree(P->arrays_d[s]);
-    hipFree(P->keys_d);
-    hipFree(P->keys_sorted_d);
-    hipFree(P->idx_d);
-    hipFree(P->perm_d);
-    hipFree(P->cell_d);
-    hipFree(P->cell_sorted_d);
-    hipFree(P->bin_start_d);
-    hipFree(P->sort_tmp_d);
+    dfree(P->patches_d);
+    dfree(P->tiles_d);
+    dfree(P->bin_patch_start_d);
+    dfree(P->bin_patches_d);
+    for (int s = 0; s < MAX_SLOTS; ++s) dfree(P->arrays_d[s]);
+    dfree(P->keys_d);
+    dfree(P->keys_sorted_d);
+    dfree(P->idx_d);
+    dfree(P->perm_d);
+    dfree(P->cell_d);
+    dfree(P->cell_sorted_d);
+    dfree(P->Xs_d);
+    dfree(P->Fs_d);
+    dfree(P->bin_start_d);
+    dfree(P->sort_tmp_d);
     delete P;
     return IBAMR_OK;
 }
@@ -828,12 +868,17 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
         return fail(IBAMR_ERR_INVALID, "unsupported tile edge");
     }
     P->bin = P->tile;
-    // per-axis (bin, image) lists of a tile hold at most 16 entries
-    if (3 * ((P->tile + 2 * P->reach - 1) / P->bin + 2) > 16)
-    {
-        delete P;
-        return fail(IBAMR_ERR_INVALID, "drift_cells too large for the tile size");
-    }
+    P->tsz[0] = P->tile;
+    P->tsz[1] = P->tile;
+    P->tsz[2] = ndim == 3 ? S3_ZC : 1;
+    // per-axis (bin, image) lists of a tile hold at most 16 entries: the image
+    // ranges partition the reachable cells, plus one split per domain side
+    for (int d = 0; d < ndim; ++d)
+        if ((P->tsz[d] + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16)
+        {
+            delete P;
+            return fail(IBAMR_ERR_INVALID, "drift_cells too large for the tile size");
+        }
     for (int d = 0; d < 3; ++d)
     {
         const bool on = d < ndim;
@@ -847,17 +892,19 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
         P->nb[d] = on ? (P->dom_hi[d] - P->dom_lo[d] + P->bin) / P->bin : 1;
     }
     const long nbins_l = (long)P->nb[0] * P->nb[1] * P->nb[2];
-    if (nbins_l > (1L << 30))
+    if (nbins_l > (1L << 26))
     {
         delete P;
         return fail(IBAMR_ERR_INVALID, "level too large for 32-bit bin keys");
     }
     P->nbins = (int)nbins_l;
+    for (int d = 0; d < 3; ++d) P->nsb[d] = (P->nb[d] + 1) / 2;
+    P->nmem = ndim == 3 ? P->nsb[0] * P->nsb[1] * P->nsb[2] * 8 : P->nbins;
     P->npatches = L->npatches;
     P->max_points = C->max_points;
 
     // patches, and the patches overlapping each bin (for point -> patch lookup in interp)
-    std::vector<std::vector<int>> per_bin(P->nbins);
+    std::vector<std::vector<int>> per_bin(P->nmem);
     for (int i = 0; i < L->npatches; ++i)
     {
         const ibamr_patch_geom& g = L->patches[i];
@@ -883,16 +930,16 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
         }
         for (int bz = bl[2]; bz <= bh[2]; ++bz)
             for (int by = bl[1]; by <= bh[1]; ++by)
-                for (int bx = bl[0]; bx <= bh[0]; ++bx) per_bin[(bz * P->nb[1] + by) * P->nb[0] + bx].push_back(i);
+                for (int bx = bl[0]; bx <= bh[0]; ++bx) per_bin[bin_mem(ndim, bx, by, bz, P->nb, P->nsb)].push_back(i);
     }
-    std::vector<int> bps(P->nbins + 2, 0), bpl;
-    for (int b = 0; b < P->nbins; ++b)
+    std::vector<int> bps(P->nmem + 2, 0), bpl;
+    for (int b = 0; b < P->nmem; ++b)
     {
         bps[b] = (int)bpl.size();
         for (int i : per_bin[b]) bpl.push_back(i);
     }
-    bps[P->nbins] = (int)bpl.size();
-    bps[P->nbins + 1] = (int)bpl.size();
+    bps[P->nmem] = (int)bpl.size();
+    bps[P->nmem + 1] = (int)bpl.size();
     if (bpl.empty()) bpl.push_back(0);
 
     // work list: every tile of every array of every patch
@@ -908,7 +955,7 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
             for (int d = 0; d < ndim; ++d)
             {
                 const int n = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
-                nt[d] = (n + P->tile - 1) / P->tile;
+                nt[d] = (n + P->tsz[d] - 1) / P->tsz[d];
             }
             for (int tz = 0; tz < nt[2]; ++tz)
                 for (int ty = 0; ty < nt[1]; ++ty)
@@ -950,9 +997,11 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
     PLAN_HIP(hipMalloc(&P->perm_d, sizeof(int) * np));
     PLAN_HIP(hipMalloc(&P->cell_d, sizeof(int4) * np));
     PLAN_HIP(hipMalloc(&P->cell_sorted_d, sizeof(int4) * np));
-    PLAN_HIP(hipMalloc(&P->bin_start_d, sizeof(int) * (P->nbins + 2)));
+    PLAN_HIP(hipMalloc(&P->Xs_d, sizeof(double) * ndim * np));
+    PLAN_HIP(hipMalloc(&P->Fs_d, (C->precision == IBAMR_FP64 ? 8 : 4) * (size_t)P->fdepth * np));
+    PLAN_HIP(hipMalloc(&P->bin_start_d, sizeof(int) * (P->nmem + 2)));
     int bits = 1;
-    while ((1L << bits) <= (long)P->nbins) ++bits;  // keys 0..nbins
+    while ((1L << bits) <= (long)P->nmem) ++bits;  // keys 0..nmem
     P->sort_bits = bits;
     PLAN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, P->sort_tmp_bytes, reinterpret_cast<unsigned*>(P->keys_d),
                                                 reinterpret_cast<unsigned*>(P->keys_sorted_d), P->idx_d, P->perm_d,
@@ -994,7 +1043,7 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     P->npoints = npoints;
     if (npoints == 0)
     {
-        HIP_OK(hipMemsetAsync(P->bin_start_d, 0, sizeof(int) * (P->nbins + 2), s));
+        HIP_OK(hipMemsetAsync(P->bin_start_d, 0, sizeof(int) * (P->nmem + 2), s));
         return IBAMR_OK;
     }
     BinArgs a{};
@@ -1008,9 +1057,10 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
         a.dom_lo[d] = P->dom_lo[d];
         a.dom_hi[d] = P->dom_hi[d];
         a.nb[d] = P->nb[d];
+        a.nsb[d] = P->nsb[d];
     }
     a.bin = P->bin;
-    a.nbins = P->nbins;
+    a.nmem = P->nmem;
     a.keys = P->keys_d;
     a.idx = P->idx_d;
     a.cell = P->cell_d;
@@ -1026,7 +1076,7 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
                                               reinterpret_cast<unsigned*>(P->keys_sorted_d), P->idx_d, P->perm_d,
                                               npoints, 0, P->sort_bits, s));
     hipLaunchKernelGGL(k_bin_bounds, dim3((npoints + 1 + B - 1) / B), dim3(B), 0, s, P->keys_sorted_d, npoints,
-                       P->nbins, P->bin_start_d);
+                       P->nmem, P->bin_start_d);
     HIP_OK(hipGetLastError());
     hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d);
     HIP_OK(hipGetLastError());
@@ -1039,7 +1089,7 @@ int ibamr_plan_check(ibamr_plan* P, void* stream)
     if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
     hipStream_t s = static_cast<hipStream_t>(stream);
     int first_sentinel = 0;
-    HIP_OK(hipMemcpyAsync(&first_sentinel, P->bin_start_d + P->nbins, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&first_sentinel, P->bin_start_d + P->nmem, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     if (first_sentinel != P->npoints)
         return fail(IBAMR_ERR_OUT_OF_DOMAIN, std::to_string(P->npoints - first_sentinel) +
@@ -1067,10 +1117,28 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         using T = decltype(prec);
         constexpr int NDIM = decltype(nd)::value;
         constexpr int KER = decltype(ker)::value;
-        constexpr int TILE = NDIM == 3 ? 8 : 16;
         SpreadArgs<T> a{};
         fill_spread_args<T>(P, slot, F, X, mode, a);
-        hipLaunchKernelGGL((k_spread_tiles<NDIM, KER, T, TILE>), dim3(P->ntiles), dim3(WAVE), 0, s, a);
+        if (P->npoints > 0)
+        {
+            const int B = 256;
+            hipLaunchKernelGGL(k_permute_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
+                               P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
+                               static_cast<T*>(P->Fs_d));
+            HIP_OK(hipGetLastError());
+        }
+        a.X = P->Xs_d;
+        a.F = static_cast<const T*>(P->Fs_d);
+        if (!a.tiles || !a.patches || !a.arrays || !a.X || !a.F || !a.cells || !a.bin_start)
+            return fail(IBAMR_ERR_INVALID, "internal: incomplete spread arguments");
+        if (NDIM == 3)
+        {
+            // FP32C packs points across bins; FP64 keeps chunks inside bins (fixed order)
+            constexpr bool MERGE = sizeof(T) == 4;
+            hipLaunchKernelGGL((k_spread3<KER, T, MERGE>), dim3(P->ntiles), dim3(64), 0, s, a);
+        }
+        else
+            hipLaunchKernelGGL((k_spread_tiles<2, KER, T, 16>), dim3(P->ntiles), dim3(WAVE), 0, s, a);
         HIP_OK(hipGetLastError());
         return IBAMR_OK;
     });
@@ -1094,11 +1162,19 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
         a.bin_patch_start = P->bin_patch_start_d;
         a.bin_patches = P->bin_patches_d;
         a.arrays = reinterpret_cast<T* const*>(P->arrays_d[slot]);
-        a.X = X;
-        a.U = static_cast<T*>(U);
         a.perm = P->perm_d;
+        {
+            const int B = 256;
+            hipLaunchKernelGGL(k_permute_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
+                               P->npoints, NDIM, X, P->Xs_d, 0, static_cast<const T*>(nullptr),
+                               static_cast<T*>(nullptr));
+            HIP_OK(hipGetLastError());
+        }
+        a.X = P->Xs_d;
+        a.U = static_cast<T*>(U);
         a.cells = P->cell_sorted_d;
         a.keys_sorted = P->keys_sorted_d;
+        a.bin_start = P->bin_start_d;
         a.n = P->npoints;
         for (int d = 0; d < 3; ++d)
         {
@@ -1109,8 +1185,24 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
         a.fdepth = P->fdepth;
         a.depth = P->depth;
         a.side = P->centering == IBAMR_SIDE;
-        const int B = 256;
-        hipLaunchKernelGGL((k_interp<NDIM, KER, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, a);
+        if (!a.patches || !a.bin_patch_start || !a.bin_patches || !a.arrays || !a.X || !a.U || !a.perm || !a.cells ||
+            !a.keys_sorted || !a.bin_start)
+            return fail(IBAMR_ERR_INVALID, "internal: incomplete interpolation arguments");
+        if (NDIM == 3)
+        {
+            // one workgroup per superbin; LDS holds one staged block of the array
+            const int bytes = sizeof(T) == 4 ? 60 * 1024 : 120 * 1024;
+            const int elems = bytes / (int)sizeof(T);
+            HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp3<KER, T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
+            hipLaunchKernelGGL((k_interp3<KER, T>), dim3(nsb), dim3(I3_THREADS), bytes, s, a, elems);
+        }
+        else
+        {
+            const int B = 256;
+            hipLaunchKernelGGL((k_interp<2, KER, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, a);
+        }
         HIP_OK(hipGetLastError());
         return IBAMR_OK;
     });

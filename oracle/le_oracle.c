@@ -536,6 +536,77 @@ int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* i
     return ne;
 }
 
+/* --------------------------------------------------- z-bucketed entry order */
+
+typedef struct
+{
+    long key[4];
+    int pos;
+} zkey_t;
+
+static int cmp_zkey(const void* a, const void* b)
+{
+    const zkey_t* x = (const zkey_t*)a;
+    const zkey_t* y = (const zkey_t*)b;
+    for (int k = 0; k < 4; ++k)
+    {
+        if (x->key[k] < y->key[k]) return -1;
+        if (x->key[k] > y->key[k]) return 1;
+    }
+    return 0;
+}
+
+void orc_order_entries_z(int kernel, const int* cell, const int* indices, const double* Xshift, int nentries,
+                         const double* X, const double* x_lower, const double* dx, const int* domain_lower,
+                         const int* bin, int chunk, int* perm)
+{
+    zkey_t* z = (zkey_t*)malloc(sizeof(zkey_t) * (size_t)(nentries > 0 ? nentries : 1));
+    long group = -1;
+    int prev_b[3] = {0, 0, 0}, prev_o[3] = {0, 0, 0};
+    for (int l = 0; l < nentries; ++l)
+    {
+        const int p = indices[l];
+        int b[3], o[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            b[d] = floor_div(cell[3 * p + d] - domain_lower[d], bin[d]);
+            o[d] = (int)lround(Xshift[3 * l + d] / dx[d]);
+        }
+        /* entries arrive in canonical (bz,oz,by,oy,bx,ox,p) order: a (bin, image)
+         * group is a run of consecutive entries */
+        int newgroup = l == 0;
+        for (int d = 0; d < 3; ++d) newgroup = newgroup || b[d] != prev_b[d] || o[d] != prev_o[d];
+        if (newgroup)
+        {
+            ++group;
+            for (int d = 0; d < 3; ++d)
+            {
+                prev_b[d] = b[d];
+                prev_o[d] = o[d];
+            }
+        }
+        /* rank of p among all points of its bin (the bin stores them in index
+         * order and the GPU cuts that whole list into chunks) */
+        long rank = 0;
+        for (int q = 0; q < p; ++q)
+        {
+            int same = 1;
+            for (int d = 0; d < 3 && same; ++d) same = floor_div(cell[3 * q + d] - domain_lower[d], bin[d]) == b[d];
+            rank += same;
+        }
+        double w[8];
+        const double Xs = X[3 * p + 2] + Xshift[3 * l + 2];
+        z[l].key[0] = group;
+        z[l].key[1] = rank / chunk;
+        z[l].key[2] = orc_weights_1d(kernel, Xs, x_lower[2], dx[2], w);
+        z[l].key[3] = p;
+        z[l].pos = l;
+    }
+    qsort(z, (size_t)nentries, sizeof(zkey_t), cmp_zkey);
+    for (int l = 0; l < nentries; ++l) perm[l] = z[l].pos;
+    free(z);
+}
+
 /* ------------------------------------------------------- brute force check */
 
 double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const double* dx, const double* period,

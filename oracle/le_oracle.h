@@ -110,6 +110,20 @@ int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* i
                             int idx_gcw, const int* periodic_shift, const int* domain_lower, const double* dx,
                             const int* bin, int interior_only, int max_entries, int* indices, double* Xshift);
 
+/* Reorders a patch entry list (from orc_build_patch_entries, canonical bin
+ * order) into the order in which the GPU 3-D spread sums the contributions to
+ * one array: within each (bin, image offset) group the bin's points are cut
+ * into consecutive blocks of `chunk` points (bin memory order = point index
+ * order), and inside a block the entries are ordered by the first stencil
+ * index along z for this array (x_lower is the array's lower corner, i.e. the
+ * patch x_lower shifted by -dx/2 along the side axis), then by point.  Any
+ * order gives the same sum to round-off; this one makes the FP64 GPU result
+ * bitwise comparable.  `cell` holds the level cell of every point; perm
+ * receives the new order as positions into indices[]. */
+void orc_order_entries_z(int kernel, const int* cell, const int* indices, const double* Xshift, int nentries,
+                         const double* X, const double* x_lower, const double* dx, const int* domain_lower,
+                         const int* bin, int chunk, int* perm);
+
 /* Brute-force O(N*G) evaluation with the scalar delta functions: value of the
  * spread of point values V (depth 1) at one grid location, periodic images
  * included.  Independent of the kernel loops (used for IB_4 and BSPLINE_3). */

@@ -54,6 +54,8 @@ def lib():
                                       ctypes.POINTER(_dp), _ip, _dp, ctypes.c_int, _dp, _dp]
         L.orc_build_patch_entries.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip,
                                               _ip, _dp, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.orc_order_entries_z.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, _dp, _dp, _dp, _ip, _ip,
+                                          ctypes.c_int, _ip]
         L.orc_bruteforce_spread_at.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
                                                ctypes.c_int]
         L.orc_bruteforce_spread_at.restype = ctypes.c_double
@@ -235,3 +237,54 @@ def bruteforce_spread_at(kernel, xg, dx, period, X, V):
     X = _dv(X).reshape(-1, ndim)
     return lib().orc_bruteforce_spread_at(kernel_id(kernel), ndim, _d(_dv(xg)), _d(_dv(dx)), _d(_dv(period)),
                                           _d(X), _d(_dv(V)), X.shape[0])
+
+
+def order_entries_z(kernel, cell, indices, Xshift, X, x_lower, dx, domain_lower, bin_size, chunk=64):
+    """Permutation of a canonical 3-D entry list into the GPU spread order for one
+    array (bin/image group, 64-point chunk of the bin, stencil z-start, point)."""
+    cell = _iv(cell).reshape(-1, 3)
+    indices = _iv(indices)
+    Xshift = _dv(Xshift).reshape(-1, 3)
+    X = _dv(X).reshape(-1, 3)
+    perm = np.zeros(len(indices), dtype=np.int32)
+    lib().orc_order_entries_z(kernel_id(kernel), _i(cell), _i(indices), _d(Xshift), len(indices), _d(X),
+                              _d(_dv(x_lower)), _d(_dv(dx)), _i(_iv(domain_lower)), _i(_iv(bin_size)), int(chunk),
+                              _i(perm))
+    return perm
+
+
+def spread_patch(kernel, centering_side, dx, x_lower, ilower, iupper, gcw, arrays, X, F, cells, idx_gcw,
+                 periodic_shift, domain_lower, depth=1, bin_size=None):
+    """LEInteractor::spread of one patch over its Lagrangian ghost box (entries
+    with periodic images, LIndexSetData model) into `arrays` (side: one float64
+    array per axis; cell: one (depth, ...) array), in the summation order the
+    GPU library uses: canonical (bin, image, point) order in 2-D; in 3-D each
+    array's entries re-ordered by orc_order_entries_z (64-point chunks of each
+    bin, then stencil z-start)."""
+    ndim = len(ilower)
+    if bin_size is None:
+        bin_size = [8] * ndim if ndim == 3 else [16] * ndim
+    idx, sh = build_patch_entries(cells, ilower, iupper, idx_gcw, periodic_shift, domain_lower, dx, bin_size)
+    if len(idx) == 0:
+        return arrays
+    X = _dv(X).reshape(-1, ndim)
+    F = _dv(F).reshape(X.shape[0], -1)
+    if centering_side:
+        for a in range(ndim):
+            xl = list(x_lower)
+            xl[a] -= 0.5 * dx[a]
+            iu = list(iupper)
+            iu[a] += 1
+            if ndim == 3:
+                perm = order_entries_z(kernel, cells, idx, sh, X, xl, dx, domain_lower, bin_size)
+                ii, ss = idx[perm], sh[perm]
+            else:
+                ii, ss = idx, sh
+            q = arrays[a].reshape((1,) + arrays[a].shape)
+            spread(kernel, dx, xl, ilower, iu, gcw, q, X, np.ascontiguousarray(F[:, a]), indices=ii, Xshift=ss)
+    else:
+        if ndim == 3:
+            perm = order_entries_z(kernel, cells, idx, sh, X, x_lower, dx, domain_lower, bin_size)
+            idx, sh = idx[perm], sh[perm]
+        spread(kernel, dx, x_lower, ilower, iupper, gcw, arrays, X, F, indices=idx, Xshift=sh, depth=depth)
+    return arrays

@@ -4,8 +4,9 @@ FP64 plans must match the oracle bitwise:
   * interp: every point, every axis/component (same per-point arithmetic);
   * spread: every grid value of every ghosted array, starting from the same
     non-zero field, against the oracle run over the patch's entry list
-    (points + periodic images, LIndexSetData.cpp:55-150) in the canonical
-    (bin, image, point) order the GPU uses.
+    (points + periodic images, LIndexSetData.cpp:55-150) in the order the
+    library sums contributions (oracle.spread_patch: bin, image, 64-point
+    chunk, stencil z-start, point).
 FP32C plans: relative L-infinity error against the FP64 oracle <= 1e-6 (the
 bar of the precision form).
 """
@@ -57,22 +58,20 @@ def _tile(ndim):
 
 
 def _oracle_spread(orc, level, plan, arrays_np, X, F, kernel):
-    """LDataManager::spread over all patches of the level with the oracle."""
+    """LDataManager::spread over all patches of the level with the oracle, in the
+    library's summation order (the FP64 comparison is bitwise)."""
     ndim = level.ndim
-    gw = plan.data_gcw
     igcw = ia.get_minimum_ghost_width(kernel)
     cells = orc.get_cell_index(X, level.domain_x_lower, level.domain_x_upper, level.dx, level.domain_lower,
                                level.domain_upper)
     out = [a.copy() for a in arrays_np]
     for pi, P in enumerate(level.patches):
-        idx, sh = orc.build_patch_entries(cells, P.ilower, P.iupper, igcw, level.periodic_shift, level.domain_lower,
-                                          level.dx, [_tile(ndim)] * ndim)
         if plan.centering == ia.SIDE:
             q = out[pi * ndim:(pi + 1) * ndim]
-            orc.spread_side(kernel, level.dx, P.x_lower, P.ilower, P.iupper, gw, q, X, F, indices=idx, Xshift=sh)
         else:
-            orc.spread(kernel, level.dx, P.x_lower, P.ilower, P.iupper, gw, out[pi], X, F, indices=idx, Xshift=sh,
-                       depth=plan.depth)
+            q = out[pi]
+        orc.spread_patch(kernel, plan.centering == ia.SIDE, level.dx, P.x_lower, P.ilower, P.iupper, plan.data_gcw,
+                         q, X, F, cells, igcw, level.periodic_shift, level.domain_lower, depth=plan.depth)
     return out
 
 
