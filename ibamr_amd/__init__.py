@@ -42,7 +42,9 @@ C_API = [
     "ibamr_plan_create", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_info", "ibamr_last_error",
-]
+    "ibamr_s0_interp", "ibamr_s0_spread",
+] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3")
+     for op in ("interp", "spread") for d in ("2d", "3d")]
 
 
 class IBAMRError(RuntimeError):
@@ -97,6 +99,11 @@ def lib():
     L.ibamr_plan_fill_periodic_ghosts.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.ibamr_s0_interp.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, dp, ctypes.c_int, ip, ip, ip, dp, ip, dp,
+                                  ctypes.c_int, dp, dp]
+    L.ibamr_s0_spread.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, dp, ctypes.c_int, ip, dp, ctypes.c_int, dp,
+                                  dp, ip, ip, ip, dp]
     _LIB = L
     return L
 
@@ -264,3 +271,45 @@ class LevelPlan:
 
     def fill_periodic_ghosts(self, slot, stream=None):
         _check(lib().ibamr_plan_fill_periodic_ghosts(self._h, slot, _stream_ptr(stream)))
+
+
+# --------------------------------------------------------------------------- S0: Fortran-kernel drop-in
+
+def _np_d(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _np_i(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def s0_interp(kernel, dx, x_lower, ilower, iupper, nugc, u, X, V, indices, Xshift, depth=1):
+    """lagrangian_<kernel>_interp{2,3}d on host arrays (runs on the GPU); V is
+    updated in place (float64, C-contiguous, shape (N, depth))."""
+    import numpy as np
+    ndim = len(ilower)
+    assert V.dtype == np.float64 and V.flags.c_contiguous
+    keep = [_np_d(dx), _np_d(x_lower), _np_i(ilower), _np_i(iupper), _np_i(nugc), _np_d(u), _np_i(indices),
+            _np_d(Xshift), _np_d(X)]
+    _check(lib().ibamr_s0_interp(kernel.encode(), ndim, keep[0][1], keep[1][1], depth, keep[2][1], keep[3][1],
+                                 keep[4][1], keep[5][1], keep[6][1], keep[7][1], len(keep[6][0]), keep[8][1],
+                                 V.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return V
+
+
+def s0_spread(kernel, dx, x_lower, ilower, iupper, nugc, u, X, V, indices, Xshift, depth=1):
+    """lagrangian_<kernel>_spread{2,3}d on host arrays (runs on the GPU); u is
+    accumulated in place (float64, C-contiguous)."""
+    import numpy as np
+    ndim = len(ilower)
+    assert u.dtype == np.float64 and u.flags.c_contiguous
+    keep = [_np_d(dx), _np_d(x_lower), _np_i(indices), _np_d(Xshift), _np_d(X), _np_d(V), _np_i(ilower),
+            _np_i(iupper), _np_i(nugc)]
+    _check(lib().ibamr_s0_spread(kernel.encode(), ndim, keep[0][1], keep[1][1], depth, keep[2][1], keep[3][1],
+                                 len(keep[2][0]), keep[4][1], keep[5][1], keep[6][1], keep[7][1], keep[8][1],
+                                 u.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return u

@@ -56,6 +56,7 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     const int lane = threadIdx.x;
     const int x = lane & 7, y = lane >> 3;
     const TileDesc td = a.tiles[blockIdx.x];
+    if (td.patch < 0) return;  // padding of the XCD-ordered work list
     const PatchDev P = a.patches[td.patch];
     const int axis = a.side ? td.comp : -1;
     const int tsz[3] = {8, 8, ZC};
@@ -197,23 +198,23 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
         const int gtotal = s_cp[ngc];
 
         // chunks: MERGE packs any 64 consecutive points of the group; otherwise a
-        // chunk never crosses a combination (64-point blocks of each bin)
-        int r = 0;          // current combination (non-merge)
-        int rpos = 0;       // position inside it (non-merge)
-        int q0 = 0;         // position in the concatenation (merge)
-        while (MERGE ? (q0 < gtotal) : (r < ngc))
-        {
-            if (!MERGE && rpos >= s_cl[r])
-            {
-                ++r;
-                rpos = 0;
-                continue;
-            }
-            // ---- lane's point
-            int k = -1, oc = 13;
+        // chunk never crosses a combination (64-point blocks of each bin).  The
+        // next chunk's point data is loaded while the current one is applied.
+        int cq = 0, cr = 0, crp = 0;
+        auto normalize = [&]() {
+            if (!MERGE)
+                while (cr < ngc && crp >= s_cl[cr])
+                {
+                    ++cr;
+                    crp = 0;
+                }
+        };
+        auto lane_point = [&](int& k, int& oc) {
+            k = -1;
+            oc = 13;
             if (MERGE)
             {
-                const int q = q0 + lane;
+                const int q = cq + lane;
                 if (q < gtotal)
                 {
                     int lo = 0, hi = ngc - 1;  // last combination with prefix <= q
@@ -229,19 +230,55 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                     oc = s_co[lo];
                 }
             }
+            else if (cr < ngc && crp + lane < s_cl[cr])
+            {
+                k = s_cs[cr] + crp + lane;
+                oc = s_co[cr];
+            }
+        };
+        normalize();
+        bool cvalid = MERGE ? cq < gtotal : cr < ngc;
+        int nk, noc;
+        int4 nc4 = make_int4(0, 0, 0, 0);
+        double nX[3] = {0.0, 0.0, 0.0};
+        T nF = T(0);
+        lane_point(nk, noc);
+        if (nk >= 0)
+        {
+            nc4 = a.cells[nk];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
+            nF = a.F[(size_t)a.fdepth * nk + fcomp];
+        }
+        while (cvalid)
+        {
+            const int k = nk, oc = noc;
+            const int4 c4 = nc4;
+            const double Xk[3] = {nX[0], nX[1], nX[2]};
+            const T Fv = nF;
+            // advance and prefetch the next chunk
+            if (MERGE)
+                cq += 64;
             else
             {
-                if (rpos + lane < s_cl[r])
-                {
-                    k = s_cs[r] + rpos + lane;
-                    oc = s_co[r];
-                }
+                crp += 64;
+                normalize();
             }
+            cvalid = MERGE ? cq < gtotal : cr < ngc;
+            nk = -1;
+            if (cvalid) lane_point(nk, noc);
+            if (nk >= 0)
+            {
+                nc4 = a.cells[nk];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
+                nF = a.F[(size_t)a.fdepth * nk + fcomp];
+            }
+
             int my_zz = 1 << 20;
             if (k >= 0)
             {
                 const int o[3] = {oc % 3 - 1, (oc / 3) % 3 - 1, oc / 9 - 1};
-                const int4 c4 = a.cells[k];
                 const int cc[3] = {c4.x, c4.y, c4.z};
                 bool ok = true;
 #pragma unroll
@@ -258,13 +295,12 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                     for (int d = 0; d < 3; ++d)
                     {
                         // X + Xshift, Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
-                        const double Xs = a.X[3 * (size_t)k + d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
+                        const double Xs = Xk[d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
                         bse[d] = P.ilo[d] + weights_1d<KER>(Xs, xl[d], a.dx[d], w[d]);
                     }
                     const int dx0 = bse[0] - flo[0], dy0 = bse[1] - flo[1], zz = bse[2] - flo[2];
                     if (dx0 > -W && dx0 < 8 && dy0 > -W && dy0 < 8 && zz > -W && zz < ZC)
                     {
-                        const T Fv = a.F[(size_t)a.fdepth * k + fcomp];
                         T* rx = s_rx + lane * S3_PAD;
                         T* ry = s_ry + lane * S3_PAD;
                         // only positions W-1 .. W+6 (the 8 lanes) are ever read
@@ -321,12 +357,12 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                                                 (unsigned)(y - s_dy0[j]) < (unsigned)W;
                             if (inside)
                             {
-                                const T Fv = s_F[j];
+                                const T Fj = s_F[j];
 #pragma unroll
                                 for (int i = 0; i < W; ++i)
                                     if (zz + i >= 0 && zz + i < ZC)
                                     {
-                                        const T c = KT::IB ? (wx * (wy * wz[i])) * Fv : (((wx * wy) * wz[i]) * Fv) / vol;
+                                        const T c = KT::IB ? (wx * (wy * wz[i])) * Fj : (((wx * wy) * wz[i]) * Fj) / vol;
                                         acc[zz + i] = acc[zz + i] + c;
                                     }
                             }
@@ -348,10 +384,6 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                 }
             }
             __syncthreads();
-            if (MERGE)
-                q0 += 64;
-            else
-                rpos += 64;
         }
     }
 
@@ -373,7 +405,7 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
 constexpr int I3_THREADS = 256;
 
 template <int KER, typename T>
-__global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds_elems)
+__global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds_elems, int nsb, int per)
 {
     using KT = KernelTraits<KER>;
     constexpr int W = KT::W;
@@ -382,7 +414,10 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
     T* s_u = reinterpret_cast<T*>(s_raw);
     __shared__ int s_lo[3], s_hi[3], s_pid, s_flag;
 
-    const int sb = blockIdx.x;
+    // XCD-aware: workgroup i runs on XCD i % 8, which takes a contiguous run
+    // of superbins (a z-slab), so neighbours share their halo in one L2
+    const int sb = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (sb >= nsb) return;
     const int s0 = a.bin_start[sb * 8], s1 = a.bin_start[sb * 8 + 8];
     if (s0 == s1) return;
     const int tid = threadIdx.x;
@@ -412,13 +447,21 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
             const int4 c4 = a.cells[k];
             const int cc[3] = {c4.x, c4.y, c4.z};
             int pid = -1;
-            for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && pid < 0; ++j)
+            if (a.npatches == 1)
             {
-                const PatchDev& Q = a.patches[a.bin_patches[j]];
+                const PatchDev& Q = a.patches[0];
                 if (cc[0] >= Q.ilo[0] && cc[0] <= Q.ihi[0] && cc[1] >= Q.ilo[1] && cc[1] <= Q.ihi[1] &&
                     cc[2] >= Q.ilo[2] && cc[2] <= Q.ihi[2])
-                    pid = a.bin_patches[j];
+                    pid = 0;
             }
+            else
+                for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && pid < 0; ++j)
+                {
+                    const PatchDev& Q = a.patches[a.bin_patches[j]];
+                    if (cc[0] >= Q.ilo[0] && cc[0] <= Q.ihi[0] && cc[1] >= Q.ilo[1] && cc[1] <= Q.ihi[1] &&
+                        cc[2] >= Q.ilo[2] && cc[2] <= Q.ihi[2])
+                        pid = a.bin_patches[j];
+                }
             if (pid < 0) continue;
             if (lpid >= 0 && lpid != pid) lflag |= 1;
             lpid = pid;
@@ -491,13 +534,21 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
             {
                 const int lo0 = s_lo[0], lo1 = s_lo[1], lo2 = s_lo[2];
                 const int nrow = e1 * e2;
-                for (int idx = tid; idx < e0 * nrow; idx += I3_THREADS)
+                // rows of the block: 32 threads per row, 8 rows at a time (no divisions per element)
+                const int c = tid & 31;
+                int r = tid >> 5;
+                int i1 = r % e1, i2 = r / e1;
+                for (; r < nrow; r += I3_THREADS / 32)
                 {
-                    const int i0 = idx % e0;
-                    const int r = idx / e0;
-                    const int i1 = r % e1, i2 = r / e1;
-                    s_u[idx] = u[(size_t)(lo0 + i0 - alo[0]) + (size_t)an[0] * (lo1 + i1 - alo[1]) +
-                                 plane * (size_t)(lo2 + i2 - alo[2])];
+                    const T* src = u + (size_t)(lo0 - alo[0]) + (size_t)an[0] * (lo1 + i1 - alo[1]) +
+                                   plane * (size_t)(lo2 + i2 - alo[2]);
+                    for (int i0 = c; i0 < e0; i0 += 32) s_u[r * e0 + i0] = src[i0];
+                    i1 += I3_THREADS / 32;
+                    while (i1 >= e1)
+                    {
+                        i1 -= e1;
+                        ++i2;
+                    }
                 }
                 __syncthreads();
             }

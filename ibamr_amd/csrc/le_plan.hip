@@ -43,6 +43,9 @@ static int fail(int code, const std::string& msg)
     return code;
 }
 
+// shared with the other translation units of the library
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
 #define HIP_OK(expr)                                                                          \
     do                                                                                        \
     {                                                                                         \
@@ -311,6 +314,7 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
 
     const int lane = threadIdx.x;
     const TileDesc td = a.tiles[blockIdx.x];
+    if (td.patch < 0) return;  // padding of the XCD-ordered work list
     const PatchDev P = a.patches[td.patch];
     const int axis = a.side ? td.comp : -1;
 
@@ -962,6 +966,20 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
                     for (int tx = 0; tx < nt[0]; ++tx) tiles.push_back(TileDesc{i, c, {tx, ty, tz}});
         }
     }
+    // XCD-aware launch order: workgroup i runs on XCD i % 8, so give every XCD a
+    // contiguous run of the natural (patch, array, z, y, x) order; neighbouring
+    // tiles then share point data and output lines in the same L2.
+    {
+        const size_t n = tiles.size();
+        const size_t per = (n + 7) / 8;
+        std::vector<TileDesc> ord(8 * per, TileDesc{-1, 0, {0, 0, 0}});
+        for (size_t i = 0; i < ord.size(); ++i)
+        {
+            const size_t nat = (i % 8) * per + i / 8;
+            if (nat < n) ord[i] = tiles[nat];
+        }
+        tiles.swap(ord);
+    }
     P->ntiles = (int)tiles.size();
 
     auto cleanup_fail = [&](int code) {
@@ -1196,7 +1214,8 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
             HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp3<KER, T>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
             const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
-            hipLaunchKernelGGL((k_interp3<KER, T>), dim3(nsb), dim3(I3_THREADS), bytes, s, a, elems);
+            const int per = (nsb + 7) / 8;  // superbins per XCD (see k_interp3)
+            hipLaunchKernelGGL((k_interp3<KER, T>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems, nsb, per);
         }
         else
         {
