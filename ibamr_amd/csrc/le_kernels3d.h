@@ -29,7 +29,7 @@
 namespace
 {
 constexpr int S3_ZC = 32;    // z values per spread tile (registers per lane)
-constexpr int S3_PAD = 25;   // staged weight row: W-1 zeros, 8 lanes, W-1 zeros (odd stride)
+constexpr int S3_PAD = 9;    // staged weight row: the 8 tile lanes (odd stride: no bank conflicts)
 constexpr int S3_MAXCOMB = 64;
 
 template <int KER, typename T, bool MERGE>
@@ -142,13 +142,21 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
         }
     }
 
-    // total number of points in the visited bins; an empty tile keeps its values
-    int total = 0;
+    // total number of points in the visited bins; an empty tile keeps its values.
+    // The lane's combination of the first group is kept for the chunk loop.
+    int total = 0, g0_st = 0, g0_ln = 0, g0_oc = 0;
     for (int c = lane; c < ncomb; c += 64)
     {
         const int kx = c % nx, ky = (c / nx) % ny, kz = c / (nx * ny);
         const int m = bin_mem(3, s_lb[0][kx], s_lb[1][ky], s_lb[2][kz], a.nb, a.nsb);
-        total += a.bin_start[m + 1] - a.bin_start[m];
+        const int st = a.bin_start[m], ln = a.bin_start[m + 1] - st;
+        total += ln;
+        if (c == lane)
+        {
+            g0_st = st;
+            g0_ln = ln;
+            g0_oc = (s_lo[0][kx] + 1) + 3 * (s_lo[1][ky] + 1) + 9 * (s_lo[2][kz] + 1);
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off);
@@ -167,8 +175,8 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
         // ---- this group of combinations: ranges and their prefix (in canonical order)
         {
             const int c = cb + lane;
-            int st = 0, ln = 0, oc = 0;
-            if (lane < S3_MAXCOMB && c < ncomb)
+            int st = cb == 0 ? g0_st : 0, ln = cb == 0 ? g0_ln : 0, oc = cb == 0 ? g0_oc : 0;
+            if (cb > 0 && lane < S3_MAXCOMB && c < ncomb)
             {
                 const int kx = c % nx, ky = (c / nx) % ny, kz = c / (nx * ny);
                 const int m = bin_mem(3, s_lb[0][kx], s_lb[1][ky], s_lb[2][kz], a.nb, a.nsb);
@@ -301,11 +309,11 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                     const int dx0 = bse[0] - flo[0], dy0 = bse[1] - flo[1], zz = bse[2] - flo[2];
                     if (dx0 > -W && dx0 < 8 && dy0 > -W && dy0 < 8 && zz > -W && zz < ZC)
                     {
+                        // rows hold the weight of each of the tile's 8 x (y) lanes, zero outside the stencil
                         T* rx = s_rx + lane * S3_PAD;
                         T* ry = s_ry + lane * S3_PAD;
-                        // only positions W-1 .. W+6 (the 8 lanes) are ever read
 #pragma unroll
-                        for (int i = W - 1; i < W + 7; ++i)
+                        for (int i = 0; i < 8; ++i)
                         {
                             rx[i] = T(0);
                             ry[i] = T(0);
@@ -314,8 +322,8 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
 #pragma unroll
                         for (int j = 0; j < W; ++j)
                         {
-                            rx[dx0 + W - 1 + j] = FP64 ? w[0][j] : w[0][j] * fx;
-                            ry[dy0 + W - 1 + j] = w[1][j];
+                            if ((unsigned)(dx0 + j) < 8u) rx[dx0 + j] = FP64 ? w[0][j] : w[0][j] * fx;
+                            if ((unsigned)(dy0 + j) < 8u) ry[dy0 + j] = w[1][j];
                             // FP64 IB folds 1/vol into w2 exactly as the reference (f.m4:1435)
                             s_wz[lane * 8 + j] = (FP64 && KT::IB) ? w[2][j] / vol
                                                                   : ((!FP64 && KT::IB) ? w[2][j] * inv_vol : w[2][j]);
@@ -337,32 +345,57 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                     unsigned long long m = __ballot(my_zz == zz);
                     while (m)
                     {
+                        // two points per step so their LDS reads overlap
                         const int j = __builtin_ctzll(m);
                         m &= m - 1;
-                        const T wx = s_rx[j * S3_PAD + x + W - 1];
-                        const T wy = s_ry[j * S3_PAD + y + W - 1];
-                        T wz[W];
+                        const int j2 = m ? __builtin_ctzll(m) : j;
+                        const bool two = m != 0;
+                        if (two) m &= m - 1;
+                        const T wx = s_rx[j * S3_PAD + x];
+                        const T wy = s_ry[j * S3_PAD + y];
+                        const T wx2 = s_rx[j2 * S3_PAD + x];
+                        const T wy2 = s_ry[j2 * S3_PAD + y];
+                        T wz[W], wz2[W];
 #pragma unroll
-                        for (int i = 0; i < W; ++i) wz[i] = s_wz[j * 8 + i];
+                        for (int i = 0; i < W; ++i)
+                        {
+                            wz[i] = s_wz[j * 8 + i];
+                            wz2[i] = s_wz[j2 * 8 + i];
+                        }
                         if (!FP64)
                         {
                             const T b = wx * wy;
+                            const T b2 = two ? wx2 * wy2 : T(0);
 #pragma unroll
                             for (int i = 0; i < W; ++i)
-                                if (zz + i >= 0 && zz + i < ZC) part[zz + i] = fmaf(b, wz[i], part[zz + i]);
+                                if (zz + i >= 0 && zz + i < ZC)
+                                    part[zz + i] = fmaf(b2, wz2[i], fmaf(b, wz[i], part[zz + i]));
                         }
                         else
                         {
                             const bool inside = (unsigned)(x - s_dx0[j]) < (unsigned)W &&
                                                 (unsigned)(y - s_dy0[j]) < (unsigned)W;
+                            const bool inside2 = two && (unsigned)(x - s_dx0[j2]) < (unsigned)W &&
+                                                 (unsigned)(y - s_dy0[j2]) < (unsigned)W;
+                            const T Fj = s_F[j], Fj2 = s_F[j2];
                             if (inside)
                             {
-                                const T Fj = s_F[j];
 #pragma unroll
                                 for (int i = 0; i < W; ++i)
                                     if (zz + i >= 0 && zz + i < ZC)
                                     {
                                         const T c = KT::IB ? (wx * (wy * wz[i])) * Fj : (((wx * wy) * wz[i]) * Fj) / vol;
+                                        acc[zz + i] = acc[zz + i] + c;
+                                    }
+                            }
+                            if (inside2)
+                            {
+#pragma unroll
+                                for (int i = 0; i < W; ++i)
+                                    if (zz + i >= 0 && zz + i < ZC)
+                                    {
+                                        const T c = KT::IB ? (wx2 * (wy2 * wz2[i])) * Fj2
+                                                           : (((wx2 * wy2) * wz2[i]) * Fj2) / vol;
                                         acc[zz + i] = acc[zz + i] + c;
                                     }
                             }
@@ -534,20 +567,37 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
             {
                 const int lo0 = s_lo[0], lo1 = s_lo[1], lo2 = s_lo[2];
                 const int nrow = e1 * e2;
-                // rows of the block: 32 threads per row, 8 rows at a time (no divisions per element)
-                const int c = tid & 31;
-                int r = tid >> 5;
-                int i1 = r % e1, i2 = r / e1;
-                for (; r < nrow; r += I3_THREADS / 32)
+                // element e of the block -> (i0, row); every thread issues 8
+                // independent loads before its LDS stores, so the memory
+                // latency is paid once per 8 elements, not per element
+                const int total = e0 * nrow;
+                const float inv_e0 = 1.0f / (float)e0, inv_e1 = 1.0f / (float)e1;
+                for (int base = tid; base < total; base += 8 * I3_THREADS)
                 {
-                    const T* src = u + (size_t)(lo0 - alo[0]) + (size_t)an[0] * (lo1 + i1 - alo[1]) +
-                                   plane * (size_t)(lo2 + i2 - alo[2]);
-                    for (int i0 = c; i0 < e0; i0 += 32) s_u[r * e0 + i0] = src[i0];
-                    i1 += I3_THREADS / 32;
-                    while (i1 >= e1)
+                    T v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
                     {
-                        i1 -= e1;
-                        ++i2;
+                        const int e = base + q * I3_THREADS;
+                        if (e < total)
+                        {
+                            int r = (int)((float)e * inv_e0);       // exact for e < 2^24 after the fix-up
+                            r -= (r * e0 > e);
+                            r += ((r + 1) * e0 <= e);
+                            const int i0 = e - r * e0;
+                            int i2 = (int)((float)r * inv_e1);
+                            i2 -= (i2 * e1 > r);
+                            i2 += ((i2 + 1) * e1 <= r);
+                            const int i1 = r - i2 * e1;
+                            v[q] = u[(size_t)(lo0 + i0 - alo[0]) + (size_t)an[0] * (lo1 + i1 - alo[1]) +
+                                     plane * (size_t)(lo2 + i2 - alo[2])];
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                    {
+                        const int e = base + q * I3_THREADS;
+                        if (e < total) s_u[e] = v[q];
                     }
                 }
                 __syncthreads();

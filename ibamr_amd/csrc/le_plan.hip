@@ -25,7 +25,6 @@
 #include "le_weights.h"
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -150,16 +149,16 @@ struct ibamr_plan
     void** arrays_d[MAX_SLOTS] = {};
     int* keys_d = nullptr;
     int* keys_sorted_d = nullptr;
-    int* idx_d = nullptr;
+    int* idx_d = nullptr;              // point indices in bin order before the per-bin sort
+    int* cursor_d = nullptr;           // [nmem+1] scatter cursors
+    int* scan_tmp_d = nullptr;         // block sums of the scan
+    int* scan_part_d = nullptr;        // [nmem+1] block-local exclusive prefixes
     int* perm_d = nullptr;
     int4* cell_d = nullptr;
     int4* cell_sorted_d = nullptr;
     double* Xs_d = nullptr;            // positions in bin order
     void* Fs_d = nullptr;              // forces in bin order
     int* bin_start_d = nullptr;        // [nmem+2]
-    void* sort_tmp_d = nullptr;
-    size_t sort_tmp_bytes = 0;
-    int sort_bits = 1;
 };
 
 // ----------------------------------------------------------------------------- binning
@@ -177,7 +176,7 @@ struct BinArgs
     int bin;
     int nmem;
     int* keys;
-    int* idx;
+    int* count;
     int4* cell;
 };
 
@@ -213,18 +212,209 @@ __global__ void k_bin_keys(BinArgs a)
         key = bin_mem(NDIM, b[0], b[1], b[2], a.nb, a.nsb);
     }
     a.keys[p] = key;
-    a.idx[p] = p;
+    atomicAdd(&a.count[key], 1);  // integer counts: exact and order independent
     a.cell[p] = make_int4(c[0], c[1], c[2], 0);
 }
 
-// bin_start[b] = first sorted position with key >= b, for b = 0..nmem+1
-__global__ void k_bin_bounds(const int* __restrict__ keys, int n, int nmem, int* __restrict__ start)
+__global__ void k_fill_int(int* __restrict__ a, int n, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    const int prev = i == 0 ? -1 : keys[i - 1];
-    const int cur = i == n ? nmem + 1 : keys[i];
-    for (int b = prev + 1; b <= cur; ++b) start[b] = i;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_copy_int(const int* __restrict__ a, int n, int* __restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// Exclusive scan of count[0..m) into start[0..m] (start[m] = total), three passes.
+constexpr int SCAN_B = 1024;
+
+__global__ __launch_bounds__(SCAN_B) void k_scan_blocks(const int* __restrict__ in, int m, int* __restrict__ out,
+                                                        int* __restrict__ bsum)
+{
+    __shared__ int s_w[SCAN_B / 64];
+    const int i = blockIdx.x * SCAN_B + threadIdx.x;
+    const int v = i < m ? in[i] : 0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const int t = __shfl_up(inc, off);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) s_w[wv] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int run = 0;
+        for (int w = 0; w < SCAN_B / 64; ++w)
+        {
+            const int t = s_w[w];
+            s_w[w] = run;
+            run += t;
+        }
+        bsum[blockIdx.x] = run;
+    }
+    __syncthreads();
+    if (i < m) out[i] = inc - v + s_w[wv];
+}
+
+__global__ __launch_bounds__(SCAN_B) void k_scan_sums(int* __restrict__ bsum, int nb)
+{
+    // one block: exclusive scan of the block sums, SCAN_B at a time
+    __shared__ int s_w[SCAN_B / 64];
+    __shared__ int s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int base = 0; base < nb; base += SCAN_B)
+    {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? bsum[i] : 0;
+        int inc = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            const int t = __shfl_up(inc, off);
+            if (lane >= off) inc += t;
+        }
+        if (lane == 63) s_w[wv] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            int run = s_carry;
+            for (int w = 0; w < SCAN_B / 64; ++w)
+            {
+                const int t = s_w[w];
+                s_w[w] = run;
+                run += t;
+            }
+            s_carry = run;
+        }
+        __syncthreads();
+        if (i < nb) bsum[i] = inc - v + s_w[wv];
+        __syncthreads();
+    }
+}
+
+__global__ void k_scan_add(const int* __restrict__ part, int m, const int* __restrict__ bsum, const int* __restrict__ in,
+                           int* __restrict__ start)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int v = part[i] + bsum[i / SCAN_B];
+    start[i] = v;
+    if (i == m - 1) start[m] = v + in[i];
+}
+
+// Scatter point indices into their bins (slot order inside a bin is arbitrary here).
+__global__ void k_bin_scatter(const int* __restrict__ keys, int n, int* __restrict__ cursor, int* __restrict__ idx)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    idx[atomicAdd(&cursor[keys[p]], 1)] = p;
+}
+
+// Sorts every bin's point indices ascending (unique values, so the result is
+// unique and deterministic) and writes the bin slot of every sorted position.
+// Runs of up to BS_CHUNK are sorted in LDS (bitonic); longer bins are then
+// merged pairwise in global memory (ping-pong between idx and perm).
+constexpr int BS_THREADS = 256;
+constexpr int BS_CHUNK = 4096;
+
+__global__ __launch_bounds__(BS_THREADS) void k_bin_sort(const int* __restrict__ start, int nbins,
+                                                         int* __restrict__ idx, int* __restrict__ perm,
+                                                         int* __restrict__ keys_sorted)
+{
+    __shared__ int s_v[BS_CHUNK];
+    const int b = blockIdx.x;
+    if (b >= nbins) return;
+    const int s0 = start[b], s1 = start[b + 1];
+    const int n = s1 - s0;
+    if (n == 0) return;
+    const int t = threadIdx.x;
+    for (int i = t; i < n; i += BS_THREADS) keys_sorted[s0 + i] = b;
+    if (n == 1)
+    {
+        if (t == 0) perm[s0] = idx[s0];
+        return;
+    }
+    // 1. sort chunks of BS_CHUNK in LDS; result stays in idx
+    for (int c0 = 0; c0 < n; c0 += BS_CHUNK)
+    {
+        const int cn = min(BS_CHUNK, n - c0);
+        int P = 1;
+        while (P < cn) P <<= 1;
+        for (int i = t; i < P; i += BS_THREADS) s_v[i] = i < cn ? idx[s0 + c0 + i] : 0x7fffffff;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1)
+            {
+                for (int i = t; i < P; i += BS_THREADS)
+                {
+                    const int l = i ^ j;
+                    if (l > i)
+                    {
+                        const int a = s_v[i], c = s_v[l];
+                        const bool up = (i & k) == 0;
+                        if ((a > c) == up)
+                        {
+                            s_v[i] = c;
+                            s_v[l] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int i = t; i < cn; i += BS_THREADS) idx[s0 + c0 + i] = s_v[i];
+        __syncthreads();
+    }
+    // 2. merge sorted runs: element of run A at a goes to a + |{y in B : y < x}|, and symmetrically
+    int* src = idx + s0;
+    int* dst = perm + s0;
+    for (int wdt = BS_CHUNK; wdt < n; wdt <<= 1)
+    {
+        for (int i = t; i < n; i += BS_THREADS)
+        {
+            const int pair = i / (2 * wdt);
+            const int a0 = pair * 2 * wdt;
+            const int b0 = min(a0 + wdt, n), b1 = min(a0 + 2 * wdt, n);
+            const int x = src[i];
+            int lo, hi, off;
+            if (i < b0)
+            {
+                lo = b0;
+                hi = b1;
+                off = i - a0;
+            }
+            else
+            {
+                lo = a0;
+                hi = b0;
+                off = i - b0;
+            }
+            int cnt = 0, L = lo, H = hi;  // elements < x in the other run
+            while (L < H)
+            {
+                const int mid = (L + H) >> 1;
+                if (src[mid] < x)
+                    L = mid + 1;
+                else
+                    H = mid;
+            }
+            cnt = L - lo;
+            dst[a0 + off + cnt] = x;
+        }
+        __syncthreads();
+        int* tmp = src;
+        src = dst;
+        dst = tmp;
+    }
+    if (src != perm + s0)
+        for (int i = t; i < n; i += BS_THREADS) perm[s0 + i] = src[i];
 }
 
 __global__ void k_gather_cells(const int* __restrict__ perm, const int4* __restrict__ cell, int n,
@@ -816,7 +1006,9 @@ int ibamr_plan_destroy(ibamr_plan* P)
     dfree(P->Xs_d);
     dfree(P->Fs_d);
     dfree(P->bin_start_d);
-    dfree(P->sort_tmp_d);
+    dfree(P->cursor_d);
+    dfree(P->scan_tmp_d);
+    dfree(P->scan_part_d);
     delete P;
     return IBAMR_OK;
 }
@@ -1018,13 +1210,9 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
     PLAN_HIP(hipMalloc(&P->Xs_d, sizeof(double) * ndim * np));
     PLAN_HIP(hipMalloc(&P->Fs_d, (C->precision == IBAMR_FP64 ? 8 : 4) * (size_t)P->fdepth * np));
     PLAN_HIP(hipMalloc(&P->bin_start_d, sizeof(int) * (P->nmem + 2)));
-    int bits = 1;
-    while ((1L << bits) <= (long)P->nmem) ++bits;  // keys 0..nmem
-    P->sort_bits = bits;
-    PLAN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, P->sort_tmp_bytes, reinterpret_cast<unsigned*>(P->keys_d),
-                                                reinterpret_cast<unsigned*>(P->keys_sorted_d), P->idx_d, P->perm_d,
-                                                (int)np, 0, bits));
-    PLAN_HIP(hipMalloc(&P->sort_tmp_d, std::max<size_t>(16, P->sort_tmp_bytes)));
+    PLAN_HIP(hipMalloc(&P->cursor_d, sizeof(int) * (P->nmem + 1)));
+    PLAN_HIP(hipMalloc(&P->scan_tmp_d, sizeof(int) * ((P->nmem + 1 + SCAN_B - 1) / SCAN_B + 1)));
+    PLAN_HIP(hipMalloc(&P->scan_part_d, sizeof(int) * (P->nmem + 1)));
 #undef PLAN_HIP
     *out = P;
     return IBAMR_OK;
@@ -1059,9 +1247,13 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     if (npoints > 0 && !X) return fail(IBAMR_ERR_INVALID, "null X");
     hipStream_t s = static_cast<hipStream_t>(stream);
     P->npoints = npoints;
+    const int B = 256;
+    const int m = P->nmem + 1;  // bins plus the out-of-domain sentinel
     if (npoints == 0)
     {
-        HIP_OK(hipMemsetAsync(P->bin_start_d, 0, sizeof(int) * (P->nmem + 2), s));
+        // a fill kernel, not a memset: safe to capture in a HIP graph
+        hipLaunchKernelGGL(k_fill_int, dim3((m + 1 + B - 1) / B), dim3(B), 0, s, P->bin_start_d, m + 1, 0);
+        HIP_OK(hipGetLastError());
         return IBAMR_OK;
     }
     BinArgs a{};
@@ -1080,21 +1272,29 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     a.bin = P->bin;
     a.nmem = P->nmem;
     a.keys = P->keys_d;
-    a.idx = P->idx_d;
+    a.count = P->cursor_d;
     a.cell = P->cell_d;
-    const int B = 256;
     const int G = (npoints + B - 1) / B;
+    // counting sort by bin slot: count, exclusive scan, scatter, then sort each
+    // bin's point indices so the order inside a bin is the index order
+    hipLaunchKernelGGL(k_fill_int, dim3((m + B - 1) / B), dim3(B), 0, s, P->cursor_d, m, 0);
     if (P->ndim == 3)
         hipLaunchKernelGGL(k_bin_keys<3>, dim3(G), dim3(B), 0, s, a);
     else
         hipLaunchKernelGGL(k_bin_keys<2>, dim3(G), dim3(B), 0, s, a);
     HIP_OK(hipGetLastError());
-    size_t tb = P->sort_tmp_bytes;
-    HIP_OK(hipcub::DeviceRadixSort::SortPairs(P->sort_tmp_d, tb, reinterpret_cast<unsigned*>(P->keys_d),
-                                              reinterpret_cast<unsigned*>(P->keys_sorted_d), P->idx_d, P->perm_d,
-                                              npoints, 0, P->sort_bits, s));
-    hipLaunchKernelGGL(k_bin_bounds, dim3((npoints + 1 + B - 1) / B), dim3(B), 0, s, P->keys_sorted_d, npoints,
-                       P->nmem, P->bin_start_d);
+    const int nsb = (m + SCAN_B - 1) / SCAN_B;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nsb), dim3(SCAN_B), 0, s, P->cursor_d, m, P->scan_part_d,
+                       P->scan_tmp_d);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, s, P->scan_tmp_d, nsb);
+    hipLaunchKernelGGL(k_scan_add, dim3((m + B - 1) / B), dim3(B), 0, s, P->scan_part_d, m, P->scan_tmp_d,
+                       P->cursor_d, P->bin_start_d);
+    HIP_OK(hipGetLastError());
+    // cursors start at the bin offsets
+    hipLaunchKernelGGL(k_copy_int, dim3((m + B - 1) / B), dim3(B), 0, s, P->bin_start_d, m, P->cursor_d);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(G), dim3(B), 0, s, P->keys_d, npoints, P->cursor_d, P->idx_d);
+    hipLaunchKernelGGL(k_bin_sort, dim3(m), dim3(BS_THREADS), 0, s, P->bin_start_d, m, P->idx_d, P->perm_d,
+                       P->keys_sorted_d);
     HIP_OK(hipGetLastError());
     hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d);
     HIP_OK(hipGetLastError());
