@@ -40,7 +40,36 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=400000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of 2 GiB (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(torch):
+    """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE of this
+    dispatch divided by 2 GiB is the counter scale factor for the traffic figures."""
+    nbytes = 2 << 30
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "dispatches": 3}), flush=True)
+
+
+def traffic_from_profile(workload_tag):
+    """HBM bytes per dispatch of the dominant kernel, from the committed
+    rocprofv3 --pmc summary of this workload (FETCH_SIZE and WRITE_SIZE, each
+    scaled by the same counter on a plain copy of known size).  None if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s.txt" % workload_tag)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("TRAFFIC_JSON "):
+                return json.loads(line[len("TRAFFIC_JSON "):])
+    return None
 
 
 def make_points(torch, n, N, seed):
@@ -130,15 +159,19 @@ def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
         step()
     torch.cuda.synchronize()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    plan.set_timing(True)   # HIP events on stream s right around the two main kernels
     t0 = time.perf_counter()
     for k in range(steps):
         step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    kt = plan.kernel_times()
+    plan.set_timing(False)
     sp = [e[0].elapsed_time(e[1]) for e in evs]
     it = [e[1].elapsed_time(e[2]) for e in evs]
     return {"plan": plan, "f": f, "u": u, "U": U, "wall_s": wall, "spread_ms": sum(sp) / len(sp),
-            "interp_ms": sum(it) / len(it), "bin_ms": bin_ms, "info": plan.info()}
+            "interp_ms": sum(it) / len(it), "bin_ms": bin_ms, "info": plan.info(),
+            "spread_kernel_ms": kt["spread_ms"], "interp_kernel_ms": kt["interp_ms"]}
 
 
 def cpu_baseline(X_np, F_np, N, kernel, sample, seed=5):
@@ -178,6 +211,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if args.copy_probe:
+        copy_probe(torch)
+        return
     import ibamr_amd as ia
 
     ia.lib()
@@ -190,10 +227,13 @@ def main():
     d, sx, s32 = 3, 8, 4
     B_spread = n * (sx * d + s32 * d) + s32 * T
     B_interp = n * (sx * d + s32 * d) + s32 * T
-    dom = "spread" if r32["spread_ms"] >= r32["interp_ms"] else "interp"
+    dom = "spread" if r32["spread_kernel_ms"] >= r32["interp_kernel_ms"] else "interp"
     B_dom = B_spread if dom == "spread" else B_interp
-    t_dom = r32["spread_ms"] if dom == "spread" else r32["interp_ms"]
+    t_dom = r32["spread_kernel_ms"] if dom == "spread" else r32["interp_kernel_ms"]
     achieved = B_dom / (t_dom * 1e-3) / 1e9
+    kname = "k_spread3" if dom == "spread" else "k_interp3"
+    tag = "%s_%dc_side_%s_%dpts_fp32c" % (kname, N, kernel.lower(), n)
+    tr = traffic_from_profile(tag)
 
     out = {
         "metric": METRIC,
@@ -212,10 +252,12 @@ def main():
                                "r=0.3), FP32 compensated spread+interp" % (N, kernel, n),
                    "grid": N, "points": n, "kernel": kernel, "centering": "side", "precision": "fp32c",
                    "tile": r32["info"]["tile"], "spread_tiles": r32["info"]["ntiles"]},
-        "roofline": {"bound": "hbm", "kernel": "k_spread_tiles" if dom == "spread" else "k_interp",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": B_dom},
+                     "traffic": tr["bytes_per_dispatch"] if tr else None, "algorithmic_bytes": B_dom,
+                     "traffic_source": ("profiles/hbm_traffic_%s.txt" % tag) if tr else None},
         "detail": {"spread_ms": r32["spread_ms"], "interp_ms": r32["interp_ms"], "bin_ms": r32["bin_ms"],
+                   "spread_kernel_ms": r32["spread_kernel_ms"], "interp_kernel_ms": r32["interp_kernel_ms"],
                    "touched_faces": T, "bytes_step": B_spread + B_interp,
                    "step_GBs": (B_spread + B_interp) / (ms32 * 1e-3) / 1e9},
     }

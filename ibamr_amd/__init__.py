@@ -34,14 +34,16 @@ IBAMR_ERR_NOT_BINNED = -8
 
 CELL, SIDE = 0, 1
 FP64, FP32C = 0, 1
-ACCUMULATE, OVERWRITE = 0, 1
+ACCUMULATE, OVERWRITE, ACCUMULATE_INTERIOR = 0, 1, 2
 
 # Every symbol include/ibamr_amd.h declares (checked by the CPU test suite).
 C_API = [
     "ibamr_is_known_kernel", "ibamr_get_stencil_size", "ibamr_get_minimum_ghost_width",
-    "ibamr_plan_create", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
+    "ibamr_plan_create", "ibamr_plan_create_patch", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
-    "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_info", "ibamr_last_error",
+    "ibamr_plan_fill_periodic_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
+    "ibamr_plan_info",
+    "ibamr_last_error",
     "ibamr_s0_interp", "ibamr_s0_spread",
 ] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3")
      for op in ("interp", "spread") for d in ("2d", "3d")]
@@ -89,7 +91,14 @@ def lib():
     L.ibamr_get_stencil_size.argtypes = [ctypes.c_char_p]
     L.ibamr_get_minimum_ghost_width.argtypes = [ctypes.c_char_p]
     L.ibamr_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(LevelGeom), ctypes.POINTER(PlanConfig)]
+    L.ibamr_plan_create_patch.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(PatchGeom), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(PlanConfig)]
     L.ibamr_plan_destroy.argtypes = [vp]
+    L.ibamr_plan_set_timing.argtypes = [vp, ctypes.c_int]
+    L.ibamr_forward_euler_step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_long, vp]
+    L.ibamr_plan_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     L.ibamr_plan_bind_data.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
     L.ibamr_plan_bin.argtypes = [vp, vp, ctypes.c_int, vp]
     L.ibamr_plan_bin_async.argtypes = [vp, vp, ctypes.c_int, vp]
@@ -201,10 +210,14 @@ def array_shape(level, patch_index, gcw, centering, axis=0, depth=1):
 
 
 class LevelPlan:
-    """Spread/interp plan for one level (the LDataManager::spread/interp level loop)."""
+    """Spread/interp plan for one level (the LDataManager::spread/interp level loop).
+
+    With box=(lower, upper) the level must hold one patch, and the plan is the
+    raw-array LEInteractor form instead (ibamr_plan_create_patch): no periodic
+    images, points taken when their cell (getCellIndex on the patch) lies in box."""
 
     def __init__(self, level, kernel="IB_4", centering=SIDE, precision=FP64, depth=1, data_gcw=None,
-                 idx_gcw=0, max_points=0, drift_cells=0):
+                 idx_gcw=0, max_points=0, drift_cells=0, box=None):
         self.level = level
         self.kernel = kernel
         self.centering = centering
@@ -225,7 +238,15 @@ class LevelPlan:
         cfg.drift_cells = drift_cells
         cfg.tile = 0
         h = ctypes.c_void_p()
-        _check(lib().ibamr_plan_create(ctypes.byref(h), ctypes.byref(Lc), ctypes.byref(cfg)))
+        if box is None:
+            _check(lib().ibamr_plan_create(ctypes.byref(h), ctypes.byref(Lc), ctypes.byref(cfg)))
+        else:
+            assert len(level.patches) == 1, "a box plan covers one patch"
+            lo = (ctypes.c_int * 3)(*(list(box[0]) + [0] * (3 - level.ndim)))
+            hi = (ctypes.c_int * 3)(*(list(box[1]) + [0] * (3 - level.ndim)))
+            dx = (ctypes.c_double * 3)(*(level.dx + [1.0] * (3 - level.ndim)))
+            _check(lib().ibamr_plan_create_patch(ctypes.byref(h), level.ndim, dx, self._pg, lo, hi,
+                                                 ctypes.byref(cfg)))
         self._h = h
         self._bound = {}
 
@@ -271,6 +292,27 @@ class LevelPlan:
 
     def fill_periodic_ghosts(self, slot, stream=None):
         _check(lib().ibamr_plan_fill_periodic_ghosts(self._h, slot, _stream_ptr(stream)))
+
+    def set_timing(self, on=True):
+        """HIP events around each spread / interpolation kernel launch (measurement only)."""
+        _check(lib().ibamr_plan_set_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        """Mean kernel durations (ms) and launch counts since the last call."""
+        sm, im = ctypes.c_double(), ctypes.c_double()
+        ns, ni = ctypes.c_int(), ctypes.c_int()
+        _check(lib().ibamr_plan_kernel_times(self._h, ctypes.byref(sm), ctypes.byref(ns), ctypes.byref(im),
+                                             ctypes.byref(ni)))
+        return {"spread_ms": sm.value, "nspread": ns.value, "interp_ms": im.value, "ninterp": ni.value}
+
+
+def forward_euler_step(X_new, X, U, dt, stream=None):
+    """IBMethod::forwardEulerStep: X_new = X + dt U (device tensors; U float64 or float32)."""
+    import torch
+    prec = FP64 if U.dtype == torch.float64 else FP32C
+    assert X.dtype == torch.float64 and X_new.dtype == torch.float64 and X.numel() == U.numel() == X_new.numel()
+    _check(lib().ibamr_forward_euler_step(_ptr(X_new), _ptr(X), _ptr(U), prec, float(dt), X.numel(),
+                                          _stream_ptr(stream)))
 
 
 # --------------------------------------------------------------------------- S0: Fortran-kernel drop-in

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lib", "libibamr_amd.so")
-SOURCES = ["le_plan.hip", "le_fortran_abi.hip"]
+SOURCES = ["le_plan.hip", "le_fortran_abi.hip", "ibtk_facade.cpp"]
 
 
 def hipcc():
@@ -46,5 +46,31 @@ def build(verbose=False):
     return OUT
 
 
+DRIVERS = {"interpolate_01": os.path.join(ROOT, "tests", "cpp", "interpolate_01.cpp")}
+DRIVER_DIR = os.path.join(ROOT, "tests", "cpp", "bin")
+
+
+def build_drivers(verbose=False):
+    """C++ test drivers on the ibtk_amd layer (host code only: g++, linked to the library)."""
+    os.makedirs(DRIVER_DIR, exist_ok=True)
+    outs = []
+    for name, src in DRIVERS.items():
+        if not os.path.exists(src):
+            continue
+        out = os.path.join(DRIVER_DIR, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", src,
+               "-L", os.path.dirname(OUT), "-libamr_amd", "-Wl,-rpath,$ORIGIN/../../../ibamr_amd/lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("g++ failed building %s" % out)
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_drivers(verbose=True))

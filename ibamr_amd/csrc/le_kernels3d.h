@@ -70,8 +70,14 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
         an[d] = ahi[d] - alo[d] + 1;
         flo[d] = alo[d] + td.t[d] * tsz[d];
         fhi[d] = min(flo[d] + tsz[d] - 1, ahi[d]);
+        if (a.mode == IBAMR_SPREAD_ACCUMULATE_INTERIOR)
+        {
+            flo[d] = max(flo[d], P.ilo[d]);
+            fhi[d] = min(fhi[d], P.ihi[d] + (d == axis ? 1 : 0));
+        }
         xl[d] = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);  // LEInteractor.cpp:3689
     }
+    if (flo[0] > fhi[0] || flo[1] > fhi[1] || flo[2] > fhi[2]) return;  // tile wholly in the ghost region
     T* __restrict__ arr = a.arrays[td.patch * a.narr + (a.side ? td.comp : 0)];
     const size_t plane = (size_t)an[0] * an[1];
     if (!a.side) arr += (size_t)td.comp * plane * an[2];
@@ -483,16 +489,16 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
             if (a.npatches == 1)
             {
                 const PatchDev& Q = a.patches[0];
-                if (cc[0] >= Q.ilo[0] && cc[0] <= Q.ihi[0] && cc[1] >= Q.ilo[1] && cc[1] <= Q.ihi[1] &&
-                    cc[2] >= Q.ilo[2] && cc[2] <= Q.ihi[2])
+                if (cc[0] >= Q.slo[0] && cc[0] <= Q.shi[0] && cc[1] >= Q.slo[1] && cc[1] <= Q.shi[1] &&
+                    cc[2] >= Q.slo[2] && cc[2] <= Q.shi[2])
                     pid = 0;
             }
             else
                 for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && pid < 0; ++j)
                 {
                     const PatchDev& Q = a.patches[a.bin_patches[j]];
-                    if (cc[0] >= Q.ilo[0] && cc[0] <= Q.ihi[0] && cc[1] >= Q.ilo[1] && cc[1] <= Q.ihi[1] &&
-                        cc[2] >= Q.ilo[2] && cc[2] <= Q.ihi[2])
+                    if (cc[0] >= Q.slo[0] && cc[0] <= Q.shi[0] && cc[1] >= Q.slo[1] && cc[1] <= Q.shi[1] &&
+                        cc[2] >= Q.slo[2] && cc[2] <= Q.shi[2])
                         pid = a.bin_patches[j];
                 }
             if (pid < 0) continue;
@@ -614,15 +620,15 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                     for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && qid < 0; ++j)
                     {
                         const PatchDev& Q = a.patches[a.bin_patches[j]];
-                        if (c4.x >= Q.ilo[0] && c4.x <= Q.ihi[0] && c4.y >= Q.ilo[1] && c4.y <= Q.ihi[1] &&
-                            c4.z >= Q.ilo[2] && c4.z <= Q.ihi[2])
+                        if (c4.x >= Q.slo[0] && c4.x <= Q.shi[0] && c4.y >= Q.slo[1] && c4.y <= Q.shi[1] &&
+                            c4.z >= Q.slo[2] && c4.z <= Q.shi[2])
                             qid = a.bin_patches[j];
                     }
                     if (qid < 0) continue;
                 }
                 const PatchDev Pq = a.patches[qid];
-                if (c4.x < Pq.ilo[0] || c4.x > Pq.ihi[0] || c4.y < Pq.ilo[1] || c4.y > Pq.ihi[1] ||
-                    c4.z < Pq.ilo[2] || c4.z > Pq.ihi[2])
+                if (c4.x < Pq.slo[0] || c4.x > Pq.shi[0] || c4.y < Pq.slo[1] || c4.y > Pq.shi[1] ||
+                    c4.z < Pq.slo[2] || c4.z > Pq.shi[2])
                     continue;
                 const int p = a.perm[k];
                 T w[3][W];

@@ -61,6 +61,8 @@ struct PatchDev
     int ilo[3];
     int ihi[3];
     double xlo[3];
+    int slo[3];   // interpolation ownership box: the patch box (level plans) or
+    int shi[3];   // the caller's box (single-patch plans)
 };
 
 struct TileDesc
@@ -70,14 +72,42 @@ struct TileDesc
     int t[3];   // tile coordinates inside the array's ghost box
 };
 
+// The reference's kernel registry (LEInteractor.cpp:2035-2108): every known
+// name and its stencil size.  Only IB_4, IB_6 and BSPLINE_3 have GPU kernels;
+// the other names are known (isKnownKernel, getStencilSize) but plans reject them.
+struct KernelName
+{
+    const char* name;
+    int stencil;
+    int id;   // K_* of the GPU kernel, or -2 (registry only)
+};
+static const KernelName k_registry[] = {
+    {"BSPLINE_3", 4, K_BS3},          {"BSPLINE_4", 4, -2},
+    {"BSPLINE_5", 6, -2},             {"BSPLINE_6", 6, -2},
+    {"COMPOSITE_BSPLINE_23", 4, -2},  {"COMPOSITE_BSPLINE_32", 4, -2},
+    {"COMPOSITE_BSPLINE_34", 4, -2},  {"COMPOSITE_BSPLINE_43", 4, -2},
+    {"COMPOSITE_BSPLINE_45", 5, -2},  {"COMPOSITE_BSPLINE_54", 5, -2},
+    {"COMPOSITE_BSPLINE_56", 6, -2},  {"COMPOSITE_BSPLINE_65", 6, -2},
+    {"DISCONTINUOUS_LINEAR", 2, -2},  {"PIECEWISE_CONSTANT", 1, -2},
+    {"PIECEWISE_LINEAR", 2, K_PWL},   {"PIECEWISE_CUBIC", 4, -2},
+    {"IB_3", 4, -2},                  {"IB_4", 4, K_IB4},
+    {"IB_4_W8", 8, -2},               {"IB_5", 6, -2},
+    {"IB_6", 6, K_IB6},               {"USER_DEFINED", 0, -2},
+};
+
+static const KernelName* find_kernel(const char* name)
+{
+    if (!name) return nullptr;
+    for (const KernelName& k : k_registry)
+        if (!std::strcmp(name, k.name)) return &k;
+    return nullptr;
+}
+
+// K_* id of a kernel with a GPU implementation; -2 known without one; -1 unknown
 static int kernel_id(const char* name)
 {
-    if (!name) return -1;
-    if (!std::strcmp(name, "IB_4")) return K_IB4;
-    if (!std::strcmp(name, "IB_6")) return K_IB6;
-    if (!std::strcmp(name, "BSPLINE_3")) return K_BS3;
-    if (!std::strcmp(name, "PIECEWISE_LINEAR")) return K_PWL;
-    return -1;
+    const KernelName* k = find_kernel(name);
+    return k ? k->id : -1;
 }
 
 static int stencil_size(int k)
@@ -128,8 +158,10 @@ struct ibamr_plan
     int bin = 8;
     int drift = 0;
     double dx[3] = {1, 1, 1};
-    int dom_lo[3] = {0, 0, 0}, dom_hi[3] = {0, 0, 0};
+    int dom_lo[3] = {0, 0, 0}, dom_hi[3] = {0, 0, 0};   // bin grid box
     double dom_xlo[3] = {0, 0, 0}, dom_xhi[3] = {0, 0, 0};
+    int ref_lo[3] = {0, 0, 0}, ref_hi[3] = {0, 0, 0};   // box of the getCellIndex formula (x range dom_x*)
+    int sel_lo[3] = {0, 0, 0}, sel_hi[3] = {0, 0, 0};   // points kept: cell inside this box
     int pshift[3] = {0, 0, 0};
     int nb[3] = {1, 1, 1};
     int nsb[3] = {1, 1, 1};
@@ -159,6 +191,12 @@ struct ibamr_plan
     double* Xs_d = nullptr;            // positions in bin order
     void* Fs_d = nullptr;              // forces in bin order
     int* bin_start_d = nullptr;        // [nmem+2]
+
+    // optional timing of the main spread / interpolation kernels: event pairs
+    // recorded on the launch stream around each launch (0: spread, 1: interp)
+    bool timing = false;
+    std::vector<hipEvent_t> tev[2];
+    int tcount[2] = {0, 0};
 };
 
 // ----------------------------------------------------------------------------- binning
@@ -170,6 +208,8 @@ struct BinArgs
     const double* X;
     int n;
     double xlo[3], xhi[3], dx[3];
+    int ref_lo[3], ref_hi[3];
+    int sel_lo[3], sel_hi[3];
     int dom_lo[3], dom_hi[3];
     int nb[3];
     int nsb[3];
@@ -196,11 +236,11 @@ __global__ void k_bin_keys(BinArgs a)
         const double dXl = X - a.xlo[d], dXu = X - a.xhi[d];
         double v;
         if (fabs(dXl) <= fabs(dXu))
-            v = (double)a.dom_lo[d] + floor(dXl / a.dx[d]);
+            v = (double)a.ref_lo[d] + floor(dXl / a.dx[d]);
         else
-            v = ((double)a.dom_hi[d] + floor(dXu / a.dx[d])) + 1.0;
-        // NaN or far outside: not inside the domain
-        inside = inside && (v >= (double)a.dom_lo[d]) && (v <= (double)a.dom_hi[d]);
+            v = ((double)a.ref_hi[d] + floor(dXu / a.dx[d])) + 1.0;
+        // NaN or far outside: not selected
+        inside = inside && (v >= (double)a.sel_lo[d]) && (v <= (double)a.sel_hi[d]);
         c[d] = inside ? (int)v : a.dom_lo[d];
     }
     int key = a.nmem;  // sentinel: sorts after every bin
@@ -520,9 +560,15 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
         an[d] = ahi[d] - alo[d] + 1;
         flo[d] = alo[d] + td.t[d] * TILE;
         fhi[d] = min(flo[d] + TILE - 1, ahi[d]);
+        if (a.mode == IBAMR_SPREAD_ACCUMULATE_INTERIOR)
+        {
+            flo[d] = max(flo[d], P.ilo[d]);
+            fhi[d] = min(fhi[d], P.ihi[d] + (d == axis ? 1 : 0));
+        }
         fn[d] = fhi[d] - flo[d] + 1;
         xl[d] = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);  // LEInteractor.cpp:3689
     }
+    if (fn[0] < 1 || fn[1] < 1 || fn[2] < 1) return;  // tile wholly in the ghost region
     T* __restrict__ arr = a.arrays[td.patch * a.narr + (a.side ? td.comp : 0)];
     const size_t comp_off = a.side ? 0 : (size_t)td.comp * an[0] * an[1] * (NDIM == 3 ? an[2] : 1);
     const int fcomp = td.comp;
@@ -794,7 +840,7 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a)
             const PatchDev& Q = a.patches[a.bin_patches[j]];
             bool in = true;
 #pragma unroll
-            for (int d = 0; d < NDIM; ++d) in = in && cc[d] >= Q.ilo[d] && cc[d] <= Q.ihi[d];
+            for (int d = 0; d < NDIM; ++d) in = in && cc[d] >= Q.slo[d] && cc[d] <= Q.shi[d];
             if (in) pid = a.bin_patches[j];
         }
     }
@@ -969,29 +1015,43 @@ void dfree(void* p)
 }
 } // namespace
 
+// IBMethod::forwardEulerStep position update (IBMethod.cpp:713-739, a VecWAXPY):
+// X_new = X + dt U, elementwise over n values; U in FP64 or FP32 (FP32C plans).
+template <typename TU>
+__global__ void k_euler(double* __restrict__ Xn, const double* __restrict__ X, const TU* __restrict__ U, double dt,
+                        long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Xn[i] = X[i] + dt * (double)U[i];
+}
+
 extern "C" {
 
 const char* ibamr_last_error(void) { return ibamr::g_err.c_str(); }
 
-int ibamr_is_known_kernel(const char* k) { return kernel_id(k) >= 0 ? 1 : 0; }
+int ibamr_is_known_kernel(const char* k) { return find_kernel(k) ? 1 : 0; }
 
+// USER_DEFINED reports the stencil size of the user's kernel in the reference
+// (LEInteractor.cpp:2100-2101); no user kernel is registered here, so it reports 0.
 int ibamr_get_stencil_size(const char* k)
 {
-    const int id = kernel_id(k);
-    return id < 0 ? fail(IBAMR_ERR_BAD_KERNEL, std::string("unknown kernel function ") + (k ? k : "(null)"))
-                  : stencil_size(id);
+    const KernelName* e = find_kernel(k);
+    if (!e) return fail(IBAMR_ERR_BAD_KERNEL, std::string("LEInteractor::getStencilSize()\n  Unknown kernel function ") + (k ? k : "(null)"));
+    return e->stencil;
 }
 
 int ibamr_get_minimum_ghost_width(const char* k)
 {
-    const int id = kernel_id(k);
-    return id < 0 ? fail(IBAMR_ERR_BAD_KERNEL, std::string("unknown kernel function ") + (k ? k : "(null)"))
-                  : min_ghost(id);
+    const KernelName* e = find_kernel(k);
+    if (!e) return fail(IBAMR_ERR_BAD_KERNEL, std::string("LEInteractor::getStencilSize()\n  Unknown kernel function ") + (k ? k : "(null)"));
+    return e->stencil / 2 + 1;
 }
 
 int ibamr_plan_destroy(ibamr_plan* P)
 {
     if (!P) return IBAMR_OK;
+    for (auto& v : P->tev)
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
     dfree(P->patches_d);
     dfree(P->tiles_d);
     dfree(P->bin_patch_start_d);
@@ -1013,13 +1073,24 @@ int ibamr_plan_destroy(ibamr_plan* P)
     return IBAMR_OK;
 }
 
-int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_plan_config* C)
+// Single-patch selection (the raw-array LEInteractor form): getCellIndex on the
+// patch box and x range, points kept when their cell lies in sel_lo..sel_hi.
+struct PatchSelect
+{
+    int ref_lo[3], ref_hi[3];
+    int sel_lo[3], sel_hi[3];
+};
+
+static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_plan_config* C,
+                            const PatchSelect* ps)
 {
     if (!out || !L || !C) return fail(IBAMR_ERR_INVALID, "null argument");
     *out = nullptr;
     const int k = kernel_id(C->kernel);
-    if (k < 0) return fail(IBAMR_ERR_BAD_KERNEL, std::string("unknown kernel function ") + (C->kernel ? C->kernel : "(null)"));
-    if (k == K_PWL) return fail(IBAMR_ERR_BAD_KERNEL, "PIECEWISE_LINEAR has no GPU spread/interp");
+    if (k == -1) return fail(IBAMR_ERR_BAD_KERNEL, std::string("Unknown kernel function ") + (C->kernel ? C->kernel : "(null)"));
+    if (k < 0 || k == K_PWL)
+        return fail(IBAMR_ERR_BAD_KERNEL, std::string("kernel function ") + C->kernel +
+                                              " has no GPU spread/interpolation (IB_4, IB_6 and BSPLINE_3 do)");
     if (L->ndim != 2 && L->ndim != 3) return fail(IBAMR_ERR_INVALID, "ndim must be 2 or 3");
     if (C->centering != IBAMR_CELL && C->centering != IBAMR_SIDE) return fail(IBAMR_ERR_INVALID, "bad centering");
     if (C->precision != IBAMR_FP64 && C->precision != IBAMR_FP32C) return fail(IBAMR_ERR_INVALID, "bad precision");
@@ -1035,8 +1106,11 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
     if (gmin < mg)
         return fail(IBAMR_ERR_GHOST_WIDTH, "insufficient ghost cells: kernel " + std::string(C->kernel) + " needs " +
                                                std::to_string(mg) + ", data ghost width " + std::to_string(gmin));
-    const int igcw = C->idx_gcw > 0 ? C->idx_gcw : mg;
+    int igcw = C->idx_gcw > 0 ? C->idx_gcw : mg;
     if (igcw < mg) return fail(IBAMR_ERR_GHOST_WIDTH, "Lagrangian index ghost width below the kernel minimum");
+    if (ps)  // the spread must take every selected point
+        for (int d = 0; d < ndim; ++d)
+            igcw = std::max(igcw, std::max(ps->ref_lo[d] - ps->sel_lo[d], ps->sel_hi[d] - ps->ref_hi[d]));
     for (int d = 0; d < ndim; ++d)
     {
         if (!(L->dx[d] > 0.0)) return fail(IBAMR_ERR_INVALID, "dx must be positive");
@@ -1086,6 +1160,10 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
         P->dom_xhi[d] = on ? L->domain_x_upper[d] : 1.0;
         P->pshift[d] = on ? L->periodic_shift[d] : 0;
         P->nb[d] = on ? (P->dom_hi[d] - P->dom_lo[d] + P->bin) / P->bin : 1;
+        P->ref_lo[d] = ps && on ? ps->ref_lo[d] : P->dom_lo[d];
+        P->ref_hi[d] = ps && on ? ps->ref_hi[d] : P->dom_hi[d];
+        P->sel_lo[d] = ps && on ? ps->sel_lo[d] : P->dom_lo[d];
+        P->sel_hi[d] = ps && on ? ps->sel_hi[d] : P->dom_hi[d];
     }
     const long nbins_l = (long)P->nb[0] * P->nb[1] * P->nb[2];
     if (nbins_l > (1L << 26))
@@ -1111,6 +1189,8 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
             pd.ilo[d] = on ? g.ilower[d] : 0;
             pd.ihi[d] = on ? g.iupper[d] : 0;
             pd.xlo[d] = on ? g.x_lower[d] : 0.0;
+            pd.slo[d] = ps && on ? ps->sel_lo[d] : pd.ilo[d];
+            pd.shi[d] = ps && on ? ps->sel_hi[d] : pd.ihi[d];
             if (on && (g.iupper[d] < g.ilower[d] || g.ilower[d] < P->dom_lo[d] || g.iupper[d] > P->dom_hi[d]))
             {
                 delete P;
@@ -1121,8 +1201,8 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
         int bl[3] = {0, 0, 0}, bh[3] = {0, 0, 0};
         for (int d = 0; d < ndim; ++d)
         {
-            bl[d] = (pd.ilo[d] - P->dom_lo[d]) / P->bin;
-            bh[d] = (pd.ihi[d] - P->dom_lo[d]) / P->bin;
+            bl[d] = (std::max(pd.slo[d], P->dom_lo[d]) - P->dom_lo[d]) / P->bin;
+            bh[d] = (std::min(pd.shi[d], P->dom_hi[d]) - P->dom_lo[d]) / P->bin;
         }
         for (int bz = bl[2]; bz <= bh[2]; ++bz)
             for (int by = bl[1]; by <= bh[1]; ++by)
@@ -1218,6 +1298,39 @@ int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_p
     return IBAMR_OK;
 }
 
+int ibamr_plan_create(ibamr_plan** out, const ibamr_level_geom* L, const ibamr_plan_config* C)
+{
+    return plan_create_impl(out, L, C, nullptr);
+}
+
+int ibamr_plan_create_patch(ibamr_plan** out, int ndim, const double* dx, const ibamr_patch_geom* patch,
+                            const int* box_lower, const int* box_upper, const ibamr_plan_config* C)
+{
+    if (!out || !dx || !patch || !box_lower || !box_upper || !C) return fail(IBAMR_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (ndim != 2 && ndim != 3) return fail(IBAMR_ERR_INVALID, "ndim must be 2 or 3");
+    PatchSelect ps{};
+    ibamr_level_geom L{};
+    L.ndim = ndim;
+    L.npatches = 1;
+    L.patches = patch;
+    for (int d = 0; d < ndim; ++d)
+    {
+        if (box_upper[d] < box_lower[d]) return fail(IBAMR_ERR_INVALID, "empty box");
+        ps.ref_lo[d] = patch->ilower[d];
+        ps.ref_hi[d] = patch->iupper[d];
+        ps.sel_lo[d] = box_lower[d];
+        ps.sel_hi[d] = box_upper[d];
+        L.dx[d] = dx[d];
+        L.domain_lower[d] = std::min(box_lower[d], patch->ilower[d]);
+        L.domain_upper[d] = std::max(box_upper[d], patch->iupper[d]);
+        L.domain_x_lower[d] = patch->x_lower[d];
+        L.domain_x_upper[d] = patch->x_upper[d];
+        L.periodic_shift[d] = 0;
+    }
+    return plan_create_impl(out, &L, C, &ps);
+}
+
 int ibamr_plan_info(const ibamr_plan* P, int* ntiles, int* tile, int* nbins)
 {
     if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
@@ -1263,6 +1376,10 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     {
         a.xlo[d] = P->dom_xlo[d];
         a.xhi[d] = P->dom_xhi[d];
+        a.ref_lo[d] = P->ref_lo[d];
+        a.ref_hi[d] = P->ref_hi[d];
+        a.sel_lo[d] = P->sel_lo[d];
+        a.sel_hi[d] = P->sel_hi[d];
         a.dx[d] = P->dx[d];
         a.dom_lo[d] = P->dom_lo[d];
         a.dom_hi[d] = P->dom_hi[d];
@@ -1322,12 +1439,67 @@ int ibamr_plan_bin(ibamr_plan* P, const double* X, int npoints, void* stream)
     return ibamr_plan_check(P, stream);
 }
 
+// Records the start event of launch `which` when timing is on; *end receives
+// the event to record after the launch (nullptr when timing is off).
+static int time_begin(ibamr_plan* P, int which, hipStream_t s, hipEvent_t* end)
+{
+    *end = nullptr;
+    if (!P->timing) return IBAMR_OK;
+    std::vector<hipEvent_t>& v = P->tev[which];
+    const int i = P->tcount[which];
+    if ((int)v.size() < 2 * (i + 1))
+    {
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        v.push_back(e0);
+        v.push_back(e1);
+    }
+    HIP_OK(hipEventRecord(v[2 * i], s));
+    *end = v[2 * i + 1];
+    P->tcount[which] = i + 1;
+    return IBAMR_OK;
+}
+
+int ibamr_plan_set_timing(ibamr_plan* P, int on)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    P->timing = on != 0;
+    P->tcount[0] = P->tcount[1] = 0;
+    return IBAMR_OK;
+}
+
+int ibamr_plan_kernel_times(ibamr_plan* P, double* spread_ms, int* nspread, double* interp_ms, int* ninterp)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    double mean[2] = {0.0, 0.0};
+    for (int w = 0; w < 2; ++w)
+    {
+        double sum = 0.0;
+        for (int i = 0; i < P->tcount[w]; ++i)
+        {
+            HIP_OK(hipEventSynchronize(P->tev[w][2 * i + 1]));
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, P->tev[w][2 * i], P->tev[w][2 * i + 1]));
+            sum += ms;
+        }
+        mean[w] = P->tcount[w] ? sum / P->tcount[w] : 0.0;
+    }
+    if (spread_ms) *spread_ms = mean[0];
+    if (nspread) *nspread = P->tcount[0];
+    if (interp_ms) *interp_ms = mean[1];
+    if (ninterp) *ninterp = P->tcount[1];
+    P->tcount[0] = P->tcount[1] = 0;
+    return IBAMR_OK;
+}
+
 int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, int mode, void* stream)
 {
     if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
     if (slot < 0 || slot >= MAX_SLOTS || !P->arrays_d[slot]) return fail(IBAMR_ERR_INVALID, "slot not bound");
     if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
-    if (mode != IBAMR_SPREAD_ACCUMULATE && mode != IBAMR_SPREAD_OVERWRITE) return fail(IBAMR_ERR_INVALID, "bad mode");
+    if (mode != IBAMR_SPREAD_ACCUMULATE && mode != IBAMR_SPREAD_OVERWRITE && mode != IBAMR_SPREAD_ACCUMULATE_INTERIOR)
+        return fail(IBAMR_ERR_INVALID, "bad mode");
     if (P->npoints > 0 && (!F || !X)) return fail(IBAMR_ERR_INVALID, "null F or X");
     if (P->ntiles == 0) return IBAMR_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1349,6 +1521,9 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         a.F = static_cast<const T*>(P->Fs_d);
         if (!a.tiles || !a.patches || !a.arrays || !a.X || !a.F || !a.cells || !a.bin_start)
             return fail(IBAMR_ERR_INVALID, "internal: incomplete spread arguments");
+        hipEvent_t t_end;
+        const int trc = time_begin(P, 0, s, &t_end);
+        if (trc != IBAMR_OK) return trc;
         if (NDIM == 3)
         {
             // FP32C packs points across bins; FP64 keeps chunks inside bins (fixed order)
@@ -1358,6 +1533,7 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         else
             hipLaunchKernelGGL((k_spread_tiles<2, KER, T, 16>), dim3(P->ntiles), dim3(WAVE), 0, s, a);
         HIP_OK(hipGetLastError());
+        if (t_end) HIP_OK(hipEventRecord(t_end, s));
         return IBAMR_OK;
     });
 }
@@ -1406,6 +1582,7 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
         if (!a.patches || !a.bin_patch_start || !a.bin_patches || !a.arrays || !a.X || !a.U || !a.perm || !a.cells ||
             !a.keys_sorted || !a.bin_start)
             return fail(IBAMR_ERR_INVALID, "internal: incomplete interpolation arguments");
+        hipEvent_t t_end = nullptr;
         if (NDIM == 3)
         {
             // one workgroup per superbin; LDS holds one staged block of the array
@@ -1413,18 +1590,42 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
             const int elems = bytes / (int)sizeof(T);
             HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp3<KER, T>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            const int trc = time_begin(P, 1, s, &t_end);
+            if (trc != IBAMR_OK) return trc;
             const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
             const int per = (nsb + 7) / 8;  // superbins per XCD (see k_interp3)
             hipLaunchKernelGGL((k_interp3<KER, T>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems, nsb, per);
         }
         else
         {
+            const int trc = time_begin(P, 1, s, &t_end);
+            if (trc != IBAMR_OK) return trc;
             const int B = 256;
             hipLaunchKernelGGL((k_interp<2, KER, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, a);
         }
         HIP_OK(hipGetLastError());
+        if (t_end) HIP_OK(hipEventRecord(t_end, s));
         return IBAMR_OK;
     });
+}
+
+// IBMethod::forwardEulerStep (IBMethod.cpp:713-739)
+int ibamr_forward_euler_step(double* X_new, const double* X, const void* U, int u_precision, double dt, long n,
+                             void* stream)
+{
+    if (n < 0) return fail(IBAMR_ERR_INVALID, "n < 0");
+    if (n == 0) return IBAMR_OK;
+    if (!X_new || !X || !U) return fail(IBAMR_ERR_INVALID, "null array");
+    if (u_precision != IBAMR_FP64 && u_precision != IBAMR_FP32C) return fail(IBAMR_ERR_INVALID, "bad precision");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int B = 256;
+    const dim3 g((unsigned)((n + B - 1) / B));
+    if (u_precision == IBAMR_FP64)
+        hipLaunchKernelGGL(k_euler<double>, g, dim3(B), 0, s, X_new, X, static_cast<const double*>(U), dt, n);
+    else
+        hipLaunchKernelGGL(k_euler<float>, g, dim3(B), 0, s, X_new, X, static_cast<const float*>(U), dt, n);
+    HIP_OK(hipGetLastError());
+    return IBAMR_OK;
 }
 
 int ibamr_plan_fill_periodic_ghosts(ibamr_plan* P, int slot, void* stream)

@@ -52,13 +52,18 @@ enum
 };
 enum
 {
-    IBAMR_SPREAD_ACCUMULATE = 0, /* u += S F (LEInteractor::spread) */
-    IBAMR_SPREAD_OVERWRITE = 1   /* u = S F over the whole ghost box (zero + spread) */
+    IBAMR_SPREAD_ACCUMULATE = 0, /* u += S F over the whole ghost box (LEInteractor::spread) */
+    IBAMR_SPREAD_OVERWRITE = 1,  /* u = S F over the whole ghost box (zero + spread) */
+    IBAMR_SPREAD_ACCUMULATE_INTERIOR = 2 /* u += S F on the patch interiors, ghost values unchanged
+                                            (LDataManager::spread, LDataManager.cpp:589-594, 661-663) */
 };
 
 /* Kernel registry, LEInteractor::isKnownKernel / getStencilSize /
- * getMinimumGhostWidth (LEInteractor.cpp:2035-2114).  Names: "IB_4", "IB_6",
- * "BSPLINE_3" (spread/interp on the GPU) and "PIECEWISE_LINEAR" (registry only). */
+ * getMinimumGhostWidth (LEInteractor.cpp:2035-2114): the reference's 22 names
+ * with their stencil sizes (USER_DEFINED reports 0: no user kernel is
+ * registered).  "IB_4", "IB_6" and "BSPLINE_3" have GPU spread/interp; plans
+ * for the other names fail with IBAMR_ERR_BAD_KERNEL.  Unknown names: 0 from
+ * ibamr_is_known_kernel, IBAMR_ERR_BAD_KERNEL from the other two. */
 int ibamr_is_known_kernel(const char* kernel_fcn);
 int ibamr_get_stencil_size(const char* kernel_fcn);
 int ibamr_get_minimum_ghost_width(const char* kernel_fcn);
@@ -109,6 +114,19 @@ typedef struct ibamr_plan ibamr_plan;
 /* Creates a plan for one level: validates the kernel and ghost widths, builds
  * the (patch, tile) work list and allocates device scratch for max_points. */
 int ibamr_plan_create(ibamr_plan** plan, const ibamr_level_geom* level, const ibamr_plan_config* config);
+
+/* A plan for LEInteractor's raw-array form on one patch
+ * (LEInteractor::interpolate/spread with double* Q_data, X_data and a box,
+ * LEInteractor.cpp:2805-2860, 3950-4009; point selection by
+ * LEInteractor::buildLocalIndices, LEInteractor.cpp:6089-6124): no periodic
+ * images; a point is taken when its cell, by IndexUtilities::getCellIndex on
+ * the patch box and x range, lies in box_lower..box_upper (any box: the patch
+ * box, a grown box, a sub-box).  Points not taken are left out silently by
+ * ibamr_plan_bin_async (ibamr_plan_check and ibamr_plan_bin report them as
+ * IBAMR_ERR_OUT_OF_DOMAIN); interp leaves their U untouched and spread skips
+ * them.  Stencils are clipped to the data ghost box as in the reference. */
+int ibamr_plan_create_patch(ibamr_plan** plan, int ndim, const double* dx, const ibamr_patch_geom* patch,
+                            const int* box_lower, const int* box_upper, const ibamr_plan_config* config);
 int ibamr_plan_destroy(ibamr_plan* plan);
 
 /* Binds Eulerian arrays to a slot (0..7): arrays[p*narr + a], narr = ndim for
@@ -144,6 +162,22 @@ int ibamr_plan_interp(ibamr_plan* plan, int slot, void* U, const double* X, void
  * interior for a single-patch level that spans a fully periodic domain (the
  * RefineSchedule::fillData step before interp, LDataManager.cpp:742-745). */
 int ibamr_plan_fill_periodic_ghosts(ibamr_plan* plan, int slot, void* stream);
+
+/* IBMethod::forwardEulerStep (IBMethod.cpp:713-739): X_new = X + dt U over
+ * n values (n = ndim * points), on the stream; X_new may be X.  U is double
+ * (u_precision IBAMR_FP64) or float (IBAMR_FP32C).  Kernel only: graph-safe. */
+int ibamr_forward_euler_step(double* X_new, const double* X, const void* U, int u_precision, double dt, long n,
+                             void* stream);
+
+/* Kernel timing for measurement.  With timing on, HIP events are recorded on
+ * the launch stream right around each spread tile kernel and each
+ * interpolation kernel (the point permutation before them is not included).
+ * ibamr_plan_kernel_times waits for those events and returns the mean
+ * duration (ms) and the number of launches since timing was switched on or
+ * last read, then starts a new count.  Keep timing off for work captured in
+ * a HIP graph. */
+int ibamr_plan_set_timing(ibamr_plan* plan, int on);
+int ibamr_plan_kernel_times(ibamr_plan* plan, double* spread_ms, int* nspread, double* interp_ms, int* ninterp);
 
 /* Plan facts for reports: number of spread tiles, tile edge, bins. */
 int ibamr_plan_info(const ibamr_plan* plan, int* ntiles, int* tile, int* nbins);
