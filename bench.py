@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=400000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", choices=["shell512", "amr"], default="shell512",
+                    help="shell512: periodic 512^3 IB_6 (default); amr: 240-patch fine level, BSPLINE_3")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 2 GiB (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -204,6 +206,82 @@ def cpu_baseline(X_np, F_np, N, kernel, sample, seed=5):
                       % (X.shape[0], X_np.shape[0], N, t)}
 
 
+def run_amr_form(torch, ia, precision, level, X, F, kernel, steps, warmup, seed=77):
+    """Spread (overwrite) + interp on every patch of a multi-patch level, one launch each."""
+    import numpy as np
+
+    dt = torch.float64 if precision == ia.FP64 else torch.float32
+    n = X.shape[0]
+    plan = ia.LevelPlan(level, kernel, ia.SIDE, precision, max_points=n)
+    g = plan.data_gcw
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    f, u = [], []
+    for p in range(len(level.patches)):
+        for a in range(3):
+            shp = ia.array_shape(level, p, g, ia.SIDE, axis=a)
+            f.append(torch.zeros(shp, dtype=dt, device="cuda"))
+            u.append((2 * torch.rand(shp, generator=gen, device="cuda", dtype=torch.float64) - 1).to(dt))
+    plan.bind(0, f)
+    plan.bind(1, u)
+    s = torch.cuda.current_stream()
+    Fd = F.to(dt).contiguous()
+    U = torch.zeros((n, 3), dtype=dt, device="cuda")
+    plan.bin(X, stream=s)
+    for _ in range(warmup):
+        plan.spread(0, Fd, X, mode=ia.OVERWRITE, stream=s)
+        plan.interp(1, U, X, stream=s)
+    torch.cuda.synchronize()
+    plan.set_timing(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        plan.spread(0, Fd, X, mode=ia.OVERWRITE, stream=s)
+        plan.interp(1, U, X, stream=s)
+    e1.record(s)
+    torch.cuda.synchronize()
+    kt = plan.kernel_times()
+    plan.set_timing(False)
+    return {"f": f, "U": U, "ms": e0.elapsed_time(e1) / steps, "kt": kt, "info": plan.info()}
+
+
+def main_amr(args, torch, ia):
+    import numpy as np
+    from ibamr_amd import synthetic
+
+    kernel = "BSPLINE_3"
+    level, Xn = synthetic.amr_level(coarse=128, ratio=4, npoints=1 << 21, nshell=4)
+    n = Xn.shape[0]
+    cells = sum(int(np.prod([p.iupper[d] - p.ilower[d] + 1 for d in range(3)])) for p in level.patches)
+    X = torch.tensor(Xn, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(99)
+    F = torch.randn((n, 3), generator=g, device="cuda", dtype=torch.float64)
+    print("bench amr: %d patches, %d fine cells, %d points" % (len(level.patches), cells, n), file=sys.stderr,
+          flush=True)
+    r32 = run_amr_form(torch, ia, ia.FP32C, level, X, F, kernel, args.steps, args.warmup)
+    r64 = run_amr_form(torch, ia, ia.FP64, level, X, F, kernel, max(2, args.steps // 2), 1)
+    err_f = max(float((a.double() - b).abs().max()) for a, b in zip(r32["f"], r64["f"])) / \
+        max(float(b.abs().max()) for b in r64["f"])
+    err_u = float((r32["U"].double() - r64["U"]).abs().max() / r64["U"].abs().max())
+    out = {
+        "metric": METRIC, "value": n / (r32["ms"] * 1e-3), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r32["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "AMR fine level: coarse 128^3 x ratio 4, %d patches (edges 16-64), %d fine cells, "
+                               "%d points on 4 ellipsoidal shells, side-centred BSPLINE_3, FP32 compensated"
+                               % (len(level.patches), cells, n),
+                   "patches": len(level.patches), "fine_cells": cells, "points": n, "kernel": kernel,
+                   "spread_tiles": r32["info"]["ntiles"]},
+        "detail": {"spread_kernel_ms": r32["kt"]["spread_ms"], "interp_kernel_ms": r32["kt"]["interp_ms"],
+                   "fp64_points_per_s": n / (r64["ms"] * 1e-3), "fp64_spread_kernel_ms": r64["kt"]["spread_ms"],
+                   "fp64_interp_kernel_ms": r64["kt"]["interp_ms"], "fp32c_over_fp64": r64["ms"] / r32["ms"],
+                   "fp32c_rel_linf_err_spread": err_f, "fp32c_rel_linf_err_interp": err_u},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -218,6 +296,9 @@ def main():
     import ibamr_amd as ia
 
     ia.lib()
+    if args.workload == "amr":
+        main_amr(args, torch, ia)
+        return
     N, n, kernel = args.grid, args.points, args.kernel
     X, F64 = make_points(torch, n, N, seed=2026)
     print("bench: %d^3 %s side, %d points; FP32C form" % (N, kernel, n), file=sys.stderr, flush=True)

@@ -28,7 +28,10 @@
 
 namespace
 {
-constexpr int S3_ZC = 32;    // z values per spread tile (registers per lane)
+#ifndef IBAMR_S3_ZC
+#define IBAMR_S3_ZC 32
+#endif
+constexpr int S3_ZC = IBAMR_S3_ZC;  // z values per spread tile (registers per lane)
 constexpr int S3_PAD = 9;    // staged weight row: the 8 tile lanes (odd stride: no bank conflicts)
 constexpr int S3_MAXCOMB = 64;
 
@@ -439,9 +442,438 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     }
 }
 
+// ----------------------------------------------------------------------------- FP32C spread, 4 waves per tile
+//
+// The FP32C form of the spread on a cooperative workgroup: four wavefronts
+// share one 8 x 8 x 32 tile, wave w owning z values [8w, 8w + 8) of every
+// column (8 accumulators, 8 partials and 8 compensations per lane, so many
+// more waves fit on a SIMD than with one wave holding all 32).  Candidates are
+// taken 256 at a time in the canonical (bin, image) order; every thread tests
+// one candidate against the tile and computes its weights; the hits are ranked
+// by their first z index (ballots per index, then a prefix over (index, wave))
+// and written to LDS in that order.  Each wave then walks, for each z start
+// that reaches its slab, the contiguous run of points with that start: the
+// register indices stay compile-time constants and the loops have wave-uniform
+// trip counts.  The order of contributions to every grid value is a pure
+// function of the inputs, so results are bitwise reproducible.
+constexpr int SQ_NW = 4;                    // waves per tile
+constexpr int SQ_ZS = S3_ZC / SQ_NW;        // z values per wave
+constexpr int SQ_QN = 64 * SQ_NW;           // candidates per round
+static_assert(SQ_ZS * SQ_NW == S3_ZC, "tile z extent must split evenly over the waves");
+
+template <int KER>
+__global__ __launch_bounds__(64 * SQ_NW, 5) void k_spread3q(SpreadArgs<float> a)
+{
+    using T = float;
+    using KT = KernelTraits<KER>;
+    constexpr int W = KT::W;
+    constexpr int ZS = SQ_ZS;
+    constexpr int NK = S3_ZC + W - 1;  // z starts that reach the tile: -(W-1) .. S3_ZC-1
+    __shared__ T s_rx[SQ_QN * S3_PAD];
+    __shared__ T s_ry[SQ_QN * S3_PAD];
+    __shared__ __align__(16) T s_wz[SQ_QN * 8];
+    __shared__ int s_lb[3][16];
+    __shared__ int s_lo[3][16];
+    __shared__ int s_ln[3];
+    __shared__ int s_cs[S3_MAXCOMB];
+    __shared__ int s_co[S3_MAXCOMB];
+    __shared__ int s_cp[S3_MAXCOMB + 1];
+    __shared__ int s_cnt[SQ_NW][NK];
+    __shared__ int s_woff[SQ_NW][NK];
+    __shared__ int s_start[NK + 1];
+    __shared__ int s_total;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int x = lane & 7, y = lane >> 3;
+    const TileDesc td = a.tiles[blockIdx.x];
+    if (td.patch < 0) return;  // padding of the XCD-ordered work list
+    const PatchDev P = a.patches[td.patch];
+    const int axis = a.side ? td.comp : -1;
+    const int tsz[3] = {8, 8, S3_ZC};
+    int alo[3], ahi[3], an[3], flo[3], fhi[3];
+    double xl[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        alo[d] = P.ilo[d] - a.gcw[d];
+        ahi[d] = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0);
+        an[d] = ahi[d] - alo[d] + 1;
+        flo[d] = alo[d] + td.t[d] * tsz[d];
+        fhi[d] = min(flo[d] + tsz[d] - 1, ahi[d]);
+        if (a.mode == IBAMR_SPREAD_ACCUMULATE_INTERIOR)
+        {
+            flo[d] = max(flo[d], P.ilo[d]);
+            fhi[d] = min(fhi[d], P.ihi[d] + (d == axis ? 1 : 0));
+        }
+        xl[d] = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);  // LEInteractor.cpp:3689
+    }
+    if (flo[0] > fhi[0] || flo[1] > fhi[1] || flo[2] > fhi[2]) return;  // tile wholly in the ghost region
+    T* __restrict__ arr = a.arrays[td.patch * a.narr + (a.side ? td.comp : 0)];
+    const size_t plane = (size_t)an[0] * an[1];
+    if (!a.side) arr += (size_t)td.comp * plane * an[2];
+    const int fcomp = td.comp;
+    const int gx = flo[0] + x, gy = flo[1] + y;
+    const int zb = flo[2] + ZS * wv;  // first z of this wave's slab
+    const bool col_in = gx <= fhi[0] && gy <= fhi[1];
+    const size_t col_off = (size_t)(gx - alo[0]) + (size_t)an[0] * (size_t)(gy - alo[1]);
+
+    // cells whose points can reach the tile, inside the patch's Lagrangian ghost box
+    int clo[3], chi[3];
+    bool any = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        clo[d] = max(flo[d] - a.reach, P.ilo[d] - a.idx_gcw);
+        chi[d] = min(fhi[d] + a.reach, P.ihi[d] + a.idx_gcw);
+        any = any && clo[d] <= chi[d];
+    }
+    // per-axis (bin, image offset) lists in increasing (bin, offset) order
+    if (tid < 3)
+    {
+        const int d = tid;
+        int n = 0;
+        if (any)
+        {
+            const int N = a.pshift[d];
+            const int omax = N > 0 ? 1 : 0;
+            for (int o = -omax; o <= omax; ++o)
+            {
+                const int pl = max(clo[d] - o * N, a.dom_lo[d]);
+                const int ph = min(chi[d] - o * N, a.dom_hi[d]);
+                if (pl > ph) continue;
+                const int bl = floor_div(pl - a.dom_lo[d], a.bin);
+                const int bh = floor_div(ph - a.dom_lo[d], a.bin);
+                for (int b = bl; b <= bh && n < 16; ++b)
+                {
+                    int k = n++;
+                    while (k > 0 && (s_lb[d][k - 1] > b || (s_lb[d][k - 1] == b && s_lo[d][k - 1] > o)))
+                    {
+                        s_lb[d][k] = s_lb[d][k - 1];
+                        s_lo[d][k] = s_lo[d][k - 1];
+                        --k;
+                    }
+                    s_lb[d][k] = b;
+                    s_lo[d][k] = o;
+                }
+            }
+        }
+        s_ln[d] = n;
+    }
+    __syncthreads();
+    const int nx = s_ln[0], ny = s_ln[1], nz = s_ln[2];
+    const int ncomb = nx * ny * nz;
+
+    // number of candidate points (wave 0), shared through LDS
+    if (wv == 0)
+    {
+        int total = 0;
+        for (int c = lane; c < ncomb; c += 64)
+        {
+            const int kx = c % nx, ky = (c / nx) % ny, kz = c / (nx * ny);
+            const int m = bin_mem(3, s_lb[0][kx], s_lb[1][ky], s_lb[2][kz], a.nb, a.nsb);
+            total += a.bin_start[m + 1] - a.bin_start[m];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off);
+        if (lane == 0) s_total = total;
+    }
+    __syncthreads();
+    const int total = s_total;
+    const bool overwrite = a.mode == IBAMR_SPREAD_OVERWRITE;
+    if (total == 0 && !overwrite) return;
+
+    T acc[ZS], part[ZS], comp[ZS];
+#pragma unroll
+    for (int z = 0; z < ZS; ++z)
+    {
+        acc[z] = T(0);
+        part[z] = T(0);
+        comp[z] = T(0);
+    }
+    if (!overwrite && col_in)
+    {
+#pragma unroll
+        for (int z = 0; z < ZS; ++z)
+            if (zb + z <= fhi[2]) acc[z] = arr[col_off + plane * (size_t)(zb + z - alo[2])];
+    }
+
+    const T inv_vol = T(1.0 / a.vol);
+    for (int cb = 0; cb < ncomb && total > 0; cb += S3_MAXCOMB)
+    {
+        // ---- this group of combinations: ranges and their prefix (wave 0)
+        if (wv == 0)
+        {
+            const int c = cb + lane;
+            int st = 0, ln = 0, oc = 0;
+            if (lane < S3_MAXCOMB && c < ncomb)
+            {
+                const int kx = c % nx, ky = (c / nx) % ny, kz = c / (nx * ny);
+                const int m = bin_mem(3, s_lb[0][kx], s_lb[1][ky], s_lb[2][kz], a.nb, a.nsb);
+                st = a.bin_start[m];
+                ln = a.bin_start[m + 1] - st;
+                oc = (s_lo[0][kx] + 1) + 3 * (s_lo[1][ky] + 1) + 9 * (s_lo[2][kz] + 1);
+            }
+            int inc = ln;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1)
+            {
+                const int t = __shfl_up(inc, off);
+                if (lane >= off) inc += t;
+            }
+            if (lane < S3_MAXCOMB)
+            {
+                s_cs[lane] = st;
+                s_co[lane] = oc;
+                s_cp[lane + 1] = inc;
+            }
+            if (lane == 0) s_cp[0] = 0;
+        }
+        __syncthreads();
+        const int ngc = min(S3_MAXCOMB, ncomb - cb);
+        const int gtotal = s_cp[ngc];
+
+        // candidate q of the group -> sorted point index and image code
+        auto cand = [&](int q, int& k, int& oc) {
+            k = -1;
+            oc = 13;
+            if (q < gtotal)
+            {
+                int lo = 0, hi = ngc - 1;  // last combination with prefix <= q
+                while (lo < hi)
+                {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_cp[mid] <= q)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                k = s_cs[lo] + (q - s_cp[lo]);
+                oc = s_co[lo];
+            }
+        };
+        int nk, noc;
+        int4 nc4 = make_int4(0, 0, 0, 0);
+        double nX[3] = {0.0, 0.0, 0.0};
+        T nF = T(0);
+        cand(tid, nk, noc);
+        if (nk >= 0)
+        {
+            nc4 = a.cells[nk];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
+            nF = a.F[(size_t)a.fdepth * nk + fcomp];
+        }
+        for (int cq = 0; cq < gtotal; cq += SQ_QN)
+        {
+            const int k = nk, oc = noc;
+            const int4 c4 = nc4;
+            const double Xk[3] = {nX[0], nX[1], nX[2]};
+            const T Fv = nF;
+            // prefetch the next round's candidate
+            cand(cq + SQ_QN + tid, nk, noc);
+            if (nk >= 0)
+            {
+                nc4 = a.cells[nk];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
+                nF = a.F[(size_t)a.fdepth * nk + fcomp];
+            }
+
+            // ---- test and weights of this thread's candidate
+            int key = -1;
+            T w[3][W];
+            int dx0 = 0, dy0 = 0;
+            if (k >= 0)
+            {
+                const int o[3] = {oc % 3 - 1, (oc / 3) % 3 - 1, oc / 9 - 1};
+                const int cc[3] = {c4.x, c4.y, c4.z};
+                bool ok = true;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    const int ci = cc[d] + o[d] * a.pshift[d];
+                    ok = ok && ci >= clo[d] && ci <= chi[d];
+                }
+                if (ok)
+                {
+                    int bse[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        // X + Xshift, Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
+                        const double Xs = Xk[d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
+                        bse[d] = P.ilo[d] + weights_1d<KER>(Xs, xl[d], a.dx[d], w[d]);
+                    }
+                    dx0 = bse[0] - flo[0];
+                    dy0 = bse[1] - flo[1];
+                    const int zz = bse[2] - flo[2];
+                    if (dx0 > -W && dx0 < 8 && dy0 > -W && dy0 < 8 && zz > -W && zz < S3_ZC) key = zz + (W - 1);
+                }
+            }
+            // ---- rank the hits by z start: ballots per start, then a prefix over (start, wave)
+            int rank = 0;
+#pragma unroll
+            for (int v = 0; v < NK; ++v)
+            {
+                const unsigned long long m = __ballot(key == v);
+                if (key == v) rank = __popcll(m & ((1ull << lane) - 1ull));
+                if (lane == 0) s_cnt[wv][v] = __popcll(m);
+            }
+            __syncthreads();
+            if (wv == 0)
+            {
+                int c[SQ_NW], tot = 0;
+#pragma unroll
+                for (int w2 = 0; w2 < SQ_NW; ++w2)
+                {
+                    c[w2] = lane < NK ? s_cnt[w2][lane] : 0;
+                    tot += c[w2];
+                }
+                int inc = tot;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1)
+                {
+                    const int t = __shfl_up(inc, off);
+                    if (lane >= off) inc += t;
+                }
+                if (lane < NK)
+                {
+                    int run = inc - tot;
+                    s_start[lane] = run;
+#pragma unroll
+                    for (int w2 = 0; w2 < SQ_NW; ++w2)
+                    {
+                        s_woff[w2][lane] = run;
+                        run += c[w2];
+                    }
+                }
+                if (lane == NK - 1) s_start[NK] = inc;
+            }
+            __syncthreads();
+            if (key >= 0)
+            {
+                const int pos = s_woff[wv][key] + rank;
+                // rows hold the weight of each of the tile's 8 x (y) lanes, zero outside the stencil
+                T* rx = s_rx + pos * S3_PAD;
+                T* ry = s_ry + pos * S3_PAD;
+                const T fx = KT::IB ? Fv : Fv * inv_vol;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                {
+                    rx[i] = T(0);
+                    ry[i] = T(0);
+                }
+#pragma unroll
+                for (int j = 0; j < W; ++j)
+                {
+                    if ((unsigned)(dx0 + j) < 8u) rx[dx0 + j] = w[0][j] * fx;
+                    if ((unsigned)(dy0 + j) < 8u) ry[dy0 + j] = w[1][j];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    s_wz[pos * 8 + j] = j < W ? (KT::IB ? w[2][j] * inv_vol : w[2][j]) : T(0);
+            }
+            __syncthreads();
+
+            // ---- apply: for each z start reaching this wave's slab, its run of points
+#pragma unroll
+            for (int kz = -(W - 1); kz < ZS; ++kz)
+            {
+                const int v = ZS * wv + kz + (W - 1);  // key of start (slab base + kz)
+                if (v < 0 || v >= NK) continue;
+                const int q0 = s_start[v], q1 = s_start[v + 1];
+                int q = q0;
+                for (; q + 1 < q1; q += 2)
+                {
+                    const T wx = s_rx[q * S3_PAD + x], wy = s_ry[q * S3_PAD + y];
+                    const T wx2 = s_rx[(q + 1) * S3_PAD + x], wy2 = s_ry[(q + 1) * S3_PAD + y];
+                    const float4 za = *reinterpret_cast<const float4*>(s_wz + q * 8);
+                    const float4 zc = *reinterpret_cast<const float4*>(s_wz + q * 8 + 4);
+                    const float4 za2 = *reinterpret_cast<const float4*>(s_wz + (q + 1) * 8);
+                    const float4 zc2 = *reinterpret_cast<const float4*>(s_wz + (q + 1) * 8 + 4);
+                    const T wz[8] = {za.x, za.y, za.z, za.w, zc.x, zc.y, zc.z, zc.w};
+                    const T wz2[8] = {za2.x, za2.y, za2.z, za2.w, zc2.x, zc2.y, zc2.z, zc2.w};
+                    const T b = wx * wy, b2 = wx2 * wy2;
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+                        if (kz + i >= 0 && kz + i < ZS) part[kz + i] = fmaf(b2, wz2[i], fmaf(b, wz[i], part[kz + i]));
+                }
+                if (q < q1)
+                {
+                    const T wx = s_rx[q * S3_PAD + x], wy = s_ry[q * S3_PAD + y];
+                    const float4 za = *reinterpret_cast<const float4*>(s_wz + q * 8);
+                    const float4 zc = *reinterpret_cast<const float4*>(s_wz + q * 8 + 4);
+                    const T wz[8] = {za.x, za.y, za.z, za.w, zc.x, zc.y, zc.z, zc.w};
+                    const T b = wx * wy;
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+                        if (kz + i >= 0 && kz + i < ZS) part[kz + i] = fmaf(b, wz[i], part[kz + i]);
+                }
+            }
+            // fold this round's partial sums into the compensated totals
+#pragma unroll
+            for (int z = 0; z < ZS; ++z)
+            {
+                const T yv = part[z] - comp[z];
+                const T t = acc[z] + yv;
+                comp[z] = (t - acc[z]) - yv;
+                acc[z] = t;
+                part[z] = T(0);
+            }
+            __syncthreads();
+        }
+    }
+
+    if (col_in)
+    {
+#pragma unroll
+        for (int z = 0; z < ZS; ++z)
+            if (zb + z <= fhi[2]) arr[col_off + plane * (size_t)(zb + z - alo[2])] = acc[z] - comp[z];
+    }
+}
+
 // ----------------------------------------------------------------------------- interpolation
 
 constexpr int I3_THREADS = 256;
+
+// Largest staged block edge: a superbin's 16 cells plus the stencil reach on
+// both sides (cell- and side-centred starts included).
+template <int KER>
+constexpr int interp3_edge()
+{
+    return 16 + KernelTraits<KER>::W;
+}
+
+template <int KER, typename T>
+constexpr int interp3_lds_elems()
+{
+    return interp3_edge<KER>() * interp3_edge<KER>() * interp3_edge<KER>();
+}
+
+// Owning patch of a sorted point: the patch whose selection box holds its
+// cell (LDataManager.cpp:748-753), or -1.
+template <typename T>
+__device__ __forceinline__ int interp_owner(const InterpArgs<T>& a, int k, const int4& c4)
+{
+    if (a.npatches == 1)
+    {
+        const PatchDev& Q = a.patches[0];
+        return (c4.x >= Q.slo[0] && c4.x <= Q.shi[0] && c4.y >= Q.slo[1] && c4.y <= Q.shi[1] && c4.z >= Q.slo[2] &&
+                c4.z <= Q.shi[2])
+                   ? 0
+                   : -1;
+    }
+    const int key = a.keys_sorted[k];
+    for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1]; ++j)
+    {
+        const PatchDev& Q = a.patches[a.bin_patches[j]];
+        if (c4.x >= Q.slo[0] && c4.x <= Q.shi[0] && c4.y >= Q.slo[1] && c4.y <= Q.shi[1] && c4.z >= Q.slo[2] &&
+            c4.z <= Q.shi[2])
+            return a.bin_patches[j];
+    }
+    return -1;
+}
 
 template <int KER, typename T>
 __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds_elems, int nsb, int per)
@@ -449,9 +881,12 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
     using KT = KernelTraits<KER>;
     constexpr int W = KT::W;
     constexpr bool FP64 = sizeof(T) == 8;
+    constexpr int UNIT = sizeof(T) / 4;  // dwords per value
     extern __shared__ __align__(16) unsigned char s_raw[];
     T* s_u = reinterpret_cast<T*>(s_raw);
-    __shared__ int s_lo[3], s_hi[3], s_pid, s_flag;
+    unsigned int* s_dw = reinterpret_cast<unsigned int*>(s_raw);
+    // extents: [0] bases with the patch x_lower, [1] with x_lower - dx/2 (the side axis)
+    __shared__ int s_lo[2][3], s_hi[2][3], s_pid, s_flag;
 
     // XCD-aware: workgroup i runs on XCD i % 8, which takes a contiguous run
     // of superbins (a z-slab), so neighbours share their halo in one L2
@@ -460,110 +895,123 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
     const int s0 = a.bin_start[sb * 8], s1 = a.bin_start[sb * 8 + 8];
     if (s0 == s1) return;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+
+    if (tid < 6)
+    {
+        s_lo[tid / 3][tid % 3] = 1 << 30;
+        s_hi[tid / 3][tid % 3] = -(1 << 30);
+    }
+    if (tid == 0)
+    {
+        s_pid = -1;
+        s_flag = 0;
+    }
+    __syncthreads();
+    // pass 1 (all arrays at once): owning patch and stencil bases of every
+    // point -> block extents; flags: bit 0 several patches, bit 1+a array a
+    // needs clipping (thread-local min/max, then one LDS atomic per wavefront)
+    {
+        int lmin[2][3], lmax[2][3];
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                lmin[v][d] = 1 << 30;
+                lmax[v][d] = -(1 << 30);
+            }
+        int lpid = -1, lflag = 0;
+        for (int k = s0 + tid; k < s1; k += I3_THREADS)
+        {
+            const int4 c4 = a.cells[k];
+            const int pid = interp_owner(a, k, c4);
+            if (pid < 0) continue;
+            if (lpid >= 0 && lpid != pid) lflag |= 1;
+            lpid = pid;
+            const PatchDev P = a.patches[pid];
+            int bc[3], bs[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                T w[W];
+                const double X = a.X[3 * (size_t)k + d];
+                bc[d] = P.ilo[d] + weights_1d<KER>(X, P.xlo[d], a.dx[d], w);
+                bs[d] = a.side ? P.ilo[d] + weights_1d<KER>(X, P.xlo[d] - 0.5 * a.dx[d], a.dx[d], w) : bc[d];
+                lmin[0][d] = min(lmin[0][d], bc[d]);
+                lmax[0][d] = max(lmax[0][d], bc[d]);
+                lmin[1][d] = min(lmin[1][d], bs[d]);
+                lmax[1][d] = max(lmax[1][d], bs[d]);
+            }
+            const int narr = a.side ? 3 : 1;
+            for (int ax = 0; ax < narr; ++ax)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    const bool sh = a.side && d == ax;
+                    const int b = sh ? bs[d] : bc[d];
+                    const int alo = P.ilo[d] - a.gcw[d];
+                    const int ahi = P.ihi[d] + a.gcw[d] + (sh ? 1 : 0);
+                    if (b < alo || b + W - 1 > ahi) lflag |= 2 << ax;
+                }
+        }
+        int pmin = lpid < 0 ? (1 << 30) : lpid, pmax = lpid;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+        {
+#pragma unroll
+            for (int v = 0; v < 2; ++v)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    lmin[v][d] = min(lmin[v][d], __shfl_xor(lmin[v][d], off));
+                    lmax[v][d] = max(lmax[v][d], __shfl_xor(lmax[v][d], off));
+                }
+            pmin = min(pmin, __shfl_xor(pmin, off));
+            pmax = max(pmax, __shfl_xor(pmax, off));
+            lflag |= __shfl_xor(lflag, off);
+        }
+        if (lane == 0)
+        {
+#pragma unroll
+            for (int v = 0; v < 2; ++v)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    atomicMin(&s_lo[v][d], lmin[v][d]);
+                    atomicMax(&s_hi[v][d], lmax[v][d]);
+                }
+            if (pmax >= 0)
+            {
+                if (pmin != pmax) lflag |= 1;
+                const int prev = atomicCAS(&s_pid, -1, pmin);
+                if (prev != -1 && prev != pmin) lflag |= 1;
+            }
+            if (lflag) atomicOr(&s_flag, lflag);
+        }
+    }
+    __syncthreads();
+    const int pid = s_pid;
+    if (pid < 0) return;  // no point of this superbin lies in a patch
+    const int flag = s_flag;
+    const PatchDev P = a.patches[pid];
 
     const int narrays = a.side ? 3 : 1;
     for (int ax = 0; ax < narrays; ++ax)
     {
         const int axis = a.side ? ax : -1;
-        if (tid < 3)
-        {
-            s_lo[tid] = 1 << 30;
-            s_hi[tid] = -(1 << 30);
-        }
-        if (tid == 0)
-        {
-            s_pid = -1;
-            s_flag = 0;
-        }
-        __syncthreads();
-        // pass 1: owning patch and stencil bases of every point -> block extent
-        // (thread-local min/max, then one LDS atomic per wavefront)
-        int lmin[3] = {1 << 30, 1 << 30, 1 << 30}, lmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-        int lpid = -1, lflag = 0;
-        for (int k = s0 + tid; k < s1; k += I3_THREADS)
-        {
-            const int key = a.keys_sorted[k];
-            const int4 c4 = a.cells[k];
-            const int cc[3] = {c4.x, c4.y, c4.z};
-            int pid = -1;
-            if (a.npatches == 1)
-            {
-                const PatchDev& Q = a.patches[0];
-                if (cc[0] >= Q.slo[0] && cc[0] <= Q.shi[0] && cc[1] >= Q.slo[1] && cc[1] <= Q.shi[1] &&
-                    cc[2] >= Q.slo[2] && cc[2] <= Q.shi[2])
-                    pid = 0;
-            }
-            else
-                for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && pid < 0; ++j)
-                {
-                    const PatchDev& Q = a.patches[a.bin_patches[j]];
-                    if (cc[0] >= Q.slo[0] && cc[0] <= Q.shi[0] && cc[1] >= Q.slo[1] && cc[1] <= Q.shi[1] &&
-                        cc[2] >= Q.slo[2] && cc[2] <= Q.shi[2])
-                        pid = a.bin_patches[j];
-                }
-            if (pid < 0) continue;
-            if (lpid >= 0 && lpid != pid) lflag |= 1;
-            lpid = pid;
-            const PatchDev P = a.patches[pid];
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-            {
-                const double xl = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);
-                T w[W];
-                const int b = P.ilo[d] + weights_1d<KER>(a.X[3 * (size_t)k + d], xl, a.dx[d], w);
-                const int alo = P.ilo[d] - a.gcw[d];
-                const int ahi = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0);
-                if (b < alo || b + W - 1 > ahi) lflag |= 2;  // needs clipping
-                lmin[d] = min(lmin[d], b);
-                lmax[d] = max(lmax[d], b);
-            }
-        }
-        {
-            // wavefront reductions; patch id: any two different ids set the flag
-            int pmin = lpid < 0 ? (1 << 30) : lpid, pmax = lpid;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1)
-            {
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                {
-                    lmin[d] = min(lmin[d], __shfl_xor(lmin[d], off));
-                    lmax[d] = max(lmax[d], __shfl_xor(lmax[d], off));
-                }
-                pmin = min(pmin, __shfl_xor(pmin, off));
-                pmax = max(pmax, __shfl_xor(pmax, off));
-                lflag |= __shfl_xor(lflag, off);
-            }
-            if ((tid & 63) == 0)
-            {
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                {
-                    atomicMin(&s_lo[d], lmin[d]);
-                    atomicMax(&s_hi[d], lmax[d]);
-                }
-                if (pmax >= 0)
-                {
-                    if (pmin != pmax) lflag |= 1;
-                    const int prev = atomicCAS(&s_pid, -1, pmin);
-                    if (prev != -1 && prev != pmin) lflag |= 1;
-                }
-                if (lflag) atomicOr(&s_flag, lflag);
-            }
-        }
-        __syncthreads();
-        const int pid = s_pid;
-        if (pid < 0) return;  // no point of this superbin lies in a patch interior
-        const PatchDev P = a.patches[pid];
-        int alo[3], an[3];
+        int alo[3], an[3], blo[3], e[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
         {
+            const int v = d == axis ? 1 : 0;
             alo[d] = P.ilo[d] - a.gcw[d];
             an[d] = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0) - alo[d] + 1;
+            blo[d] = s_lo[v][d];
+            e[d] = s_hi[v][d] - s_lo[v][d] + W;
         }
-        const int e0 = s_hi[0] - s_lo[0] + W, e1 = s_hi[1] - s_lo[1] + W, e2 = s_hi[2] - s_lo[2] + W;
-        const bool staged = s_flag == 0 && e0 * e1 * e2 <= lds_elems;
+        const int total = e[0] * e[1] * e[2];
+        const bool staged = (flag & (1 | (2 << ax))) == 0 && total <= lds_elems;
         const size_t plane = (size_t)an[0] * an[1];
         const int ncomps = a.side ? 1 : a.depth;
         for (int cd = 0; cd < ncomps; ++cd)
@@ -571,65 +1019,59 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
             const T* __restrict__ u = a.arrays[pid * a.narr + (a.side ? ax : 0)] + (size_t)cd * plane * an[2];
             if (staged)
             {
-                const int lo0 = s_lo[0], lo1 = s_lo[1], lo2 = s_lo[2];
-                const int nrow = e1 * e2;
-                // element e of the block -> (i0, row); every thread issues 8
-                // independent loads before its LDS stores, so the memory
-                // latency is paid once per 8 elements, not per element
-                const int total = e0 * nrow;
-                const float inv_e0 = 1.0f / (float)e0, inv_e1 = 1.0f / (float)e1;
-                for (int base = tid; base < total; base += 8 * I3_THREADS)
+                // Stage the block straight from global memory into LDS (global_load_lds:
+                // no registers in between, every load in flight at once).  Dword q of
+                // the block goes to s_dw[q]; each wave covers 64 consecutive dwords per
+                // instruction, so the LDS destination is wave-uniform.
+                const int ndw = total * UNIT;
+                const int q0 = wv * 64 + lane;
+                int el = q0 / UNIT;
+                const int half = q0 % UNIT;
+                constexpr int STEP = I3_THREADS / UNIT;  // values per round per thread
+                int i2 = el / (e[0] * e[1]);
+                int r = el - i2 * e[0] * e[1];
+                int i1 = r / e[0];
+                int i0 = r - i1 * e[0];
+                const int st1 = STEP / e[0], st0 = STEP - st1 * e[0];
+                const unsigned int* ub = reinterpret_cast<const unsigned int*>(
+                    u + (size_t)(blo[0] - alo[0]) + (size_t)an[0] * (blo[1] - alo[1]) + plane * (size_t)(blo[2] - alo[2]));
+                for (int base = wv * 64; base < ndw; base += I3_THREADS)
                 {
-                    T v[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
+                    if (base + lane < ndw)
                     {
-                        const int e = base + q * I3_THREADS;
-                        if (e < total)
-                        {
-                            int r = (int)((float)e * inv_e0);       // exact for e < 2^24 after the fix-up
-                            r -= (r * e0 > e);
-                            r += ((r + 1) * e0 <= e);
-                            const int i0 = e - r * e0;
-                            int i2 = (int)((float)r * inv_e1);
-                            i2 -= (i2 * e1 > r);
-                            i2 += ((i2 + 1) * e1 <= r);
-                            const int i1 = r - i2 * e1;
-                            v[q] = u[(size_t)(lo0 + i0 - alo[0]) + (size_t)an[0] * (lo1 + i1 - alo[1]) +
-                                     plane * (size_t)(lo2 + i2 - alo[2])];
-                        }
+                        const size_t g = (size_t)i0 + (size_t)an[0] * i1 + plane * (size_t)i2;
+                        __builtin_amdgcn_global_load_lds(ub + g * UNIT + half, s_dw + base, 4, 0, 0);
                     }
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
+                    i0 += st0;
+                    i1 += st1;
+                    if (i0 >= e[0])
                     {
-                        const int e = base + q * I3_THREADS;
-                        if (e < total) s_u[e] = v[q];
+                        i0 -= e[0];
+                        ++i1;
+                    }
+                    while (i1 >= e[1])
+                    {
+                        i1 -= e[1];
+                        ++i2;
                     }
                 }
+                __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
             }
             for (int k = s0 + tid; k < s1; k += I3_THREADS)
             {
                 const int4 c4 = a.cells[k];
                 // the point's own patch (the superbin may straddle patches: then no staging)
-                int qid = pid;
-                if (!staged)
-                {
-                    const int key = a.keys_sorted[k];
-                    qid = -1;
-                    for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && qid < 0; ++j)
-                    {
-                        const PatchDev& Q = a.patches[a.bin_patches[j]];
-                        if (c4.x >= Q.slo[0] && c4.x <= Q.shi[0] && c4.y >= Q.slo[1] && c4.y <= Q.shi[1] &&
-                            c4.z >= Q.slo[2] && c4.z <= Q.shi[2])
-                            qid = a.bin_patches[j];
-                    }
-                    if (qid < 0) continue;
-                }
+                int qid;
+                if (staged)  // one owning patch for the whole superbin; points outside it are skipped
+                    qid = (c4.x >= P.slo[0] && c4.x <= P.shi[0] && c4.y >= P.slo[1] && c4.y <= P.shi[1] &&
+                           c4.z >= P.slo[2] && c4.z <= P.shi[2])
+                              ? pid
+                              : -1;
+                else
+                    qid = interp_owner(a, k, c4);
+                if (qid < 0) continue;
                 const PatchDev Pq = a.patches[qid];
-                if (c4.x < Pq.slo[0] || c4.x > Pq.shi[0] || c4.y < Pq.slo[1] || c4.y > Pq.shi[1] ||
-                    c4.z < Pq.slo[2] || c4.z > Pq.shi[2])
-                    continue;
                 const int p = a.perm[k];
                 T w[3][W];
                 int bse[3];
@@ -642,13 +1084,13 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                 Sum<T> acc;
                 if (staged)
                 {
-                    const T* base = s_u + (bse[0] - s_lo[0]) + e0 * ((bse[1] - s_lo[1]) + e1 * (bse[2] - s_lo[2]));
+                    const T* base = s_u + (bse[0] - blo[0]) + e[0] * ((bse[1] - blo[1]) + e[1] * (bse[2] - blo[2]));
 #pragma unroll
                     for (int i2 = 0; i2 < W; ++i2)
 #pragma unroll
                         for (int i1 = 0; i1 < W; ++i1)
                         {
-                            const T* row = base + e0 * (i1 + e1 * i2);
+                            const T* row = base + e[0] * (i1 + e[1] * i2);
                             if (FP64)
                             {
 #pragma unroll
@@ -658,10 +1100,10 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                             }
                             else
                             {
-                                T s = T(0);
+                                T sum = T(0);
 #pragma unroll
-                                for (int i0 = 0; i0 < W; ++i0) s = fmaf(w[0][i0], row[i0], s);
-                                acc.add((w[1][i1] * w[2][i2]) * s);
+                                for (int i0 = 0; i0 < W; ++i0) sum = fmaf(w[0][i0], row[i0], sum);
+                                acc.add((w[1][i1] * w[2][i2]) * sum);
                             }
                         }
                 }
@@ -700,11 +1142,11 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                             }
                             else
                             {
-                                T s = T(0);
+                                T sum = T(0);
 #pragma unroll
                                 for (int i0 = 0; i0 < W; ++i0)
-                                    if (i0 >= lo[0] && i0 <= hi[0]) s = fmaf(w[0][i0], row[i0], s);
-                                acc.add((w[1][i1] * w[2][i2]) * s);
+                                    if (i0 >= lo[0] && i0 <= hi[0]) sum = fmaf(w[0][i0], row[i0], sum);
+                                acc.add((w[1][i1] * w[2][i2]) * sum);
                             }
                         }
                     }

@@ -179,6 +179,7 @@ struct ibamr_plan
     int* bin_patch_start_d = nullptr;  // [nmem+2]
     int* bin_patches_d = nullptr;
     void** arrays_d[MAX_SLOTS] = {};
+    std::vector<void*> arrays_h[MAX_SLOTS];   // host copy of each slot's pointer table
     int* keys_d = nullptr;
     int* keys_sorted_d = nullptr;
     int* idx_d = nullptr;              // point indices in bin order before the per-bin sort
@@ -1294,6 +1295,20 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     PLAN_HIP(hipMalloc(&P->scan_tmp_d, sizeof(int) * ((P->nmem + 1 + SCAN_B - 1) / SCAN_B + 1)));
     PLAN_HIP(hipMalloc(&P->scan_part_d, sizeof(int) * (P->nmem + 1)));
 #undef PLAN_HIP
+    if (ndim == 3)
+    {
+        // raise the dynamic LDS limit of this plan's interpolation kernel here, not
+        // per launch: the launches may be captured into a HIP graph
+        const int rc = dispatch(3, k, C->precision, [&](auto, auto ker, auto prec) -> int {
+            using T = decltype(prec);
+            constexpr int KER = decltype(ker)::value;
+            HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp3<KER, T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       interp3_lds_elems<KER, T>() * (int)sizeof(T)));
+            return IBAMR_OK;
+        });
+        if (rc != IBAMR_OK) return cleanup_fail(rc);
+    }
     *out = P;
     return IBAMR_OK;
 }
@@ -1349,6 +1364,7 @@ int ibamr_plan_bind_data(ibamr_plan* P, int slot, void* const* arrays)
         if (!arrays[i]) return fail(IBAMR_ERR_INVALID, "null array pointer");
     if (!P->arrays_d[slot]) HIP_OK(hipMalloc(&P->arrays_d[slot], sizeof(void*) * n));
     HIP_OK(hipMemcpy(P->arrays_d[slot], arrays, sizeof(void*) * n, hipMemcpyHostToDevice));
+    P->arrays_h[slot].assign(arrays, arrays + n);
     return IBAMR_OK;
 }
 
@@ -1526,9 +1542,15 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         if (trc != IBAMR_OK) return trc;
         if (NDIM == 3)
         {
-            // FP32C packs points across bins; FP64 keeps chunks inside bins (fixed order)
-            constexpr bool MERGE = sizeof(T) == 4;
-            hipLaunchKernelGGL((k_spread3<KER, T, MERGE>), dim3(P->ntiles), dim3(64), 0, s, a);
+            // FP32C: four waves per tile over candidates packed across bins;
+            // FP64: one wave per tile, chunks cut per bin (the oracle's fixed order)
+#ifndef IBAMR_FP32_SPREAD_COOP
+#define IBAMR_FP32_SPREAD_COOP 1
+#endif
+            if constexpr (sizeof(T) == 4 && IBAMR_FP32_SPREAD_COOP)
+                hipLaunchKernelGGL((k_spread3q<KER>), dim3(P->ntiles), dim3(64 * SQ_NW), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_spread3<KER, T, sizeof(T) == 4>), dim3(P->ntiles), dim3(64), 0, s, a);
         }
         else
             hipLaunchKernelGGL((k_spread_tiles<2, KER, T, 16>), dim3(P->ntiles), dim3(WAVE), 0, s, a);
@@ -1586,10 +1608,9 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
         if (NDIM == 3)
         {
             // one workgroup per superbin; LDS holds one staged block of the array
-            const int bytes = sizeof(T) == 4 ? 60 * 1024 : 120 * 1024;
-            const int elems = bytes / (int)sizeof(T);
-            HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp3<KER, T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            // (the kernel's dynamic LDS limit is raised once, at plan creation)
+            const int elems = interp3_lds_elems<KER, T>();
+            const int bytes = elems * (int)sizeof(T);
             const int trc = time_begin(P, 1, s, &t_end);
             if (trc != IBAMR_OK) return trc;
             const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
@@ -1639,8 +1660,8 @@ int ibamr_plan_fill_periodic_ghosts(ibamr_plan* P, int slot, void* stream)
         if (P->pshift[d] != pd.ihi[d] - pd.ilo[d] + 1 || pd.ilo[d] != P->dom_lo[d])
             return fail(IBAMR_ERR_INVALID, "periodic ghost fill needs a fully periodic domain covered by the patch");
     }
-    std::vector<void*> ptrs((size_t)P->narr);
-    HIP_OK(hipMemcpy(ptrs.data(), P->arrays_d[slot], sizeof(void*) * P->narr, hipMemcpyDeviceToHost));
+    // host copy of the pointer table: no device-to-host copy, so this can be captured
+    const std::vector<void*>& ptrs = P->arrays_h[slot];
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int ncomps = P->centering == IBAMR_SIDE ? 1 : P->depth;
     for (int a = 0; a < P->narr; ++a)

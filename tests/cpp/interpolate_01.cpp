@@ -111,7 +111,7 @@ int run_side(const std::string& k, const std::string& out_name)
     f->fillAll(0.0);
     const int n = 200;
     std::mt19937 seq(7u);
-    std::uniform_real_distribution<double> dist(0.22, 0.53), fdist(-1.0, 1.0);
+    std::uniform_real_distribution<double> dist(0.18, 0.57), fdist(-1.0, 1.0);
     std::vector<double> X(NDIM * n), F(NDIM * n), U(NDIM * n, -7.0);
     for (double& v : X) v = dist(seq);
     for (double& v : F) v = fdist(seq);
