@@ -3,7 +3,7 @@
 //
 // Spread (k_spread3): owner-computes, register-resident.  One wavefront owns a
 // column tile of one array: 8 x 8 (x, y) grid lines, one per lane, times
-// S3_ZC consecutive z values held in registers.  Points that can reach the
+// s3_zc<T>() consecutive z values held in registers.  Points that can reach the
 // tile are staged 64 at a time: each lane computes one point's weights (FP64
 // index selection, weights in T) and writes the x and y weights, aligned to
 // the tile's lanes and zero-padded, plus the z weights to LDS.  The staged
@@ -28,10 +28,19 @@
 
 namespace
 {
-#ifndef IBAMR_S3_ZC
-#define IBAMR_S3_ZC 32
+// z values per spread tile (registers per lane): FP32C keeps three register
+// sets per z (total, partial, compensation), so its column is shorter, which
+// leaves room for more waves per SIMD
+#ifndef IBAMR_S3_ZC_FP32
+#define IBAMR_S3_ZC_FP32 16
 #endif
-constexpr int S3_ZC = IBAMR_S3_ZC;  // z values per spread tile (registers per lane)
+constexpr int S3_ZC_FP64 = 32;
+constexpr int S3_ZC_FP32 = IBAMR_S3_ZC_FP32;
+template <typename T>
+constexpr int s3_zc()
+{
+    return sizeof(T) == 8 ? S3_ZC_FP64 : S3_ZC_FP32;
+}
 constexpr int S3_PAD = 9;    // staged weight row: the 8 tile lanes (odd stride: no bank conflicts)
 constexpr int S3_MAXCOMB = 64;
 
@@ -41,7 +50,7 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     using KT = KernelTraits<KER>;
     constexpr int W = KT::W;
     constexpr bool FP64 = sizeof(T) == 8;
-    constexpr int ZC = S3_ZC;
+    constexpr int ZC = s3_zc<T>();
     __shared__ T s_rx[64 * S3_PAD];
     __shared__ T s_ry[64 * S3_PAD];
     __shared__ T s_wz[64 * 8];
@@ -442,400 +451,12 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     }
 }
 
-// ----------------------------------------------------------------------------- FP32C spread, 4 waves per tile
-//
-// The FP32C form of the spread on a cooperative workgroup: four wavefronts
-// share one 8 x 8 x 32 tile, wave w owning z values [8w, 8w + 8) of every
-// column (8 accumulators, 8 partials and 8 compensations per lane, so many
-// more waves fit on a SIMD than with one wave holding all 32).  Candidates are
-// taken 256 at a time in the canonical (bin, image) order; every thread tests
-// one candidate against the tile and computes its weights; the hits are ranked
-// by their first z index (ballots per index, then a prefix over (index, wave))
-// and written to LDS in that order.  Each wave then walks, for each z start
-// that reaches its slab, the contiguous run of points with that start: the
-// register indices stay compile-time constants and the loops have wave-uniform
-// trip counts.  The order of contributions to every grid value is a pure
-// function of the inputs, so results are bitwise reproducible.
-constexpr int SQ_NW = 4;                    // waves per tile
-constexpr int SQ_ZS = S3_ZC / SQ_NW;        // z values per wave
-constexpr int SQ_QN = 64 * SQ_NW;           // candidates per round
-static_assert(SQ_ZS * SQ_NW == S3_ZC, "tile z extent must split evenly over the waves");
-
-template <int KER>
-__global__ __launch_bounds__(64 * SQ_NW, 5) void k_spread3q(SpreadArgs<float> a)
-{
-    using T = float;
-    using KT = KernelTraits<KER>;
-    constexpr int W = KT::W;
-    constexpr int ZS = SQ_ZS;
-    constexpr int NK = S3_ZC + W - 1;  // z starts that reach the tile: -(W-1) .. S3_ZC-1
-    __shared__ T s_rx[SQ_QN * S3_PAD];
-    __shared__ T s_ry[SQ_QN * S3_PAD];
-    __shared__ __align__(16) T s_wz[SQ_QN * 8];
-    __shared__ int s_lb[3][16];
-    __shared__ int s_lo[3][16];
-    __shared__ int s_ln[3];
-    __shared__ int s_cs[S3_MAXCOMB];
-    __shared__ int s_co[S3_MAXCOMB];
-    __shared__ int s_cp[S3_MAXCOMB + 1];
-    __shared__ int s_cnt[SQ_NW][NK];
-    __shared__ int s_woff[SQ_NW][NK];
-    __shared__ int s_start[NK + 1];
-    __shared__ int s_total;
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
-    const int x = lane & 7, y = lane >> 3;
-    const TileDesc td = a.tiles[blockIdx.x];
-    if (td.patch < 0) return;  // padding of the XCD-ordered work list
-    const PatchDev P = a.patches[td.patch];
-    const int axis = a.side ? td.comp : -1;
-    const int tsz[3] = {8, 8, S3_ZC};
-    int alo[3], ahi[3], an[3], flo[3], fhi[3];
-    double xl[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-        alo[d] = P.ilo[d] - a.gcw[d];
-        ahi[d] = P.ihi[d] + a.gcw[d] + (d == axis ? 1 : 0);
-        an[d] = ahi[d] - alo[d] + 1;
-        flo[d] = alo[d] + td.t[d] * tsz[d];
-        fhi[d] = min(flo[d] + tsz[d] - 1, ahi[d]);
-        if (a.mode == IBAMR_SPREAD_ACCUMULATE_INTERIOR)
-        {
-            flo[d] = max(flo[d], P.ilo[d]);
-            fhi[d] = min(fhi[d], P.ihi[d] + (d == axis ? 1 : 0));
-        }
-        xl[d] = P.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);  // LEInteractor.cpp:3689
-    }
-    if (flo[0] > fhi[0] || flo[1] > fhi[1] || flo[2] > fhi[2]) return;  // tile wholly in the ghost region
-    T* __restrict__ arr = a.arrays[td.patch * a.narr + (a.side ? td.comp : 0)];
-    const size_t plane = (size_t)an[0] * an[1];
-    if (!a.side) arr += (size_t)td.comp * plane * an[2];
-    const int fcomp = td.comp;
-    const int gx = flo[0] + x, gy = flo[1] + y;
-    const int zb = flo[2] + ZS * wv;  // first z of this wave's slab
-    const bool col_in = gx <= fhi[0] && gy <= fhi[1];
-    const size_t col_off = (size_t)(gx - alo[0]) + (size_t)an[0] * (size_t)(gy - alo[1]);
-
-    // cells whose points can reach the tile, inside the patch's Lagrangian ghost box
-    int clo[3], chi[3];
-    bool any = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-        clo[d] = max(flo[d] - a.reach, P.ilo[d] - a.idx_gcw);
-        chi[d] = min(fhi[d] + a.reach, P.ihi[d] + a.idx_gcw);
-        any = any && clo[d] <= chi[d];
-    }
-    // per-axis (bin, image offset) lists in increasing (bin, offset) order
-    if (tid < 3)
-    {
-        const int d = tid;
-        int n = 0;
-        if (any)
-        {
-            const int N = a.pshift[d];
-            const int omax = N > 0 ? 1 : 0;
-            for (int o = -omax; o <= omax; ++o)
-            {
-                const int pl = max(clo[d] - o * N, a.dom_lo[d]);
-                const int ph = min(chi[d] - o * N, a.dom_hi[d]);
-                if (pl > ph) continue;
-                const int bl = floor_div(pl - a.dom_lo[d], a.bin);
-                const int bh = floor_div(ph - a.dom_lo[d], a.bin);
-                for (int b = bl; b <= bh && n < 16; ++b)
-                {
-                    int k = n++;
-                    while (k > 0 && (s_lb[d][k - 1] > b || (s_lb[d][k - 1] == b && s_lo[d][k - 1] > o)))
-                    {
-                        s_lb[d][k] = s_lb[d][k - 1];
-                        s_lo[d][k] = s_lo[d][k - 1];
-                        --k;
-                    }
-                    s_lb[d][k] = b;
-                    s_lo[d][k] = o;
-                }
-            }
-        }
-        s_ln[d] = n;
-    }
-    __syncthreads();
-    const int nx = s_ln[0], ny = s_ln[1], nz = s_ln[2];
-    const int ncomb = nx * ny * nz;
-
-    // number of candidate points (wave 0), shared through LDS
-    if (wv == 0)
-    {
-        int total = 0;
-        for (int c = lane; c < ncomb; c += 64)
-        {
-            const int kx = c % nx, ky = (c / nx) % ny, kz = c / (nx * ny);
-            const int m = bin_mem(3, s_lb[0][kx], s_lb[1][ky], s_lb[2][kz], a.nb, a.nsb);
-            total += a.bin_start[m + 1] - a.bin_start[m];
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off);
-        if (lane == 0) s_total = total;
-    }
-    __syncthreads();
-    const int total = s_total;
-    const bool overwrite = a.mode == IBAMR_SPREAD_OVERWRITE;
-    if (total == 0 && !overwrite) return;
-
-    T acc[ZS], part[ZS], comp[ZS];
-#pragma unroll
-    for (int z = 0; z < ZS; ++z)
-    {
-        acc[z] = T(0);
-        part[z] = T(0);
-        comp[z] = T(0);
-    }
-    if (!overwrite && col_in)
-    {
-#pragma unroll
-        for (int z = 0; z < ZS; ++z)
-            if (zb + z <= fhi[2]) acc[z] = arr[col_off + plane * (size_t)(zb + z - alo[2])];
-    }
-
-    const T inv_vol = T(1.0 / a.vol);
-    for (int cb = 0; cb < ncomb && total > 0; cb += S3_MAXCOMB)
-    {
-        // ---- this group of combinations: ranges and their prefix (wave 0)
-        if (wv == 0)
-        {
-            const int c = cb + lane;
-            int st = 0, ln = 0, oc = 0;
-            if (lane < S3_MAXCOMB && c < ncomb)
-            {
-                const int kx = c % nx, ky = (c / nx) % ny, kz = c / (nx * ny);
-                const int m = bin_mem(3, s_lb[0][kx], s_lb[1][ky], s_lb[2][kz], a.nb, a.nsb);
-                st = a.bin_start[m];
-                ln = a.bin_start[m + 1] - st;
-                oc = (s_lo[0][kx] + 1) + 3 * (s_lo[1][ky] + 1) + 9 * (s_lo[2][kz] + 1);
-            }
-            int inc = ln;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1)
-            {
-                const int t = __shfl_up(inc, off);
-                if (lane >= off) inc += t;
-            }
-            if (lane < S3_MAXCOMB)
-            {
-                s_cs[lane] = st;
-                s_co[lane] = oc;
-                s_cp[lane + 1] = inc;
-            }
-            if (lane == 0) s_cp[0] = 0;
-        }
-        __syncthreads();
-        const int ngc = min(S3_MAXCOMB, ncomb - cb);
-        const int gtotal = s_cp[ngc];
-
-        // candidate q of the group -> sorted point index and image code
-        auto cand = [&](int q, int& k, int& oc) {
-            k = -1;
-            oc = 13;
-            if (q < gtotal)
-            {
-                int lo = 0, hi = ngc - 1;  // last combination with prefix <= q
-                while (lo < hi)
-                {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_cp[mid] <= q)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                k = s_cs[lo] + (q - s_cp[lo]);
-                oc = s_co[lo];
-            }
-        };
-        int nk, noc;
-        int4 nc4 = make_int4(0, 0, 0, 0);
-        double nX[3] = {0.0, 0.0, 0.0};
-        T nF = T(0);
-        cand(tid, nk, noc);
-        if (nk >= 0)
-        {
-            nc4 = a.cells[nk];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
-            nF = a.F[(size_t)a.fdepth * nk + fcomp];
-        }
-        for (int cq = 0; cq < gtotal; cq += SQ_QN)
-        {
-            const int k = nk, oc = noc;
-            const int4 c4 = nc4;
-            const double Xk[3] = {nX[0], nX[1], nX[2]};
-            const T Fv = nF;
-            // prefetch the next round's candidate
-            cand(cq + SQ_QN + tid, nk, noc);
-            if (nk >= 0)
-            {
-                nc4 = a.cells[nk];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
-                nF = a.F[(size_t)a.fdepth * nk + fcomp];
-            }
-
-            // ---- test and weights of this thread's candidate
-            int key = -1;
-            T w[3][W];
-            int dx0 = 0, dy0 = 0;
-            if (k >= 0)
-            {
-                const int o[3] = {oc % 3 - 1, (oc / 3) % 3 - 1, oc / 9 - 1};
-                const int cc[3] = {c4.x, c4.y, c4.z};
-                bool ok = true;
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                {
-                    const int ci = cc[d] + o[d] * a.pshift[d];
-                    ok = ok && ci >= clo[d] && ci <= chi[d];
-                }
-                if (ok)
-                {
-                    int bse[3];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                    {
-                        // X + Xshift, Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
-                        const double Xs = Xk[d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
-                        bse[d] = P.ilo[d] + weights_1d<KER>(Xs, xl[d], a.dx[d], w[d]);
-                    }
-                    dx0 = bse[0] - flo[0];
-                    dy0 = bse[1] - flo[1];
-                    const int zz = bse[2] - flo[2];
-                    if (dx0 > -W && dx0 < 8 && dy0 > -W && dy0 < 8 && zz > -W && zz < S3_ZC) key = zz + (W - 1);
-                }
-            }
-            // ---- rank the hits by z start: ballots per start, then a prefix over (start, wave)
-            int rank = 0;
-#pragma unroll
-            for (int v = 0; v < NK; ++v)
-            {
-                const unsigned long long m = __ballot(key == v);
-                if (key == v) rank = __popcll(m & ((1ull << lane) - 1ull));
-                if (lane == 0) s_cnt[wv][v] = __popcll(m);
-            }
-            __syncthreads();
-            if (wv == 0)
-            {
-                int c[SQ_NW], tot = 0;
-#pragma unroll
-                for (int w2 = 0; w2 < SQ_NW; ++w2)
-                {
-                    c[w2] = lane < NK ? s_cnt[w2][lane] : 0;
-                    tot += c[w2];
-                }
-                int inc = tot;
-#pragma unroll
-                for (int off = 1; off < 64; off <<= 1)
-                {
-                    const int t = __shfl_up(inc, off);
-                    if (lane >= off) inc += t;
-                }
-                if (lane < NK)
-                {
-                    int run = inc - tot;
-                    s_start[lane] = run;
-#pragma unroll
-                    for (int w2 = 0; w2 < SQ_NW; ++w2)
-                    {
-                        s_woff[w2][lane] = run;
-                        run += c[w2];
-                    }
-                }
-                if (lane == NK - 1) s_start[NK] = inc;
-            }
-            __syncthreads();
-            if (key >= 0)
-            {
-                const int pos = s_woff[wv][key] + rank;
-                // rows hold the weight of each of the tile's 8 x (y) lanes, zero outside the stencil
-                T* rx = s_rx + pos * S3_PAD;
-                T* ry = s_ry + pos * S3_PAD;
-                const T fx = KT::IB ? Fv : Fv * inv_vol;
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                {
-                    rx[i] = T(0);
-                    ry[i] = T(0);
-                }
-#pragma unroll
-                for (int j = 0; j < W; ++j)
-                {
-                    if ((unsigned)(dx0 + j) < 8u) rx[dx0 + j] = w[0][j] * fx;
-                    if ((unsigned)(dy0 + j) < 8u) ry[dy0 + j] = w[1][j];
-                }
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    s_wz[pos * 8 + j] = j < W ? (KT::IB ? w[2][j] * inv_vol : w[2][j]) : T(0);
-            }
-            __syncthreads();
-
-            // ---- apply: for each z start reaching this wave's slab, its run of points
-#pragma unroll
-            for (int kz = -(W - 1); kz < ZS; ++kz)
-            {
-                const int v = ZS * wv + kz + (W - 1);  // key of start (slab base + kz)
-                if (v < 0 || v >= NK) continue;
-                const int q0 = s_start[v], q1 = s_start[v + 1];
-                int q = q0;
-                for (; q + 1 < q1; q += 2)
-                {
-                    const T wx = s_rx[q * S3_PAD + x], wy = s_ry[q * S3_PAD + y];
-                    const T wx2 = s_rx[(q + 1) * S3_PAD + x], wy2 = s_ry[(q + 1) * S3_PAD + y];
-                    const float4 za = *reinterpret_cast<const float4*>(s_wz + q * 8);
-                    const float4 zc = *reinterpret_cast<const float4*>(s_wz + q * 8 + 4);
-                    const float4 za2 = *reinterpret_cast<const float4*>(s_wz + (q + 1) * 8);
-                    const float4 zc2 = *reinterpret_cast<const float4*>(s_wz + (q + 1) * 8 + 4);
-                    const T wz[8] = {za.x, za.y, za.z, za.w, zc.x, zc.y, zc.z, zc.w};
-                    const T wz2[8] = {za2.x, za2.y, za2.z, za2.w, zc2.x, zc2.y, zc2.z, zc2.w};
-                    const T b = wx * wy, b2 = wx2 * wy2;
-#pragma unroll
-                    for (int i = 0; i < W; ++i)
-                        if (kz + i >= 0 && kz + i < ZS) part[kz + i] = fmaf(b2, wz2[i], fmaf(b, wz[i], part[kz + i]));
-                }
-                if (q < q1)
-                {
-                    const T wx = s_rx[q * S3_PAD + x], wy = s_ry[q * S3_PAD + y];
-                    const float4 za = *reinterpret_cast<const float4*>(s_wz + q * 8);
-                    const float4 zc = *reinterpret_cast<const float4*>(s_wz + q * 8 + 4);
-                    const T wz[8] = {za.x, za.y, za.z, za.w, zc.x, zc.y, zc.z, zc.w};
-                    const T b = wx * wy;
-#pragma unroll
-                    for (int i = 0; i < W; ++i)
-                        if (kz + i >= 0 && kz + i < ZS) part[kz + i] = fmaf(b, wz[i], part[kz + i]);
-                }
-            }
-            // fold this round's partial sums into the compensated totals
-#pragma unroll
-            for (int z = 0; z < ZS; ++z)
-            {
-                const T yv = part[z] - comp[z];
-                const T t = acc[z] + yv;
-                comp[z] = (t - acc[z]) - yv;
-                acc[z] = t;
-                part[z] = T(0);
-            }
-            __syncthreads();
-        }
-    }
-
-    if (col_in)
-    {
-#pragma unroll
-        for (int z = 0; z < ZS; ++z)
-            if (zb + z <= fhi[2]) arr[col_off + plane * (size_t)(zb + z - alo[2])] = acc[z] - comp[z];
-    }
-}
-
 // ----------------------------------------------------------------------------- interpolation
 
-constexpr int I3_THREADS = 256;
+#ifndef IBAMR_I3_THREADS
+#define IBAMR_I3_THREADS 512
+#endif
+constexpr int I3_THREADS = IBAMR_I3_THREADS;  // threads per interpolation workgroup (one superbin)
 
 // Largest staged block edge: a superbin's 16 cells plus the stencil reach on
 // both sides (cell- and side-centred starts included).

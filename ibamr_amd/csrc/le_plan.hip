@@ -1141,7 +1141,7 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     P->bin = P->tile;
     P->tsz[0] = P->tile;
     P->tsz[1] = P->tile;
-    P->tsz[2] = ndim == 3 ? S3_ZC : 1;
+    P->tsz[2] = ndim == 3 ? (C->precision == IBAMR_FP64 ? S3_ZC_FP64 : S3_ZC_FP32) : 1;
     // per-axis (bin, image) lists of a tile hold at most 16 entries: the image
     // ranges partition the reachable cells, plus one split per domain side
     for (int d = 0; d < ndim; ++d)
@@ -1542,15 +1542,8 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         if (trc != IBAMR_OK) return trc;
         if (NDIM == 3)
         {
-            // FP32C: four waves per tile over candidates packed across bins;
-            // FP64: one wave per tile, chunks cut per bin (the oracle's fixed order)
-#ifndef IBAMR_FP32_SPREAD_COOP
-#define IBAMR_FP32_SPREAD_COOP 1
-#endif
-            if constexpr (sizeof(T) == 4 && IBAMR_FP32_SPREAD_COOP)
-                hipLaunchKernelGGL((k_spread3q<KER>), dim3(P->ntiles), dim3(64 * SQ_NW), 0, s, a);
-            else
-                hipLaunchKernelGGL((k_spread3<KER, T, sizeof(T) == 4>), dim3(P->ntiles), dim3(64), 0, s, a);
+            // FP32C packs chunks across bins; FP64 cuts them per bin (the oracle's fixed order)
+            hipLaunchKernelGGL((k_spread3<KER, T, sizeof(T) == 4>), dim3(P->ntiles), dim3(64), 0, s, a);
         }
         else
             hipLaunchKernelGGL((k_spread_tiles<2, KER, T, 16>), dim3(P->ntiles), dim3(WAVE), 0, s, a);

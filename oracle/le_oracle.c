@@ -561,6 +561,32 @@ void orc_order_entries_z(int kernel, const int* cell, const int* indices, const 
                          const int* bin, int chunk, int* perm)
 {
     zkey_t* z = (zkey_t*)malloc(sizeof(zkey_t) * (size_t)(nentries > 0 ? nentries : 1));
+    /* rank of every point among the points of its bin in index order (the bin
+     * stores them in index order and the GPU cuts that whole list into chunks):
+     * one counting pass over points 0 .. max index */
+    int maxp = 0;
+    for (int l = 0; l < nentries; ++l) maxp = indices[l] + 1 > maxp ? indices[l] + 1 : maxp;
+    int bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    for (int q = 0; q < maxp; ++q)
+        for (int d = 0; d < 3; ++d)
+        {
+            const int bq = floor_div(cell[3 * q + d] - domain_lower[d], bin[d]);
+            if (q == 0 || bq < bmin[d]) bmin[d] = bq;
+            if (q == 0 || bq > bmax[d]) bmax[d] = bq;
+        }
+    const long nbx = bmax[0] - bmin[0] + 1, nby = bmax[1] - bmin[1] + 1, nbz = bmax[2] - bmin[2] + 1;
+    long* count = (long*)calloc((size_t)(maxp > 0 ? nbx * nby * nbz : 1), sizeof(long));
+    long* rank_of = (long*)malloc(sizeof(long) * (size_t)(maxp > 0 ? maxp : 1));
+    for (int q = 0; q < maxp; ++q)
+    {
+        long id = 0;
+        for (int d = 2; d >= 0; --d)
+        {
+            const long nbd = d == 0 ? nbx : (d == 1 ? nby : nbz);
+            id = id * nbd + (floor_div(cell[3 * q + d] - domain_lower[d], bin[d]) - bmin[d]);
+        }
+        rank_of[q] = count[id]++;
+    }
     long group = -1;
     int prev_b[3] = {0, 0, 0}, prev_o[3] = {0, 0, 0};
     for (int l = 0; l < nentries; ++l)
@@ -585,15 +611,7 @@ void orc_order_entries_z(int kernel, const int* cell, const int* indices, const 
                 prev_o[d] = o[d];
             }
         }
-        /* rank of p among all points of its bin (the bin stores them in index
-         * order and the GPU cuts that whole list into chunks) */
-        long rank = 0;
-        for (int q = 0; q < p; ++q)
-        {
-            int same = 1;
-            for (int d = 0; d < 3 && same; ++d) same = floor_div(cell[3 * q + d] - domain_lower[d], bin[d]) == b[d];
-            rank += same;
-        }
+        const long rank = rank_of[p];
         double w[8];
         const double Xs = X[3 * p + 2] + Xshift[3 * l + 2];
         z[l].key[0] = group;
@@ -604,6 +622,8 @@ void orc_order_entries_z(int kernel, const int* cell, const int* indices, const 
     }
     qsort(z, (size_t)nentries, sizeof(zkey_t), cmp_zkey);
     for (int l = 0; l < nentries; ++l) perm[l] = z[l].pos;
+    free(rank_of);
+    free(count);
     free(z);
 }
 

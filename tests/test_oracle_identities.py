@@ -262,3 +262,32 @@ def test_side_axis_is_shifted_cell_kernel(orc):
         g = np.zeros((1,) + f[a].shape)
         orc.spread(kernel, dx, xl_a, il, iu_a, [gw] * 3, g, X, F[:, a].copy())
         assert np.array_equal(g[0], f[a])
+
+
+def test_order_entries_z_matches_definition(orc):
+    """The spread order of the oracle (group, rank-in-bin // 64, stencil z start,
+    point) against a direct O(n^2) evaluation of that definition, with bins
+    crowded past one 64-point chunk and periodic images."""
+    rng = np.random.default_rng(41)
+    n, N, h, kernel = 700, 16, 1.0 / 16, "IB_6"
+    X = rng.uniform(0.0, 1.0, (n, 3))
+    X[:300] = rng.uniform(0.0, 3 * h, (300, 3))   # >64 points in the corner bin
+    il, iu = [0] * 3, [N - 1] * 3
+    cells = orc.get_cell_index(X, [0.0] * 3, [1.0] * 3, [h] * 3, il, iu)
+    idx, sh = orc.build_patch_entries(cells, il, iu, 4, [N] * 3, il, [h] * 3, [8] * 3)
+    xl = [0.0, 0.0, -0.5 * h]
+    perm = orc.order_entries_z(kernel, cells, idx, sh, X, xl, [h] * 3, il, [8] * 3)
+    b = cells // 8
+    keys = []
+    group, prev = -1, None
+    for l, p in enumerate(idx):
+        o = tuple(int(round(v / h)) for v in sh[l])
+        g = (tuple(b[p]), o)
+        if g != prev:
+            group += 1
+            prev = g
+        rank = int(np.sum(np.all(b[:p] == b[p], axis=1)))
+        z0 = orc.weights_1d(kernel, X[p, 2] + sh[l, 2], xl[2], h)[0]
+        keys.append((group, rank // 64, z0, p, l))
+    want = [k[-1] for k in sorted(keys)]
+    assert list(perm) == want
