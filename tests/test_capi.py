@@ -90,3 +90,48 @@ def test_plan_create_rejects_small_periodic_domain(L):
     with pytest.raises(ia.IBAMRError) as e:
         ia.LevelPlan(_level3(n=3), kernel="IB_6", centering=ia.CELL, max_points=10)
     assert e.value.code == ia.IBAMR_ERR_INVALID
+
+
+def test_registry_full_reference_table(L):
+    """All 22 names of LEInteractor::isKnownKernel with the stencil sizes of
+    getStencilSize (LEInteractor.cpp:2035-2108); names without a GPU kernel are
+    known but refused by plan creation."""
+    import ibamr_amd as ia
+    table = {"BSPLINE_3": 4, "BSPLINE_4": 4, "BSPLINE_5": 6, "BSPLINE_6": 6, "COMPOSITE_BSPLINE_23": 4,
+             "COMPOSITE_BSPLINE_32": 4, "COMPOSITE_BSPLINE_34": 4, "COMPOSITE_BSPLINE_43": 4,
+             "COMPOSITE_BSPLINE_45": 5, "COMPOSITE_BSPLINE_54": 5, "COMPOSITE_BSPLINE_56": 6,
+             "COMPOSITE_BSPLINE_65": 6, "DISCONTINUOUS_LINEAR": 2, "PIECEWISE_CONSTANT": 1, "PIECEWISE_LINEAR": 2,
+             "PIECEWISE_CUBIC": 4, "IB_3": 4, "IB_4": 4, "IB_4_W8": 8, "IB_5": 6, "IB_6": 6}
+    for k, s in table.items():
+        assert ia.is_known_kernel(k), k
+        assert ia.get_stencil_size(k) == s, k
+        assert ia.get_minimum_ghost_width(k) == s // 2 + 1, k
+    assert ia.is_known_kernel("USER_DEFINED")
+    for k in ("IB_3", "BSPLINE_4", "PIECEWISE_CUBIC"):
+        with pytest.raises(ia.IBAMRError) as e:
+            ia.LevelPlan(_level3(), k, centering=ia.SIDE, max_points=10)
+        assert e.value.code == ia.IBAMR_ERR_BAD_KERNEL
+
+
+def test_cpp_driver_reports_reference_errors():
+    """The C++ layer raises the reference's TBOX_ERROR text (LEInteractor.cpp:
+    2103-2104) before any GPU call; bad usage exits 2."""
+    drv = os.path.join(ROOT, "tests", "cpp", "bin", "interpolate_01")
+    if not os.path.exists(drv):
+        from ibamr_amd import build
+        build.build_drivers()
+    r = subprocess.run([drv, "interp", "IB_7", os.devnull], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "LEInteractor::getStencilSize()" in r.stderr and "Unknown kernel function IB_7" in r.stderr
+    r = subprocess.run([drv], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+
+
+def test_patch_plan_rejects_bad_box(L):
+    import ctypes
+    import ibamr_amd as ia
+    lv = ia.Level([0.1] * 3, [0] * 3, [9] * 3, [0.0] * 3, [1.0] * 3, [0] * 3,
+                  [ia.Patch([0] * 3, [9] * 3, [0.0] * 3, [1.0] * 3)])
+    with pytest.raises(ia.IBAMRError) as e:
+        ia.LevelPlan(lv, "IB_4", ia.SIDE, max_points=10, box=([5] * 3, [4] * 3))
+    assert e.value.code == ia.IBAMR_ERR_INVALID
