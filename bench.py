@@ -154,7 +154,7 @@ def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
         plan.spread(0, Fd, X, mode=ia.OVERWRITE, stream=s)
         if ev:
             ev[1].record(s)
-        plan.interp(1, U, X, stream=s)
+        plan.interp(1, U, None, stream=s)   # same positions: reuse the spread's bin-ordered copy
         if ev:
             ev[2].record(s)
 

@@ -45,7 +45,7 @@ C_API = [
     "ibamr_plan_info",
     "ibamr_last_error",
     "ibamr_s0_interp", "ibamr_s0_spread",
-] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3")
+] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear")
      for op in ("interp", "spread") for d in ("2d", "3d")]
 
 

@@ -194,10 +194,11 @@ int kid(const char* k)
     if (!std::strcmp(k, "IB_4")) return K_IB4;
     if (!std::strcmp(k, "IB_6")) return K_IB6;
     if (!std::strcmp(k, "BSPLINE_3")) return K_BS3;
+    if (!std::strcmp(k, "PIECEWISE_LINEAR")) return K_PWL;
     return -1;
 }
 
-int width(int k) { return k == K_IB4 ? 4 : (k == K_IB6 ? 6 : 3); }
+int width(int k) { return k == K_IB4 ? 4 : (k == K_IB6 ? 6 : (k == K_BS3 ? 3 : 2)); }
 
 struct DevBuf
 {
@@ -249,6 +250,7 @@ int by_kernel(int ndim, int k, F&& f)
         case K_IB4: return f(nd, std::integral_constant<int, K_IB4>{});
         case K_IB6: return f(nd, std::integral_constant<int, K_IB6>{});
         case K_BS3: return f(nd, std::integral_constant<int, K_BS3>{});
+        case K_PWL: return f(nd, std::integral_constant<int, K_PWL>{});
         default: return set_error(IBAMR_ERR_BAD_KERNEL, "unknown kernel function");
         }
     };
@@ -439,3 +441,7 @@ S0_INTERP2(lagrangian_ib_6_interp2d_, "IB_6")
 S0_SPREAD2(lagrangian_ib_6_spread2d_, "IB_6")
 S0_INTERP2(lagrangian_bspline_3_interp2d_, "BSPLINE_3")
 S0_SPREAD2(lagrangian_bspline_3_spread2d_, "BSPLINE_3")
+S0_INTERP3(lagrangian_piecewise_linear_interp3d_, "PIECEWISE_LINEAR")
+S0_SPREAD3(lagrangian_piecewise_linear_spread3d_, "PIECEWISE_LINEAR")
+S0_INTERP2(lagrangian_piecewise_linear_interp2d_, "PIECEWISE_LINEAR")
+S0_SPREAD2(lagrangian_piecewise_linear_spread2d_, "PIECEWISE_LINEAR")

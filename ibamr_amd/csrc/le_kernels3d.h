@@ -654,26 +654,32 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                 int i1 = r / e[0];
                 int i0 = r - i1 * e[0];
                 const int st1 = STEP / e[0], st0 = STEP - st1 * e[0];
-                const unsigned int* ub = reinterpret_cast<const unsigned int*>(
-                    u + (size_t)(blo[0] - alo[0]) + (size_t)an[0] * (blo[1] - alo[1]) + plane * (size_t)(blo[2] - alo[2]));
+                const T* ub = u + (size_t)(blo[0] - alo[0]) + (size_t)an[0] * (blo[1] - alo[1]) +
+                              plane * (size_t)(blo[2] - alo[2]);
+                // offset of this thread's element in the array, stepped incrementally:
+                // +d0 per round, +c0 when i0 wraps (one more row), +c1 per i1 wrap (one more plane)
+                long off = (long)i0 + (long)an[0] * i1 + (long)plane * i2;
+                const long d0 = (long)st0 + (long)an[0] * st1;
+                const long c0 = (long)an[0] - e[0];
+                const long c1 = (long)plane - (long)e[1] * an[0];
                 for (int base = wv * 64; base < ndw; base += I3_THREADS)
                 {
                     if (base + lane < ndw)
-                    {
-                        const size_t g = (size_t)i0 + (size_t)an[0] * i1 + plane * (size_t)i2;
-                        __builtin_amdgcn_global_load_lds(ub + g * UNIT + half, s_dw + base, 4, 0, 0);
-                    }
+                        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned int*>(ub + off) + half,
+                                                         s_dw + base, 4, 0, 0);
                     i0 += st0;
                     i1 += st1;
+                    off += d0;
                     if (i0 >= e[0])
                     {
                         i0 -= e[0];
                         ++i1;
+                        off += c0;
                     }
                     while (i1 >= e[1])
                     {
                         i1 -= e[1];
-                        ++i2;
+                        off += c1;
                     }
                 }
                 __builtin_amdgcn_s_waitcnt(0);
@@ -708,6 +714,10 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                     const T* base = s_u + (bse[0] - blo[0]) + e[0] * ((bse[1] - blo[1]) + e[1] * (bse[2] - blo[2]));
 #pragma unroll
                     for (int i2 = 0; i2 < W; ++i2)
+                    {
+                        // FP32C: rows as FMA chains, a plain sum over the plane's rows,
+                        // one compensated add per plane
+                        T pl = T(0);
 #pragma unroll
                         for (int i1 = 0; i1 < W; ++i1)
                         {
@@ -724,9 +734,11 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                                 T sum = T(0);
 #pragma unroll
                                 for (int i0 = 0; i0 < W; ++i0) sum = fmaf(w[0][i0], row[i0], sum);
-                                acc.add((w[1][i1] * w[2][i2]) * sum);
+                                pl = fmaf(w[1][i1], sum, pl);
                             }
                         }
+                        if (!FP64) acc.add(w[2][i2] * pl);
+                    }
                 }
                 else
                 {

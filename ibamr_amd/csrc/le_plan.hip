@@ -190,6 +190,7 @@ struct ibamr_plan
     int4* cell_d = nullptr;
     int4* cell_sorted_d = nullptr;
     double* Xs_d = nullptr;            // positions in bin order
+    bool xs_valid = false;             // Xs holds the positions of the last spread/interp (same binning)
     void* Fs_d = nullptr;              // forces in bin order
     int* bin_start_d = nullptr;        // [nmem+2]
 
@@ -966,6 +967,7 @@ int dispatch(int ndim, int kernel, int precision, F&& f)
         case K_IB4: return f(nd, std::integral_constant<int, K_IB4>{}, prec);
         case K_IB6: return f(nd, std::integral_constant<int, K_IB6>{}, prec);
         case K_BS3: return f(nd, std::integral_constant<int, K_BS3>{}, prec);
+        case K_PWL: return f(nd, std::integral_constant<int, K_PWL>{}, prec);
         default: return fail(IBAMR_ERR_BAD_KERNEL, "kernel has no GPU spread/interp");
         }
     };
@@ -1089,9 +1091,10 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     *out = nullptr;
     const int k = kernel_id(C->kernel);
     if (k == -1) return fail(IBAMR_ERR_BAD_KERNEL, std::string("Unknown kernel function ") + (C->kernel ? C->kernel : "(null)"));
-    if (k < 0 || k == K_PWL)
+    if (k < 0)
         return fail(IBAMR_ERR_BAD_KERNEL, std::string("kernel function ") + C->kernel +
-                                              " has no GPU spread/interpolation (IB_4, IB_6 and BSPLINE_3 do)");
+                                              " has no GPU spread/interpolation (PIECEWISE_LINEAR, IB_4, IB_6 and "
+                                              "BSPLINE_3 do)");
     if (L->ndim != 2 && L->ndim != 3) return fail(IBAMR_ERR_INVALID, "ndim must be 2 or 3");
     if (C->centering != IBAMR_CELL && C->centering != IBAMR_SIDE) return fail(IBAMR_ERR_INVALID, "bad centering");
     if (C->precision != IBAMR_FP64 && C->precision != IBAMR_FP32C) return fail(IBAMR_ERR_INVALID, "bad precision");
@@ -1376,6 +1379,7 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     if (npoints > 0 && !X) return fail(IBAMR_ERR_INVALID, "null X");
     hipStream_t s = static_cast<hipStream_t>(stream);
     P->npoints = npoints;
+    P->xs_valid = false;
     const int B = 256;
     const int m = P->nmem + 1;  // bins plus the out-of-domain sentinel
     if (npoints == 0)
@@ -1532,6 +1536,7 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
                                P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
                                static_cast<T*>(P->Fs_d));
             HIP_OK(hipGetLastError());
+            P->xs_valid = true;
         }
         a.X = P->Xs_d;
         a.F = static_cast<const T*>(P->Fs_d);
@@ -1559,7 +1564,9 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
     if (slot < 0 || slot >= MAX_SLOTS || !P->arrays_d[slot]) return fail(IBAMR_ERR_INVALID, "slot not bound");
     if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
     if (P->npoints == 0) return IBAMR_OK;
-    if (!U || !X) return fail(IBAMR_ERR_INVALID, "null U or X");
+    if (!U) return fail(IBAMR_ERR_INVALID, "null U");
+    if (!X && !P->xs_valid)
+        return fail(IBAMR_ERR_INVALID, "X is null and no spread/interp since the last binning permuted positions");
     hipStream_t s = static_cast<hipStream_t>(stream);
     return dispatch(P->ndim, P->kernel, P->precision, [&](auto nd, auto ker, auto prec) -> int {
         using T = decltype(prec);
@@ -1572,12 +1579,14 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
         a.bin_patches = P->bin_patches_d;
         a.arrays = reinterpret_cast<T* const*>(P->arrays_d[slot]);
         a.perm = P->perm_d;
+        if (X)  // else: reuse the positions the last spread/interp put in bin order
         {
             const int B = 256;
             hipLaunchKernelGGL(k_permute_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
                                P->npoints, NDIM, X, P->Xs_d, 0, static_cast<const T*>(nullptr),
                                static_cast<T*>(nullptr));
             HIP_OK(hipGetLastError());
+            P->xs_valid = true;
         }
         a.X = P->Xs_d;
         a.U = static_cast<T*>(U);

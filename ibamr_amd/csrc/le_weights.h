@@ -47,6 +47,13 @@ struct KernelTraits<K_BS3>
     static constexpr bool IB = false;
     static constexpr int GHOST = 3;
 };
+template <>
+struct KernelTraits<K_PWL>
+{
+    static constexpr int W = 2;
+    static constexpr bool IB = false;  // (((w0*w1)*w2)*V)/vol (f.m4:717-721)
+    static constexpr int GHOST = 2;
+};
 
 // Fortran NINT: round half away from zero.
 __device__ __forceinline__ int nint_d(double x) { return (int)round(x); }
@@ -189,6 +196,25 @@ __device__ __forceinline__ int weights_1d<K_BS3>(double Xs, double x_lower, doub
     return lo;
 }
 
+// PIECEWISE_LINEAR, lagrangian_interaction3d.f.m4:680-699 (spread) / 556-575
+// (interp): the cell whose centre is nearest, then the linear hat on the two
+// cells straddling X.
+template <>
+__device__ __forceinline__ int weights_1d<K_PWL>(double Xs, double x_lower, double dx, double* w)
+{
+    const int c = nint_d((Xs - x_lower) / dx - 0.5);
+    const double X_cell = x_lower + ((double)c + 0.5) * dx;
+    if (Xs < X_cell)
+    {
+        w[0] = (X_cell - Xs) / dx;
+        w[1] = 1.0 - w[0];
+        return c - 1;
+    }
+    w[0] = 1.0 + (X_cell - Xs) / dx;
+    w[1] = 1.0 - w[0];
+    return c;
+}
+
 // FP32 1-D weights: index selection and the offset r in FP64, weights in FP32.
 template <int K>
 __device__ __forceinline__ int weights_1d(double Xs, double x_lower, double dx, float* w);
@@ -227,6 +253,16 @@ __device__ __forceinline__ int weights_1d<K_BS3>(double Xs, double x_lower, doub
     const float t = (float)(X_o_dx - ((double)c + 0.5));
 #pragma unroll
     for (int k = 0; k < 3; ++k) w[k] = bspline3_delta_f(t - (float)(k - 1));
+    return lo;
+}
+
+template <>
+__device__ __forceinline__ int weights_1d<K_PWL>(double Xs, double x_lower, double dx, float* w)
+{
+    double wd[2];
+    const int lo = weights_1d<K_PWL>(Xs, x_lower, dx, wd);
+    w[0] = (float)wd[0];
+    w[1] = (float)wd[1];
     return lo;
 }
 

@@ -155,7 +155,11 @@ int ibamr_plan_spread(ibamr_plan* plan, int slot, const void* F, const double* X
 /* LDataManager::interp for one level (LDataManager.cpp:748-802): each point
  * is interpolated from the patch whose interior holds its cell; ghost values
  * must be filled beforehand (see ibamr_plan_fill_periodic_ghosts).  U is
- * overwritten for those points. */
+ * overwritten for those points.  X may be NULL when the positions are those of
+ * the previous ibamr_plan_spread / ibamr_plan_interp call on this plan (same
+ * binning): the positions that call put in bin order are reused (a step that
+ * spreads and interpolates at one configuration, e.g. the midpoint of
+ * IBExplicitHierarchyIntegrator). */
 int ibamr_plan_interp(ibamr_plan* plan, int slot, void* U, const double* X, void* stream);
 
 /* Fills the ghost cells of the arrays bound to `slot` from the periodic

@@ -74,6 +74,12 @@ void lagrangian_ib_6_spread2d_(IBAMR_S0_SPREAD2D_ARGS);
 /* lagrangian_interaction2d.f.m4:2300 / :2410 */
 void lagrangian_bspline_3_interp2d_(IBAMR_S0_INTERP2D_ARGS);
 void lagrangian_bspline_3_spread2d_(IBAMR_S0_SPREAD2D_ARGS);
+/* lagrangian_interaction3d.f.m4:493 / :617 */
+void lagrangian_piecewise_linear_interp3d_(IBAMR_S0_INTERP3D_ARGS);
+void lagrangian_piecewise_linear_spread3d_(IBAMR_S0_SPREAD3D_ARGS);
+/* lagrangian_interaction2d.f.m4:469 / :587 */
+void lagrangian_piecewise_linear_interp2d_(IBAMR_S0_INTERP2D_ARGS);
+void lagrangian_piecewise_linear_spread2d_(IBAMR_S0_SPREAD2D_ARGS);
 
 /* Status-returning forms of the same operations (no abort), for callers that
  * want an error code: kernel is "IB_4" | "IB_6" | "BSPLINE_3", ndim 2 or 3,

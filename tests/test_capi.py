@@ -70,7 +70,7 @@ def _level3(n=8, periodic=True):
 
 @pytest.mark.parametrize("kw,code_name", [
     (dict(kernel="IB_7"), "IBAMR_ERR_BAD_KERNEL"),
-    (dict(kernel="PIECEWISE_LINEAR"), "IBAMR_ERR_BAD_KERNEL"),
+    (dict(kernel="IB_4_W8"), "IBAMR_ERR_BAD_KERNEL"),
     (dict(kernel="IB_6", data_gcw=[3, 3, 3]), "IBAMR_ERR_GHOST_WIDTH"),
     (dict(kernel="IB_4", data_gcw=[3, 2, 3]), "IBAMR_ERR_GHOST_WIDTH"),
     (dict(kernel="IB_4", idx_gcw=2), "IBAMR_ERR_GHOST_WIDTH"),

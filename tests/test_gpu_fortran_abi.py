@@ -32,7 +32,7 @@ def _case(ndim, n, gw, npts, seed):
 
 
 @pytest.mark.parametrize("ndim", [2, 3])
-@pytest.mark.parametrize("kernel", ["IB_4", "IB_6", "BSPLINE_3"])
+@pytest.mark.parametrize("kernel", ["IB_4", "IB_6", "BSPLINE_3", "PIECEWISE_LINEAR"])
 @pytest.mark.parametrize("depth", [1, 2])
 def test_s0_bitwise(orc, ndim, kernel, depth):
     print("s0", ndim, kernel, depth, flush=True)

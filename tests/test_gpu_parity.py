@@ -248,6 +248,14 @@ def test_multi_patch_level_fp64(orc):
     Uref = _oracle_interp(orc, level, plan, init, X, kernel)
     torch.cuda.synchronize()
     assert np.array_equal(U.cpu().numpy(), Uref)
+    # X = NULL: the positions the last call put in bin order are reused
+    U2 = torch.full((npts, 3), float("nan"), dtype=torch.float64, device=DEV)
+    plan.interp(1, U2, None)
+    torch.cuda.synchronize()
+    assert torch.equal(U2, U)
+    plan.bin(Xd)  # a new binning invalidates them
+    with pytest.raises(ia.IBAMRError):
+        plan.interp(1, U2, None)
 
 
 def test_out_of_domain_reported(orc):

@@ -111,3 +111,31 @@ def test_ghost_accumulation_3d_side_spread_golden(orc):
                 assert abs(got - v) <= PRINT_TOL * abs(v), (a, i0, i1, i2, got, v)
             n += 1
     assert n == 3 * 9 * 8 * 8
+
+
+@pytest.mark.parametrize("case", ["2d.cell.spread.a", "2d.cell.spread.b", "2d.side.spread.a", "2d.side.spread.b",
+                                  "3d.cell.spread.a", "3d.cell.spread.b", "3d.side.spread.a", "3d.side.spread.b"])
+def test_ghost_accumulation_all_cases_golden(orc, case):
+    """tests/IBTK/ghost_accumulation_01 spread cases (PIECEWISE_LINEAR, one or
+    several patches, periodic or not, cell and side data): the reference spreads
+    each patch's points over the patch, then SAMRAIGhostDataAccumulator sums every
+    degree of freedom over all patches' (ghosted) copies.  The oracle instead
+    spreads, for each patch, every point of the level within the patch's index
+    ghost box (with periodic images) -- the LDataManager::spread model the GPU
+    level plans use.  Both must give the same patch values (to summation order)."""
+    import ghost_cases as gc
+    c = gc.load(case)
+    keep = gc.level_points(orc, c)
+    X, Q = c["X"][keep], c["Q"][keep]
+    ndim, h = c["ndim"], c["h"]
+    cells = orc.get_cell_index(X, [0.0] * ndim, [1.0] * ndim, [h] * ndim, [0] * ndim, [c["nf"] - 1] * ndim)
+    n = 0
+    for pi, (lo, hi, xl, xu) in enumerate(c["boxes"]):
+        if c["side"]:
+            arrays = [np.zeros(orc.side_axis_shape(lo, hi, c["gcw"], a)) for a in range(ndim)]
+        else:
+            arrays = np.zeros(orc.ghosted_shape(lo, hi, c["gcw"], depth=Q.shape[1]))
+        orc.spread_patch(c["kernel"], c["side"], [h] * ndim, xl, lo, hi, c["gcw"], arrays, X, Q, cells,
+                         c["gcw"][0], c["pshift"], [0] * ndim, depth=Q.shape[1])
+        n += gc.compare(c, pi, arrays if c["side"] else [arrays])
+    assert n > 0
