@@ -41,7 +41,7 @@ C_API = [
     "ibamr_is_known_kernel", "ibamr_get_stencil_size", "ibamr_get_minimum_ghost_width",
     "ibamr_plan_create", "ibamr_plan_create_patch", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
-    "ibamr_plan_fill_periodic_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
+    "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
     "ibamr_plan_info",
     "ibamr_last_error",
     "ibamr_s0_interp", "ibamr_s0_spread",
@@ -106,6 +106,7 @@ def lib():
     L.ibamr_plan_spread.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
     L.ibamr_plan_interp.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     L.ibamr_plan_fill_periodic_ghosts.argtypes = [vp, ctypes.c_int, vp]
+    L.ibamr_plan_fill_ghosts.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
@@ -292,6 +293,10 @@ class LevelPlan:
 
     def fill_periodic_ghosts(self, slot, stream=None):
         _check(lib().ibamr_plan_fill_periodic_ghosts(self._h, slot, _stream_ptr(stream)))
+
+    def fill_ghosts(self, slot, stream=None):
+        """Same-level ghost fill between the level's patches (periodic wrap included)."""
+        _check(lib().ibamr_plan_fill_ghosts(self._h, slot, _stream_ptr(stream)))
 
     def set_timing(self, on=True):
         """HIP events around each spread / interpolation kernel launch (measurement only)."""

@@ -583,6 +583,17 @@ void LDataManager::fillPeriodicGhosts(std::vector<std::shared_ptr<SideData<doubl
     abi_check(ibamr_plan_fill_periodic_ghosts(d_impl->interp_plan, 1, nullptr), "LDataManager::fillPeriodicGhosts");
 }
 
+void LDataManager::fillGhosts(std::vector<std::shared_ptr<SideData<double>>>& u_data)
+{
+    std::vector<void*> a = side_arrays(u_data, d_impl->ndim, d_impl->level.patches.size());
+    if (a != d_impl->bound[1])
+    {
+        abi_check(ibamr_plan_bind_data(d_impl->interp_plan, 1, a.data()), "LDataManager::fillGhosts");
+        d_impl->bound[1] = a;
+    }
+    abi_check(ibamr_plan_fill_ghosts(d_impl->interp_plan, 1, nullptr), "LDataManager::fillGhosts");
+}
+
 void IBMethod::forwardEulerStep(LData<double>& X_new, const LData<double>& X, const LData<double>& U, double dt)
 {
     const long n = (long)X.getLocalNodeCount() * X.getDepth();

@@ -178,6 +178,8 @@ struct ibamr_plan
     int ntiles = 0;
     int* bin_patch_start_d = nullptr;  // [nmem+2]
     int* bin_patches_d = nullptr;
+    int4* fill_segs_d = nullptr;       // level ghost fill: (patch, array, first element, count) segments
+    int nfill_segs = 0;
     void** arrays_d[MAX_SLOTS] = {};
     std::vector<void*> arrays_h[MAX_SLOTS];   // host copy of each slot's pointer table
     int* keys_d = nullptr;
@@ -950,6 +952,108 @@ __global__ void k_fill_periodic(T* __restrict__ arr, int n0, int n1, int n2, int
     arr[e] = arr[(long)j0 + (long)n0 * (j1 + (long)n1 * j2)];
 }
 
+// Same-level ghost fill (the RefineSchedule::fillData step before interp,
+// LDataManager.cpp:742-745, restricted to copies between patches of one level):
+// every ghost value of every patch array whose index -- wrapped across periodic
+// directions -- lies in another patch's (or, periodically, its own) interior
+// side box is copied from there.  Ghost values outside the domain of a
+// non-periodic direction, or not covered by the level (coarse-fine boundary),
+// are left as they are.
+template <typename T>
+struct FillArgs
+{
+    const int4* segs;
+    const PatchDev* patches;
+    T* const* arrays;
+    const int* bin_patch_start;
+    const int* bin_patches;
+    int ndim, narr, side, depth;
+    int gcw[3];
+    int dom_lo[3], dom_hi[3];
+    int pshift[3];
+    int nb[3], nsb[3];
+    int bin;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_level(FillArgs<T> a)
+{
+    const int4 sg = a.segs[blockIdx.x];
+    if ((int)threadIdx.x >= sg.w) return;
+    const int pid = sg.x, arr_i = sg.y;
+    const int axis = a.side ? arr_i : -1;
+    const PatchDev P = a.patches[pid];
+    int n[3] = {1, 1, 1}, lo[3] = {0, 0, 0};
+    for (int d = 0; d < a.ndim; ++d)
+    {
+        lo[d] = P.ilo[d] - a.gcw[d];
+        n[d] = P.ihi[d] - P.ilo[d] + 1 + 2 * a.gcw[d] + (d == axis ? 1 : 0);
+    }
+    const long e = (long)sg.z + threadIdx.x;
+    int idx[3] = {(int)(e % n[0]), (int)((e / n[0]) % n[1]), (int)(e / ((long)n[0] * n[1]))};
+    bool interior = true;
+    int w[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d)
+    {
+        if (d >= a.ndim)
+            continue;
+        const int i = lo[d] + idx[d];
+        const int top = P.ihi[d] + (d == axis ? 1 : 0);
+        interior = interior && i >= P.ilo[d] && i <= top;
+        const int N = a.pshift[d];
+        const int dtop = a.dom_hi[d] + (d == axis ? 1 : 0);
+        if (N > 0)
+            w[d] = a.dom_lo[d] + (((i - a.dom_lo[d]) % N) + N) % N;
+        else
+        {
+            if (i < a.dom_lo[d] || i > dtop) return;  // physical boundary ghost: not filled here
+            w[d] = i;
+        }
+    }
+    if (interior) return;
+    // owner: a patch whose interior side box holds the wrapped index; its cell,
+    // or for a side axis also the cell below (the face is that cell's upper face)
+    int q = -1;
+    for (int tries = 0; tries < 2 && q < 0; ++tries)
+    {
+        if (tries == 1 && axis < 0) break;
+        int c[3] = {0, 0, 0};
+        bool ok = true;
+        for (int d = 0; d < a.ndim; ++d)
+        {
+            c[d] = w[d] - (tries == 1 && d == axis ? 1 : 0);
+            ok = ok && c[d] >= a.dom_lo[d] && c[d] <= a.dom_hi[d];
+        }
+        if (!ok) continue;
+        int b[3] = {0, 0, 0};
+        for (int d = 0; d < a.ndim; ++d) b[d] = (c[d] - a.dom_lo[d]) / a.bin;
+        const int key = bin_mem(a.ndim, b[0], b[1], b[2], a.nb, a.nsb);
+        for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1] && q < 0; ++j)
+        {
+            const PatchDev& Q = a.patches[a.bin_patches[j]];
+            bool in = true;
+            for (int d = 0; d < a.ndim; ++d)
+                in = in && w[d] >= Q.ilo[d] && w[d] <= Q.ihi[d] + (d == axis ? 1 : 0);
+            if (in) q = a.bin_patches[j];
+        }
+    }
+    if (q < 0) return;  // not covered by this level
+    const PatchDev Q = a.patches[q];
+    int qn[3] = {1, 1, 1};
+    long src = 0, stride = 1;
+    for (int d = 0; d < a.ndim; ++d)
+    {
+        qn[d] = Q.ihi[d] - Q.ilo[d] + 1 + 2 * a.gcw[d] + (d == axis ? 1 : 0);
+        src += stride * (long)(w[d] - (Q.ilo[d] - a.gcw[d]));
+        stride *= qn[d];
+    }
+    const long ptot = (long)n[0] * n[1] * n[2], qtot = stride;
+    T* __restrict__ dst = a.arrays[pid * a.narr + arr_i];
+    const T* __restrict__ from = a.arrays[q * a.narr + arr_i];
+    const int ncomp = a.side ? 1 : a.depth;
+    for (int cd = 0; cd < ncomp; ++cd) dst[(long)cd * ptot + e] = from[(long)cd * qtot + src];
+}
+
 } // namespace
 
 #include "le_kernels3d.h"
@@ -1058,6 +1162,7 @@ int ibamr_plan_destroy(ibamr_plan* P)
     dfree(P->patches_d);
     dfree(P->tiles_d);
     dfree(P->bin_patch_start_d);
+    dfree(P->fill_segs_d);
     dfree(P->bin_patches_d);
     for (int s = 0; s < MAX_SLOTS; ++s) dfree(P->arrays_d[s]);
     dfree(P->keys_d);
@@ -1258,6 +1363,22 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     }
     P->ntiles = (int)tiles.size();
 
+    // segments of the same-level ghost fill: every element of every patch array
+    std::vector<int4> fsegs;
+    for (int i = 0; i < L->npatches; ++i)
+    {
+        const PatchDev& pd = P->patches_h[i];
+        for (int c = 0; c < P->narr; ++c)
+        {
+            const int axis = C->centering == IBAMR_SIDE ? c : -1;
+            long total = 1;
+            for (int d = 0; d < ndim; ++d) total *= pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
+            for (long st = 0; st < total; st += 256)
+                fsegs.push_back(make_int4(i, c, (int)st, (int)std::min<long>(256, total - st)));
+        }
+    }
+    P->nfill_segs = (int)fsegs.size();
+
     auto cleanup_fail = [&](int code) {
         std::string m = g_err;
         ibamr_plan_destroy(P);
@@ -1282,6 +1403,9 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
         PLAN_HIP(hipMemcpy(P->tiles_d, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
     PLAN_HIP(hipMalloc(&P->bin_patch_start_d, sizeof(int) * bps.size()));
     PLAN_HIP(hipMemcpy(P->bin_patch_start_d, bps.data(), sizeof(int) * bps.size(), hipMemcpyHostToDevice));
+    PLAN_HIP(hipMalloc(&P->fill_segs_d, sizeof(int4) * std::max<size_t>(1, fsegs.size())));
+    if (!fsegs.empty())
+        PLAN_HIP(hipMemcpy(P->fill_segs_d, fsegs.data(), sizeof(int4) * fsegs.size(), hipMemcpyHostToDevice));
     PLAN_HIP(hipMalloc(&P->bin_patches_d, sizeof(int) * bpl.size()));
     PLAN_HIP(hipMemcpy(P->bin_patches_d, bpl.data(), sizeof(int) * bpl.size(), hipMemcpyHostToDevice));
     const size_t np = (size_t)std::max(1, C->max_points);
@@ -1693,6 +1817,41 @@ int ibamr_plan_fill_periodic_ghosts(ibamr_plan* P, int slot, void* stream)
         }
     }
     return IBAMR_OK;
+}
+
+int ibamr_plan_fill_ghosts(ibamr_plan* P, int slot, void* stream)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    if (slot < 0 || slot >= MAX_SLOTS || !P->arrays_d[slot]) return fail(IBAMR_ERR_INVALID, "slot not bound");
+    if (P->nfill_segs == 0) return IBAMR_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto go = [&](auto tag) -> int {
+        using T = decltype(tag);
+        FillArgs<T> a{};
+        a.segs = P->fill_segs_d;
+        a.patches = P->patches_d;
+        a.arrays = reinterpret_cast<T* const*>(P->arrays_d[slot]);
+        a.bin_patch_start = P->bin_patch_start_d;
+        a.bin_patches = P->bin_patches_d;
+        a.ndim = P->ndim;
+        a.narr = P->narr;
+        a.side = P->centering == IBAMR_SIDE;
+        a.depth = P->depth;
+        for (int d = 0; d < 3; ++d)
+        {
+            a.gcw[d] = P->data_gcw[d];
+            a.dom_lo[d] = P->dom_lo[d];
+            a.dom_hi[d] = P->dom_hi[d];
+            a.pshift[d] = P->pshift[d];
+            a.nb[d] = P->nb[d];
+            a.nsb[d] = P->nsb[d];
+        }
+        a.bin = P->bin;
+        hipLaunchKernelGGL(k_fill_level<T>, dim3((unsigned)P->nfill_segs), dim3(256), 0, s, a);
+        HIP_OK(hipGetLastError());
+        return IBAMR_OK;
+    };
+    return P->precision == IBAMR_FP64 ? go((double)0) : go((float)0);
 }
 
 } // extern "C"

@@ -167,6 +167,16 @@ int ibamr_plan_interp(ibamr_plan* plan, int slot, void* U, const double* X, void
  * RefineSchedule::fillData step before interp, LDataManager.cpp:742-745). */
 int ibamr_plan_fill_periodic_ghosts(ibamr_plan* plan, int slot, void* stream);
 
+/* Same-level ghost fill for a level of patches (the RefineSchedule::fillData
+ * before interp, LDataManager.cpp:742-745, between patches of one level): each
+ * ghost value whose index -- wrapped across periodic directions -- lies in the
+ * interior (side box for side data) of a patch of the level is copied from it.
+ * Ghost values beyond a non-periodic domain face or at coarse-fine boundaries
+ * are left unchanged.  Unlike ibamr_plan_fill_periodic_ghosts, interior values
+ * (including the face N of a periodic side axis) are never written.  Kernel
+ * only (graph-safe). */
+int ibamr_plan_fill_ghosts(ibamr_plan* plan, int slot, void* stream);
+
 /* IBMethod::forwardEulerStep (IBMethod.cpp:713-739): X_new = X + dt U over
  * n values (n = ndim * points), on the stream; X_new may be X.  U is double
  * (u_precision IBAMR_FP64) or float (IBAMR_FP32C).  Kernel only: graph-safe. */

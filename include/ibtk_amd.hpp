@@ -225,6 +225,10 @@ public:
     void interp(const std::vector<std::shared_ptr<SideData<double>>>& u_data, LData<double>& U_data,
                 const LData<double>& X_data);
     void fillPeriodicGhosts(std::vector<std::shared_ptr<SideData<double>>>& u_data);
+    // same-level ghost fill between the level's patches, periodic wrap included
+    // (the RefineSchedule::fillData before interp, LDataManager.cpp:742-745,
+    // without coarse-fine interpolation)
+    void fillGhosts(std::vector<std::shared_ptr<SideData<double>>>& u_data);
 
 private:
     struct Impl;
@@ -244,9 +248,9 @@ public:
     }
     // IBStrategy::interpolateVelocity (IBStrategy.h:276)
     void interpolateVelocity(std::vector<std::shared_ptr<SideData<double>>>& u_data, LData<double>& U,
-                             const LData<double>& X, bool periodic_single_patch)
+                             const LData<double>& X, bool fill_ghosts = true)
     {
-        if (periodic_single_patch) d_ldm->fillPeriodicGhosts(u_data);
+        if (fill_ghosts) d_ldm->fillGhosts(u_data);
         d_ldm->interp(u_data, U, X);
     }
     // IBMethod::forwardEulerStep (IBMethod.cpp:713-739): X_new = X + dt U

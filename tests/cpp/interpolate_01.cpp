@@ -176,7 +176,7 @@ int run_level(const std::string& k, const std::string& out_name)
     F.copyFromHost(Fh);
     ldm->endDataRedistribution(X);
     ib.spreadForce(f, F, X);
-    ib.interpolateVelocity(u, U, X, /*periodic_single_patch*/ true);
+    ib.interpolateVelocity(u, U, X);  // same-level ghost fill (periodic wrap), then interp
     IBMethod::forwardEulerStep(X_new, X, U, 0.01);
     std::vector<double> Uh, Xn;
     U.copyToHost(Uh);
