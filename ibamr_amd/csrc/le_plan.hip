@@ -184,7 +184,8 @@ struct ibamr_plan
     std::vector<void*> arrays_h[MAX_SLOTS];   // host copy of each slot's pointer table
     int* keys_d = nullptr;
     int* keys_sorted_d = nullptr;
-    int* idx_d = nullptr;              // point indices in bin order before the per-bin sort
+    int* idx_d = nullptr;              // point indices in bin order before the per-bin sort; afterwards
+                                       // the inverse permutation (slot of every point)
     int* cursor_d = nullptr;           // [nmem+1] scatter cursors
     int* scan_tmp_d = nullptr;         // block sums of the scan
     int* scan_part_d = nullptr;        // [nmem+1] block-local exclusive prefixes
@@ -461,24 +462,31 @@ __global__ __launch_bounds__(BS_THREADS) void k_bin_sort(const int* __restrict__
         for (int i = t; i < n; i += BS_THREADS) perm[s0 + i] = src[i];
 }
 
+// cells in bin order, and the inverse permutation (slot of every point)
 __global__ void k_gather_cells(const int* __restrict__ perm, const int4* __restrict__ cell, int n,
-                               int4* __restrict__ out)
+                               int4* __restrict__ out, int* __restrict__ inv)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = cell[perm[i]];
+    if (i >= n) return;
+    const int p = perm[i];
+    out[i] = cell[p];
+    inv[p] = i;
 }
 
-// Point data in bin order: every later kernel reads slot k coalesced.
+// Point data in bin order, so every later kernel reads slot k coalesced.
+// Driven by the points: read X and F in the caller's order (coalesced) and
+// write each point's record to its slot (a scattered write of a few bytes
+// costs less than a scattered read of whole cache lines).
 template <typename T>
-__global__ void k_permute_points(const int* __restrict__ perm, int n, int ndim, const double* __restrict__ X,
+__global__ void k_scatter_points(const int* __restrict__ inv, int n, int ndim, const double* __restrict__ X,
                                  double* __restrict__ Xs, int fdepth, const T* __restrict__ F, T* __restrict__ Fs)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const size_t p = (size_t)perm[k];
-    for (int d = 0; d < ndim; ++d) Xs[(size_t)ndim * k + d] = X[ndim * p + d];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const size_t k = (size_t)inv[p];
+    for (int d = 0; d < ndim; ++d) Xs[(size_t)ndim * k + d] = X[(size_t)ndim * p + d];
     if (F)
-        for (int c = 0; c < fdepth; ++c) Fs[(size_t)fdepth * k + c] = F[fdepth * p + c];
+        for (int c = 0; c < fdepth; ++c) Fs[(size_t)fdepth * k + c] = F[(size_t)fdepth * p + c];
 }
 
 // ----------------------------------------------------------------------------- spread
@@ -1557,7 +1565,9 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     hipLaunchKernelGGL(k_bin_sort, dim3(m), dim3(BS_THREADS), 0, s, P->bin_start_d, m, P->idx_d, P->perm_d,
                        P->keys_sorted_d);
     HIP_OK(hipGetLastError());
-    hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d);
+    // idx_d is free once the per-bin sort is done: it now holds the inverse permutation
+    hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d,
+                       P->idx_d);
     HIP_OK(hipGetLastError());
     return IBAMR_OK;
 }
@@ -1656,7 +1666,7 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         if (P->npoints > 0)
         {
             const int B = 256;
-            hipLaunchKernelGGL(k_permute_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
+            hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
                                P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
                                static_cast<T*>(P->Fs_d));
             HIP_OK(hipGetLastError());
@@ -1706,7 +1716,7 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
         if (X)  // else: reuse the positions the last spread/interp put in bin order
         {
             const int B = 256;
-            hipLaunchKernelGGL(k_permute_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
+            hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
                                P->npoints, NDIM, X, P->Xs_d, 0, static_cast<const T*>(nullptr),
                                static_cast<T*>(nullptr));
             HIP_OK(hipGetLastError());

@@ -265,3 +265,34 @@ def test_out_of_domain_reported(orc):
     with pytest.raises(ia.IBAMRError) as e:
         plan.bin(X)
     assert e.value.code == ia.IBAMR_ERR_OUT_OF_DOMAIN
+
+
+@pytest.mark.parametrize("precision", [ia.FP64, ia.FP32C])
+def test_spread_interp_bitwise_reproducible(precision):
+    """No floating-point atomics: two runs of the same spread and interp give
+    identical bits, also after re-binning (a fresh binning gives the same order)."""
+    n, npts, kernel = 32, 5000, "IB_6"
+    dt = torch.float64 if precision == ia.FP64 else torch.float32
+    level = ia.Level.uniform_periodic(n, 3)
+    plan = ia.LevelPlan(level, kernel, ia.SIDE, precision, max_points=npts)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    X = torch.rand((npts, 3), generator=g, device="cuda", dtype=torch.float64)
+    X[: npts // 2] = 0.5 + 0.05 * torch.rand((npts // 2, 3), generator=g, device="cuda", dtype=torch.float64)
+    F = torch.randn((npts, 3), generator=g, device="cuda", dtype=torch.float64).to(dt)
+    shapes = [ia.array_shape(level, 0, plan.data_gcw, ia.SIDE, axis=a) for a in range(3)]
+    u = [torch.rand(s, generator=g, device="cuda", dtype=torch.float64).to(dt) for s in shapes]
+    plan.bind(1, u)
+    outs = []
+    for rep in range(2):
+        plan.bin(X)
+        f = [torch.zeros(s, dtype=dt, device="cuda") for s in shapes]
+        plan.bind(0, f)
+        plan.spread(0, F, X, mode=ia.OVERWRITE)
+        U = torch.zeros((npts, 3), dtype=dt, device="cuda")
+        plan.interp(1, U, X)
+        torch.cuda.synchronize()
+        outs.append((f, U))
+    for a in range(3):
+        assert torch.equal(outs[0][0][a], outs[1][0][a]), a
+    assert torch.equal(outs[0][1], outs[1][1])
