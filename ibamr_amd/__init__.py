@@ -43,7 +43,7 @@ C_API = [
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
     "ibamr_plan_info",
-    "ibamr_last_error",
+    "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
     "ibamr_s0_interp", "ibamr_s0_spread",
 ] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear")
      for op in ("interp", "spread") for d in ("2d", "3d")]
@@ -96,6 +96,11 @@ def lib():
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(PlanConfig)]
     L.ibamr_plan_destroy.argtypes = [vp]
     L.ibamr_plan_set_timing.argtypes = [vp, ctypes.c_int]
+    ip_, dp_ = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    L.ibamr_forces_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ip_, ip_, dp_, dp_,
+                                      ctypes.c_int, ip_, ip_, ip_, dp_, dp_, ctypes.c_int, ip_, dp_, dp_, dp_]
+    L.ibamr_forces_compute.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, vp]
+    L.ibamr_forces_destroy.argtypes = [vp]
     L.ibamr_forward_euler_step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_long, vp]
     L.ibamr_plan_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]

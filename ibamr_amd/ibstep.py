@@ -1,6 +1,6 @@
 """Several Lagrangian structures on one level, each with its own delta kernel,
-stepped spread -> velocity -> ghost fill -> interp -> position update, eagerly
-or as one captured HIP graph.
+stepped force -> spread -> velocity -> ghost fill -> interp -> position update,
+eagerly or as one captured HIP graph.
 
 Reference behaviour mirrored:
   * IBStrategySet runs each strategy's spreadForce in turn into the same f, so
@@ -9,14 +9,19 @@ Reference behaviour mirrored:
     structures sharing a kernel is one level plan; the first plan overwrites f,
     the others accumulate.
   * IBMethod::forwardEulerStep: X_new = X + dt U (src/IB/IBMethod.cpp:713-739).
+  * With a force generator (ibamr_amd.structure.Forces), F is set to the force
+    at X and the previous U at the start of every step, as
+    IBMethod::computeLagrangianForce zeroes F and calls the force generator
+    before the spread (src/IB/IBMethod.cpp:834-857).
 
 The fluid solve between spread and interp is not part of this library; the
 step uses a stand-in u = u_bg + alpha * f on the whole ghosted arrays, followed
 by the periodic ghost fill, so the interpolated velocity depends on the spread.
 
-Graph capture (torch.cuda.CUDAGraph) records only kernels: point binning
-(ibamr_plan_bin_async), spread, the stand-in combination (a torch elementwise
-kernel), the periodic ghost fill, interp and the Euler update.  No memset and
+Graph capture (torch.cuda.CUDAGraph) records only kernels: the force
+generation, point binning (ibamr_plan_bin_async), spread, the stand-in
+combination (a torch elementwise kernel), the periodic ghost fill, interp and
+the Euler update.  No memset and
 no host synchronization happens inside a step.
 """
 import torch
@@ -25,10 +30,12 @@ import ibamr_amd as ia
 
 
 class StructureSet:
-    """structures: list of (kernel, npoints) in the order their points sit in X."""
+    """structures: list of (kernel, npoints) in the order their points sit in X.
+    forces: None (F is given) or a structure.Forces over all the points."""
 
-    def __init__(self, level, structures, precision=ia.FP64, alpha=0.0, u_bg=None, device="cuda"):
+    def __init__(self, level, structures, precision=ia.FP64, alpha=0.0, u_bg=None, device="cuda", forces=None):
         self.level = level
+        self.forces = forces
         self.precision = precision
         self.dtype = torch.float64 if precision == ia.FP64 else torch.float32
         self.alpha = float(alpha)
@@ -63,6 +70,8 @@ class StructureSet:
         """One step on device tensors X [n][ndim] float64, F/U [n][ndim] in the
         plan precision; X is advanced in place."""
         s = stream if stream is not None else torch.cuda.current_stream()
+        if self.forces is not None:
+            self.forces.compute(F, X, U, accumulate=False, stream=s)
         for p, a, b in self._slices():
             p.bin(X[a:b], stream=s, check=False)
         for i, (p, a, b) in enumerate(self._slices()):

@@ -183,6 +183,32 @@ int ibamr_plan_fill_ghosts(ibamr_plan* plan, int slot, void* stream);
 int ibamr_forward_euler_step(double* X_new, const double* X, const void* U, int u_precision, double dt, long n,
                              void* stream);
 
+/* Lagrangian force generation, IBStandardForceGen::computeLagrangianForce
+ * (src/IB/IBStandardForceGen.cpp:254-305): Hookean springs (default_spring_force,
+ * IBSpringForceFunctions.h:104: tension kappa (R - rest)), beams (three-point
+ * bending with rigidity and reference curvature, :1038-1147) and target points
+ * (kappa (X0 - X) - eta U, :1202-1299).  Element arrays are host arrays, node
+ * indices 0-based into the n points; curvature and X0 are [element][ndim].
+ * Per node the forces are summed in the reference's order (springs, beams,
+ * targets; element order), so FP64 results are bitwise those of the reference
+ * loops. */
+typedef struct ibamr_forces ibamr_forces;
+int ibamr_forces_create(ibamr_forces** forces, int ndim, int npoints, int nsprings, const int* spring_master,
+                        const int* spring_slave, const double* spring_kappa, const double* spring_rest, int nbeams,
+                        const int* beam_master, const int* beam_next, const int* beam_prev,
+                        const double* beam_rigidity, const double* beam_curvature, int ntargets,
+                        const int* target_node, const double* target_kappa, const double* target_eta,
+                        const double* target_X0);
+/* F (device, [n][ndim], double for IBAMR_FP64 or float for IBAMR_FP32C) += (mode
+ * IBAMR_SPREAD_ACCUMULATE) or = (IBAMR_SPREAD_OVERWRITE) the force at positions X
+ * (device, double); U (device, [n][ndim], double for u_precision IBAMR_FP64 or
+ * float for IBAMR_FP32C, read as double; may be NULL = at rest) feeds the target
+ * point damping.  Forces are summed in double either way.  Kernels only
+ * (graph-safe). */
+int ibamr_forces_compute(ibamr_forces* forces, void* F, int f_precision, const double* X, const void* U,
+                         int u_precision, int mode, void* stream);
+int ibamr_forces_destroy(ibamr_forces* forces);
+
 /* Kernel timing for measurement.  With timing on, HIP events are recorded on
  * the launch stream right around each spread tile kernel and each
  * interpolation kernel (the point permutation before them is not included).

@@ -21,6 +21,7 @@
  */
 #include "le_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -652,4 +653,54 @@ double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const do
         acc += wprod * V[p];
     }
     return acc / vol;
+}
+
+/* ------------------------------------------------ Lagrangian force generation */
+
+/* IBStandardForceGen::computeLagrangianForce (src/IB/IBStandardForceGen.cpp:254-305):
+ * F_ghost = 0; springs (:814-930, default_spring_force kappa (R - rest),
+ * IBSpringForceFunctions.h:104), beams (:1038-1147), target points (:1202-1299)
+ * scattered into F_ghost in element order; then F += F_ghost. */
+void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
+                          const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,
+                          const int* bp, const double* bK, const double* bc, int nt, const int* tn, const double* tK,
+                          const double* tE, const double* tX0)
+{
+    double* G = (double*)calloc((size_t)(n > 0 ? n : 1) * (size_t)ndim, sizeof(double));
+    for (int k = 0; k < ns; ++k)
+    {
+        const int m = ndim * sm[k], s = ndim * ss[k];
+        double D[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < ndim; ++d) D[d] = X[s + d] - X[m + d];
+        double R2 = D[0] * D[0] + D[1] * D[1];
+        if (ndim == 3) R2 = R2 + D[2] * D[2];
+        const double R = sqrt(R2);
+        if (R < DBL_EPSILON) continue;
+        const double T_over_R = (sk[k] * (R - sr[k])) / R;
+        for (int d = 0; d < ndim; ++d)
+        {
+            const double f = T_over_R * D[d];
+            G[m + d] += f;
+            G[s + d] -= f;
+        }
+    }
+    for (int k = 0; k < nb; ++k)
+    {
+        const int m = ndim * bm[k], nx = ndim * bn[k], pv = ndim * bp[k];
+        for (int d = 0; d < ndim; ++d)
+        {
+            const double f = bK[k] * (((X[nx + d] + X[pv + d]) - 2.0 * X[m + d]) - bc[ndim * k + d]);
+            G[m + d] += 2.0 * f;
+            G[nx + d] -= f;
+            G[pv + d] -= f;
+        }
+    }
+    for (int k = 0; k < nt; ++k)
+    {
+        const int i = ndim * tn[k];
+        for (int d = 0; d < ndim; ++d)
+            G[i + d] += tK[k] * (tX0[ndim * k + d] - X[i + d]) - tE[k] * (U ? U[i + d] : 0.0);
+    }
+    for (int i = 0; i < n * ndim; ++i) F[i] = F[i] + G[i];
+    free(G);
 }

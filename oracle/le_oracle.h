@@ -130,6 +130,13 @@ void orc_order_entries_z(int kernel, const int* cell, const int* indices, const 
 double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const double* dx,
                                 const double* period, const double* X, const double* V, int npoints);
 
+/* IBStandardForceGen::computeLagrangianForce: F += spring, beam and target
+ * point forces in the reference's scatter order (src/IB/IBStandardForceGen.cpp:254-305). */
+void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
+                          const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,
+                          const int* bp, const double* bK, const double* bc, int nt, const int* tn, const double* tK,
+                          const double* tE, const double* tX0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -288,3 +288,34 @@ def spread_patch(kernel, centering_side, dx, x_lower, ilower, iupper, gcw, array
             idx, sh = idx[perm], sh[perm]
         spread(kernel, dx, x_lower, ilower, iupper, gcw, arrays, X, F, indices=idx, Xshift=sh, depth=depth)
     return arrays
+
+
+def lagrangian_force(ndim, X, F=None, U=None, springs=None, beams=None, targets=None):
+    """IBStandardForceGen forces, reference scatter order (orc_lagrangian_force).
+    springs = (master, slave, kappa, rest); beams = (prev, master, next,
+    rigidity, curvature); targets = (node, kappa, eta, X0).  Returns F + forces."""
+    X = _dv(X).reshape(-1, ndim)
+    n = X.shape[0]
+    out = np.zeros((n, ndim)) if F is None else _dv(F).reshape(n, ndim).copy()
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    if springs is not None:
+        sm, ss, sk, sr = i32(springs[0]), i32(springs[1]), _dv(springs[2]), _dv(springs[3])
+    else:
+        sm = ss = np.zeros(1, np.int32)
+        sk = sr = np.zeros(1)
+    if beams is not None:
+        bp, bm, bn, bK, bc = i32(beams[0]), i32(beams[1]), i32(beams[2]), _dv(beams[3]), _dv(beams[4])
+    else:
+        bp = bm = bn = np.zeros(1, np.int32)
+        bK = bc = np.zeros(max(1, ndim))
+    if targets is not None:
+        tn, tK, tE, tX0 = i32(targets[0]), _dv(targets[1]), _dv(targets[2]), _dv(targets[3])
+    else:
+        tn = np.zeros(1, np.int32)
+        tK = tE = tX0 = np.zeros(max(1, ndim))
+    Ud = None if U is None else _dv(U).reshape(n, ndim)
+    lib().orc_lagrangian_force(ndim, n, _d(out), _d(X), _d(Ud) if Ud is not None else None,
+                               len(springs[0]) if springs is not None else 0, _i(sm), _i(ss), _d(sk), _d(sr),
+                               len(beams[0]) if beams is not None else 0, _i(bm), _i(bn), _i(bp), _d(bK), _d(bc),
+                               len(targets[0]) if targets is not None else 0, _i(tn), _d(tK), _d(tE), _d(tX0))
+    return out

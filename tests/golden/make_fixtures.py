@@ -5,6 +5,9 @@ mounted at /root/reference).
   interpolate_01_{2d,3d}.{ib_4,ib_6,bspline_3}.json
       from tests/interpolate/interpolate_01_<dim>.<kernel>.output: the printed
       point coordinates and interpolated values (cell-centred, depth NDIM).
+  curve2d_64.vertex, curve2d_64.spring
+      copied from tests/IB/ (IBStandardInitializer input files: a closed curve
+      of 304 nodes joined by 304 springs).
   ghost_accumulation_01_<case>.patches.json (8 cases, 2-D/3-D, cell/side,
       one or several patches, periodic or not): every patch of the finest
       level after the ghost accumulation, with the input settings it used.
@@ -17,6 +20,7 @@ Usage: python tests/golden/make_fixtures.py [/root/reference]
 import json
 import os
 import re
+import shutil
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -120,6 +124,9 @@ def main():
             fx = interp_fixture(ref, dim, kernel)
             with open(os.path.join(HERE, "interpolate_01_%s.%s.json" % (dim, kernel)), "w") as f:
                 json.dump(fx, f)
+    # structure input files of the reference's IB tests (data, copied as they are)
+    for fn in ("curve2d_64.vertex", "curve2d_64.spring"):
+        shutil.copyfile(os.path.join(ref, "tests", "IB", fn), os.path.join(HERE, fn))
     for case in CASES:
         fx = patches_fixture(ref, case)
         with open(os.path.join(HERE, "ghost_accumulation_01_%s.patches.json" % case), "w") as f:

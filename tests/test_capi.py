@@ -135,3 +135,24 @@ def test_patch_plan_rejects_bad_box(L):
     with pytest.raises(ia.IBAMRError) as e:
         ia.LevelPlan(lv, "IB_4", ia.SIDE, max_points=10, box=([5] * 3, [4] * 3))
     assert e.value.code == ia.IBAMR_ERR_INVALID
+
+
+def test_forces_create_rejects_bad_elements(L):
+    """Element validation happens on the host, before any device call."""
+    import numpy as np
+
+    import ibamr_amd as ia
+    from ibamr_amd import structure
+
+    with pytest.raises(ia.IBAMRError, match="spring 1: bad node index"):
+        structure.Forces(2, 3, springs=([0, 2], [1, 3], [1.0, 1.0], [0.0, 0.0]))
+    with pytest.raises(ia.IBAMRError, match="spring 0: bad node index"):
+        structure.Forces(2, 3, springs=([1], [1], [1.0], [0.0]))
+    with pytest.raises(ia.IBAMRError, match="beam 0: bad node index"):
+        structure.Forces(3, 3, beams=([0], [0], [1], [1.0], np.zeros((1, 3))))
+    with pytest.raises(ia.IBAMRError, match="target point 0: bad node index"):
+        structure.Forces(3, 3, targets=([-1], [1.0], [0.0], np.zeros((1, 3))))
+    h = ctypes.c_void_p()
+    rc = ia.lib().ibamr_forces_create(ctypes.byref(h), 4, 3, 0, None, None, None, None, 0, None, None, None, None,
+                                      None, 0, None, None, None, None)
+    assert rc != 0 and "ndim" in ia.lib().ibamr_last_error().decode()

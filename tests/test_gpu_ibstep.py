@@ -104,3 +104,63 @@ def test_fp32c_trajectory_tracks_fp64():
     torch.cuda.synchronize()
     drift = float((X64 - X32).abs().max()) * N
     assert drift < 1e-4, drift
+
+
+def _ring_forces(npoints, seed=12):
+    """Springs joining consecutive points of each structure into a closed loop,
+    beams along them and target points at every 5th point."""
+    from ibamr_amd import structure
+    rng = np.random.default_rng(seed)
+    m, s, pv, nx = [], [], [], []
+    start = 0
+    for _, n in STRUCT:
+        i = np.arange(n)
+        m.append(start + i)
+        s.append(start + (i + 1) % n)
+        pv.append(start + (i - 1) % n)
+        nx.append(start + (i + 1) % n)
+        start += n
+    m, s, pv, nx = map(np.concatenate, (m, s, pv, nx))
+    springs = (m, s, rng.uniform(0.5, 2.0, len(m)), np.full(len(m), 1e-3))
+    beams = (pv, m, nx, rng.uniform(1e-3, 1e-2, len(m)), np.zeros((len(m), 3)))
+    tn = np.arange(0, npoints, 5)
+    return springs, beams, tn, rng, structure
+
+
+def test_step_with_force_generation_graph_and_oracle(orc):
+    """F generated on the GPU inside the step: the first eager step's F equals
+    the oracle's forces bitwise, and a captured graph of the whole step
+    (force -> spread -> ... -> update) replays the eager trajectory bitwise."""
+    S, X, F, U, _ = _setup(ia.FP64)
+    springs, beams, tn, rng, structure = _ring_forces(X.shape[0])
+    Xn0 = X.cpu().numpy()
+    targets = (tn, rng.uniform(1.0, 5.0, len(tn)), rng.uniform(0.0, 1.0, len(tn)), Xn0[tn])
+    S.forces = structure.Forces(3, X.shape[0], springs, beams, targets)
+    U.copy_(torch.from_numpy(rng.standard_normal(U.shape) * 1e-2))
+    U0 = U.cpu().numpy()
+    X0 = X.clone()
+    dt = 0.25 / N / 0.1
+    print("ibstep with forces: eager", flush=True)
+    S.step(X, F, U, dt)
+    torch.cuda.synchronize()
+    want = orc.lagrangian_force(3, Xn0, U=U0, springs=springs, beams=beams, targets=targets)
+    assert np.array_equal(F.cpu().numpy(), want)
+    traj = [X.clone()]
+    for _ in range(3):
+        S.step(X, F, U, dt)
+        traj.append(X.clone())
+    torch.cuda.synchronize()
+    X.copy_(X0)
+    U.copy_(torch.from_numpy(U0))
+    torch.cuda.synchronize()
+    print("ibstep with forces: capture", flush=True)
+    g = S.capture(X, F, U, dt)
+    torch.cuda.synchronize()
+    X.copy_(X0)
+    U.copy_(torch.from_numpy(U0))
+    torch.cuda.synchronize()
+    for k in range(4):
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(X, traj[k]), k
+    S.forces.close()
