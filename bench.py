@@ -283,6 +283,42 @@ def main_amr(args, torch, ia):
     print(json.dumps(out), flush=True)
 
 
+def time_forces(torch, counts, X0, reps=20):
+    """ibamr_forces_compute on the shells' points: each shell a closed ring of
+    springs (consecutive points) with beams along it, every 4th point a target
+    point.  Returns the mean time of one force evaluation (FP32 output)."""
+    import numpy as np
+    from ibamr_amd import structure
+
+    rng = np.random.default_rng(58)
+    m, s, pv = [], [], []
+    start = 0
+    for c in counts:
+        i = np.arange(c)
+        m.append(start + i)
+        s.append(start + (i + 1) % c)
+        pv.append(start + (i - 1) % c)
+        start += c
+    m, s, pv = map(np.concatenate, (m, s, pv))
+    Xh = X0.cpu().numpy()
+    tn = np.arange(0, start, 4)
+    f = structure.Forces(3, start, springs=(m, s, rng.uniform(1, 2, start), np.full(start, 1e-4)),
+                         beams=(pv, m, s, rng.uniform(1e-3, 1e-2, start), np.zeros((start, 3))),
+                         targets=(tn, rng.uniform(1, 2, len(tn)), rng.uniform(0, 1, len(tn)), Xh[tn]))
+    F = torch.zeros((start, 3), dtype=torch.float32, device="cuda")
+    U = torch.zeros((start, 3), dtype=torch.float32, device="cuda")
+    f.compute(F, X0, U)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        f.compute(F, X0, U)
+    e1.record()
+    torch.cuda.synchronize()
+    f.close()
+    return e0.elapsed_time(e1) / reps
+
+
 def main_multi(args, torch, ia):
     """64 ellipsoidal shells on a periodic 384^3 grid, shells 0-31 IB_4 and 32-63
     BSPLINE_3, point counts log-uniform in [2000, 200000]; 100 steps of
@@ -336,6 +372,7 @@ def main_multi(args, torch, ia):
         del S, graph
     drift = float((res[ia.FP32C]["X"] - res[ia.FP64]["X"]).abs().max()) / h
     ms = res[ia.FP32C]["ms"]
+    forces_ms = time_forces(torch, counts, X0)
     out = {
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "points/s", "n_gpus": 1, "steps": steps, "warmup": 1,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
@@ -345,7 +382,10 @@ def main_multi(args, torch, ia):
                                "one HIP-graph replay per step, %d steps, FP32 compensated" % (n, steps),
                    "points": n, "grid": N, "structures": nshell},
         "detail": {"fp64_ms_per_step": res[ia.FP64]["ms"], "fp32c_over_fp64": res[ia.FP64]["ms"] / ms,
-                   "dt": res[ia.FP32C]["dt"], "fp32c_vs_fp64_position_drift_cells_after_%d_steps" % steps: drift},
+                   "dt": res[ia.FP32C]["dt"], "fp32c_vs_fp64_position_drift_cells_after_%d_steps" % steps: drift,
+                   "force_generation_ms": forces_ms,
+                   "force_generation": "ring springs + beams per shell, a target point every 4th point, "
+                                       "timed alone (not in the step above)"},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
