@@ -186,9 +186,11 @@ struct ibamr_plan
     int* keys_sorted_d = nullptr;
     int* idx_d = nullptr;              // point indices in bin order before the per-bin sort; afterwards
                                        // the inverse permutation (slot of every point)
-    int* cursor_d = nullptr;           // [nmem+1] scatter cursors
+    int* keys2_d = nullptr;            // radix sort ping-pong buffer
+    int* rs_hist_d = nullptr;          // [RS_DIGITS * blocks] (digit, block) counts
+    int* rs_off_d = nullptr;           // [RS_DIGITS * blocks + 1] their exclusive scan
     int* scan_tmp_d = nullptr;         // block sums of the scan
-    int* scan_part_d = nullptr;        // [nmem+1] block-local exclusive prefixes
+    int* scan_part_d = nullptr;        // block-local exclusive prefixes of the counts
     int* perm_d = nullptr;
     int4* cell_d = nullptr;
     int4* cell_sorted_d = nullptr;
@@ -221,7 +223,6 @@ struct BinArgs
     int bin;
     int nmem;
     int* keys;
-    int* count;
     int4* cell;
 };
 
@@ -257,7 +258,6 @@ __global__ void k_bin_keys(BinArgs a)
         key = bin_mem(NDIM, b[0], b[1], b[2], a.nb, a.nsb);
     }
     a.keys[p] = key;
-    atomicAdd(&a.count[key], 1);  // integer counts: exact and order independent
     a.cell[p] = make_int4(c[0], c[1], c[2], 0);
 }
 
@@ -355,111 +355,114 @@ __global__ void k_scan_add(const int* __restrict__ part, int m, const int* __res
     if (i == m - 1) start[m] = v + in[i];
 }
 
-// Scatter point indices into their bins (slot order inside a bin is arbitrary here).
-__global__ void k_bin_scatter(const int* __restrict__ keys, int n, int* __restrict__ cursor, int* __restrict__ idx)
+// Stable LSD radix sort of the bin keys, carrying the point indices: after it
+// the points are grouped by bin and, inside a bin, in index order, so every
+// later kernel sees one deterministic order.  Device-wide atomics are avoided
+// (on a multi-XCD GPU they are resolved past the per-XCD L2): per pass, a
+// block histogram in LDS, one exclusive scan of the (digit, block) counts, and
+// a scatter that ranks equal digits inside a block with wave ballots.
+constexpr int RS_BITS = 8;
+constexpr int RS_DIGITS = 1 << RS_BITS;
+constexpr int RS_THREADS = 256;            // one digit per thread in the bookkeeping
+constexpr int RS_ROUNDS = 16;              // rounds of RS_THREADS elements per block
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
+static_assert(RS_THREADS == RS_DIGITS, "one thread per digit");
+
+// hist[d * nblocks + block] = number of keys of this block's tile with digit d
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const int* __restrict__ keys, int n, int shift,
+                                                        int nblocks, int* __restrict__ hist)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    idx[atomicAdd(&cursor[keys[p]], 1)] = p;
+    __shared__ int s_cnt[RS_DIGITS];
+    const int t = threadIdx.x;
+    s_cnt[t] = 0;
+    __syncthreads();
+    const int i0 = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r)
+    {
+        const int i = i0 + r * RS_THREADS + t;
+        if (i < n) atomicAdd(&s_cnt[(keys[i] >> shift) & (RS_DIGITS - 1)], 1);  // LDS: exact, order free
+    }
+    __syncthreads();
+    hist[t * nblocks + blockIdx.x] = s_cnt[t];
 }
 
-// Sorts every bin's point indices ascending (unique values, so the result is
-// unique and deterministic) and writes the bin slot of every sorted position.
-// Runs of up to BS_CHUNK are sorted in LDS (bitonic); longer bins are then
-// merged pairwise in global memory (ping-pong between idx and perm).
-constexpr int BS_THREADS = 256;
-constexpr int BS_CHUNK = 4096;
-
-__global__ __launch_bounds__(BS_THREADS) void k_bin_sort(const int* __restrict__ start, int nbins,
-                                                         int* __restrict__ idx, int* __restrict__ perm,
-                                                         int* __restrict__ keys_sorted)
+// off = exclusive scan of hist (digit-major), so block b's keys of digit d go
+// to off[d * nblocks + b] onward, in tile order.  vals_in == nullptr: the
+// value of element i is i (first pass).
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const int* __restrict__ keys_in,
+                                                           const int* __restrict__ vals_in, int n, int shift,
+                                                           int nblocks, const int* __restrict__ off,
+                                                           int* __restrict__ keys_out, int* __restrict__ vals_out)
 {
-    __shared__ int s_v[BS_CHUNK];
-    const int b = blockIdx.x;
-    if (b >= nbins) return;
-    const int s0 = start[b], s1 = start[b + 1];
-    const int n = s1 - s0;
-    if (n == 0) return;
-    const int t = threadIdx.x;
-    for (int i = t; i < n; i += BS_THREADS) keys_sorted[s0 + i] = b;
-    if (n == 1)
+    constexpr int NW = RS_THREADS / 64;
+    __shared__ int s_run[RS_DIGITS];       // next output slot per digit
+    __shared__ int s_wc[NW][RS_DIGITS];    // per-wave count of a digit in the round, tagged with the round
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    s_run[t] = off[t * nblocks + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s_wc[w][t] = 0;
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int i0 = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r)
     {
-        if (t == 0) perm[s0] = idx[s0];
-        return;
-    }
-    // 1. sort chunks of BS_CHUNK in LDS; result stays in idx
-    for (int c0 = 0; c0 < n; c0 += BS_CHUNK)
-    {
-        const int cn = min(BS_CHUNK, n - c0);
-        int P = 1;
-        while (P < cn) P <<= 1;
-        for (int i = t; i < P; i += BS_THREADS) s_v[i] = i < cn ? idx[s0 + c0 + i] : 0x7fffffff;
-        __syncthreads();
-        for (int k = 2; k <= P; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1)
-            {
-                for (int i = t; i < P; i += BS_THREADS)
-                {
-                    const int l = i ^ j;
-                    if (l > i)
-                    {
-                        const int a = s_v[i], c = s_v[l];
-                        const bool up = (i & k) == 0;
-                        if ((a > c) == up)
-                        {
-                            s_v[i] = c;
-                            s_v[l] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        for (int i = t; i < cn; i += BS_THREADS) idx[s0 + c0 + i] = s_v[i];
-        __syncthreads();
-    }
-    // 2. merge sorted runs: element of run A at a goes to a + |{y in B : y < x}|, and symmetrically
-    int* src = idx + s0;
-    int* dst = perm + s0;
-    for (int wdt = BS_CHUNK; wdt < n; wdt <<= 1)
-    {
-        for (int i = t; i < n; i += BS_THREADS)
+        const int i = i0 + r * RS_THREADS + t;
+        const bool valid = i < n;
+        const int key = valid ? keys_in[i] : 0;
+        const int val = valid ? (vals_in ? vals_in[i] : i) : 0;
+        const int dig = (key >> shift) & (RS_DIGITS - 1);
+        // lanes of this wave holding the same digit
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RS_BITS; ++b)
         {
-            const int pair = i / (2 * wdt);
-            const int a0 = pair * 2 * wdt;
-            const int b0 = min(a0 + wdt, n), b1 = min(a0 + 2 * wdt, n);
-            const int x = src[i];
-            int lo, hi, off;
-            if (i < b0)
+            const unsigned long long m = __ballot((dig >> b) & 1);
+            peers &= ((dig >> b) & 1) ? m : ~m;
+        }
+        const int tag = (r + 1) << 16;
+        if (valid && (peers & lt) == 0) s_wc[wv][dig] = tag | __popcll(peers);
+        __syncthreads();
+        if (valid)
+        {
+            int pos = s_run[dig] + __popcll(peers & lt);
+            for (int w = 0; w < wv; ++w)
             {
-                lo = b0;
-                hi = b1;
-                off = i - a0;
+                const int c = s_wc[w][dig];
+                pos += (c >> 16) == r + 1 ? (c & 0xffff) : 0;
             }
-            else
-            {
-                lo = a0;
-                hi = b0;
-                off = i - b0;
-            }
-            int cnt = 0, L = lo, H = hi;  // elements < x in the other run
-            while (L < H)
-            {
-                const int mid = (L + H) >> 1;
-                if (src[mid] < x)
-                    L = mid + 1;
-                else
-                    H = mid;
-            }
-            cnt = L - lo;
-            dst[a0 + off + cnt] = x;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
         }
         __syncthreads();
-        int* tmp = src;
-        src = dst;
-        dst = tmp;
+        int tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+        {
+            const int c = s_wc[w][t];
+            tot += (c >> 16) == r + 1 ? (c & 0xffff) : 0;
+        }
+        s_run[t] += tot;
+        __syncthreads();
     }
-    if (src != perm + s0)
-        for (int i = t; i < n; i += BS_THREADS) perm[s0 + i] = src[i];
+}
+
+// start[b] = first sorted position whose key is >= b, for b in [0, m] (keys in
+// [0, m)): one binary search per bin, so the work per thread stays bounded
+// however the points are spread over the bins.
+__global__ void k_bin_bounds(const int* __restrict__ keys_sorted, int n, int m, int* __restrict__ start)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > m) return;
+    int lo = 0, hi = n;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (keys_sorted[mid] < b)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    start[b] = lo;
 }
 
 // cells in bin order, and the inverse permutation (slot of every point)
@@ -1182,7 +1185,9 @@ int ibamr_plan_destroy(ibamr_plan* P)
     dfree(P->Xs_d);
     dfree(P->Fs_d);
     dfree(P->bin_start_d);
-    dfree(P->cursor_d);
+    dfree(P->keys2_d);
+    dfree(P->rs_hist_d);
+    dfree(P->rs_off_d);
     dfree(P->scan_tmp_d);
     dfree(P->scan_part_d);
     delete P;
@@ -1426,9 +1431,14 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     PLAN_HIP(hipMalloc(&P->Xs_d, sizeof(double) * ndim * np));
     PLAN_HIP(hipMalloc(&P->Fs_d, (C->precision == IBAMR_FP64 ? 8 : 4) * (size_t)P->fdepth * np));
     PLAN_HIP(hipMalloc(&P->bin_start_d, sizeof(int) * (P->nmem + 2)));
-    PLAN_HIP(hipMalloc(&P->cursor_d, sizeof(int) * (P->nmem + 1)));
-    PLAN_HIP(hipMalloc(&P->scan_tmp_d, sizeof(int) * ((P->nmem + 1 + SCAN_B - 1) / SCAN_B + 1)));
-    PLAN_HIP(hipMalloc(&P->scan_part_d, sizeof(int) * (P->nmem + 1)));
+    {
+        const size_t nrs = (size_t)RS_DIGITS * ((np + RS_TILE - 1) / RS_TILE);
+        PLAN_HIP(hipMalloc(&P->keys2_d, sizeof(int) * np));
+        PLAN_HIP(hipMalloc(&P->rs_hist_d, sizeof(int) * nrs));
+        PLAN_HIP(hipMalloc(&P->rs_off_d, sizeof(int) * (nrs + 1)));
+        PLAN_HIP(hipMalloc(&P->scan_tmp_d, sizeof(int) * ((nrs + SCAN_B - 1) / SCAN_B + 1)));
+        PLAN_HIP(hipMalloc(&P->scan_part_d, sizeof(int) * nrs));
+    }
 #undef PLAN_HIP
     if (ndim == 3)
     {
@@ -1541,31 +1551,45 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     a.bin = P->bin;
     a.nmem = P->nmem;
     a.keys = P->keys_d;
-    a.count = P->cursor_d;
     a.cell = P->cell_d;
     const int G = (npoints + B - 1) / B;
-    // counting sort by bin slot: count, exclusive scan, scatter, then sort each
-    // bin's point indices so the order inside a bin is the index order
-    hipLaunchKernelGGL(k_fill_int, dim3((m + B - 1) / B), dim3(B), 0, s, P->cursor_d, m, 0);
     if (P->ndim == 3)
         hipLaunchKernelGGL(k_bin_keys<3>, dim3(G), dim3(B), 0, s, a);
     else
         hipLaunchKernelGGL(k_bin_keys<2>, dim3(G), dim3(B), 0, s, a);
     HIP_OK(hipGetLastError());
-    const int nsb = (m + SCAN_B - 1) / SCAN_B;
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nsb), dim3(SCAN_B), 0, s, P->cursor_d, m, P->scan_part_d,
-                       P->scan_tmp_d);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, s, P->scan_tmp_d, nsb);
-    hipLaunchKernelGGL(k_scan_add, dim3((m + B - 1) / B), dim3(B), 0, s, P->scan_part_d, m, P->scan_tmp_d,
-                       P->cursor_d, P->bin_start_d);
+    // stable radix sort of keys [0, nmem] over the bits they use; the last pass
+    // lands in (keys_sorted_d, perm_d)
+    int nbits = 1;
+    while ((P->nmem >> nbits) != 0) ++nbits;
+    const int passes = (nbits + RS_BITS - 1) / RS_BITS;
+    const int nblk = (npoints + RS_TILE - 1) / RS_TILE;
+    const int M = RS_DIGITS * nblk;
+    const int nsb = (M + SCAN_B - 1) / SCAN_B;
+    const int* kin = P->keys_d;
+    const int* vin = nullptr;
+    for (int p = 0; p < passes; ++p)
+    {
+        const bool last_parity = ((passes - 1 - p) & 1) == 0;
+        int* kout = last_parity ? P->keys_sorted_d : P->keys2_d;
+        int* vout = last_parity ? P->perm_d : P->idx_d;
+        const int shift = p * RS_BITS;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, s, kin, npoints, shift, nblk, P->rs_hist_d);
+        hipLaunchKernelGGL(k_scan_blocks, dim3(nsb), dim3(SCAN_B), 0, s, P->rs_hist_d, M, P->scan_part_d,
+                           P->scan_tmp_d);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, s, P->scan_tmp_d, nsb);
+        hipLaunchKernelGGL(k_scan_add, dim3((M + B - 1) / B), dim3(B), 0, s, P->scan_part_d, M, P->scan_tmp_d,
+                           P->rs_hist_d, P->rs_off_d);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, s, kin, vin, npoints, shift, nblk,
+                           P->rs_off_d, kout, vout);
+        HIP_OK(hipGetLastError());
+        kin = kout;
+        vin = vout;
+    }
+    hipLaunchKernelGGL(k_bin_bounds, dim3((m + 1 + B - 1) / B), dim3(B), 0, s, P->keys_sorted_d, npoints, m,
+                       P->bin_start_d);
     HIP_OK(hipGetLastError());
-    // cursors start at the bin offsets
-    hipLaunchKernelGGL(k_copy_int, dim3((m + B - 1) / B), dim3(B), 0, s, P->bin_start_d, m, P->cursor_d);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(G), dim3(B), 0, s, P->keys_d, npoints, P->cursor_d, P->idx_d);
-    hipLaunchKernelGGL(k_bin_sort, dim3(m), dim3(BS_THREADS), 0, s, P->bin_start_d, m, P->idx_d, P->perm_d,
-                       P->keys_sorted_d);
-    HIP_OK(hipGetLastError());
-    // idx_d is free once the per-bin sort is done: it now holds the inverse permutation
+    // idx_d is free once the sort is done: it now holds the inverse permutation
     hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d,
                        P->idx_d);
     HIP_OK(hipGetLastError());
