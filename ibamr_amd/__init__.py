@@ -42,7 +42,7 @@ C_API = [
     "ibamr_plan_create", "ibamr_plan_create_patch", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
-    "ibamr_plan_info",
+    "ibamr_plan_info", "ibamr_plan_redistribute", "ibamr_permute_rows",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
     "ibamr_s0_interp", "ibamr_s0_spread",
 ] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear")
@@ -112,6 +112,8 @@ def lib():
     L.ibamr_plan_interp.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     L.ibamr_plan_fill_periodic_ghosts.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_plan_fill_ghosts.argtypes = [vp, ctypes.c_int, vp]
+    L.ibamr_plan_redistribute.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+    L.ibamr_permute_rows.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     L.ibamr_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
@@ -299,6 +301,23 @@ class LevelPlan:
     def fill_periodic_ghosts(self, slot, stream=None):
         _check(lib().ibamr_plan_fill_periodic_ghosts(self._h, slot, _stream_ptr(stream)))
 
+    def redistribute(self, X, lag_index=None, error_if_points_leave=False, periodic_offsets=False, stream=None):
+        """LDataManager::begin/endDataRedistribution for the points X (device
+        float64 [n][ndim], moved into the domain in place).  Returns
+        (new_order [n] int32, patch_offsets [npatches+1] int32, offsets
+        [n][ndim] int32 or None), all device tensors; new_order[k] is the old
+        position of the point that becomes k.  Bin again after permuting."""
+        import torch
+        n = X.shape[0] if X.dim() > 0 else 0
+        order = torch.empty(max(n, 1), dtype=torch.int32, device=X.device)
+        poff = torch.empty(len(self.level.patches) + 1, dtype=torch.int32, device=X.device)
+        offs = torch.empty((max(n, 1), self.level.ndim), dtype=torch.int32, device=X.device) if periodic_offsets \
+            else None
+        lag = None if lag_index is None else lag_index.to(torch.int32).contiguous()
+        _check(lib().ibamr_plan_redistribute(self._h, _ptr(X), _ptr(lag), n, 1 if error_if_points_leave else 0,
+                                             _ptr(order), _ptr(poff), _ptr(offs), _stream_ptr(stream)))
+        return order[:n], poff, (offs[:n] if offs is not None else None)
+
     def fill_ghosts(self, slot, stream=None):
         """Same-level ghost fill between the level's patches (periodic wrap included)."""
         _check(lib().ibamr_plan_fill_ghosts(self._h, slot, _stream_ptr(stream)))
@@ -314,6 +333,15 @@ class LevelPlan:
         _check(lib().ibamr_plan_kernel_times(self._h, ctypes.byref(sm), ctypes.byref(ns), ctypes.byref(im),
                                              ctypes.byref(ni)))
         return {"spread_ms": sm.value, "nspread": ns.value, "interp_ms": im.value, "ninterp": ni.value}
+
+
+def permute_rows(dst, src, order, stream=None):
+    """dst[k] = src[order[k]] for rows of device tensors (4- or 8-byte elements),
+    the VecScatter of LData to the redistributed order."""
+    n = order.shape[0]
+    depth = (src.numel() // n) if n else 1
+    _check(lib().ibamr_permute_rows(_ptr(dst), _ptr(src), _ptr(order), n, depth, src.element_size(),
+                                    _stream_ptr(stream)))
 
 
 def forward_euler_step(X_new, X, U, dt, stream=None):

@@ -177,6 +177,40 @@ int ibamr_plan_fill_periodic_ghosts(ibamr_plan* plan, int slot, void* stream);
  * only (graph-safe). */
 int ibamr_plan_fill_ghosts(ibamr_plan* plan, int slot, void* stream);
 
+/* Redistribution of the Lagrangian points over the level's patches, as
+ * LDataManager::beginDataRedistribution / endDataRedistribution do at a regrid
+ * (src: ibtk/src/lagrangian/LDataManager.cpp:1349-1521 and :1523-1960,
+ * computeNodeDistribution :2858-2990), for the points held on this GPU:
+ *  - X (device, [n][ndim], in/out) is moved into the domain: in a periodic
+ *    direction X += L while X < x_lower and X -= L while X >= x_upper
+ *    (:1403-1407); in other directions the point is an error when
+ *    error_if_points_leave is set (:1409-1415) and otherwise clamped to
+ *    [x_lower, x_upper - L sqrt(eps)] (:1417-1420);
+ *  - periodic_offset (device, [n][ndim], may be NULL) receives
+ *    round((X_old - X_new) / dx) per coordinate (:1424-1428);
+ *  - every point's cell (IndexUtilities::getCellIndex) must lie in a patch of
+ *    the level;
+ *  - new_order (device, [n]) receives the new local order: new_order[k] is the
+ *    old position of the point that becomes k.  Points are grouped by patch in
+ *    level order, so each patch's interior points are contiguous
+ *    (:1709-1712), patch p holding [patch_offsets[p], patch_offsets[p+1])
+ *    (patch_offsets: device, [npatches+1]); inside a patch by cell in box order
+ *    (x fastest); inside a cell by Lagrangian index (lag_index, device [n], may
+ *    be NULL = the point's position), as the LNodeSet sort does (:1498-1503).
+ * Then move every LData array to the new order with ibamr_permute_rows and bin
+ * the permuted positions again (the plan's binning is void after this call).
+ * Synchronizes the stream (the error checks read device counters).  Errors:
+ * IBAMR_ERR_OUT_OF_DOMAIN (escaped, non-finite or farther than 64 periodic
+ * lengths, or in no patch), IBAMR_ERR_INVALID (negative lag_index, plan made by
+ * ibamr_plan_create_patch, or 2^31 cells or more). */
+int ibamr_plan_redistribute(ibamr_plan* plan, double* X, const int* lag_index, int npoints, int error_if_points_leave,
+                            int* new_order, int* patch_offsets, int* periodic_offset, void* stream);
+
+/* The VecScatter of the LData arrays to the new order (LDataManager.cpp:1769-1840, 1900-1915):
+ * row k of dst = row order[k] of src, rows of depth elements of elem_bytes
+ * (4 or 8) bytes; dst and src distinct device arrays.  Kernel only (graph-safe). */
+int ibamr_permute_rows(void* dst, const void* src, const int* order, int n, int depth, int elem_bytes, void* stream);
+
 /* IBMethod::forwardEulerStep (IBMethod.cpp:713-739): X_new = X + dt U over
  * n values (n = ndim * points), on the stream; X_new may be X.  U is double
  * (u_precision IBAMR_FP64) or float (IBAMR_FP32C).  Kernel only: graph-safe. */

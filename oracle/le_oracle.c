@@ -704,3 +704,125 @@ void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const dou
     for (int i = 0; i < n * ndim; ++i) F[i] = F[i] + G[i];
     free(G);
 }
+
+/* ------------------------------------------------------------- redistribution */
+
+/* LDataManager::beginDataRedistribution (LDataManager.cpp:1393-1431, 1460-1503)
+ * and computeNodeDistribution (:2893-2908) for one level held in one place:
+ * positions moved into the domain (periodic wrap by whole lengths, or clamp to
+ * [x_lower, x_upper - L sqrt(eps)], or an error when error_if_leave), the
+ * periodic offsets round((X_old - X_new)/dx), each point's patch (first patch
+ * whose box holds getCellIndex), and the new order: by patch, cell in box order
+ * (x fastest), Lagrangian index, then old position.  A point more than
+ * max_wraps lengths out (or non-finite) counts as lost.
+ * Returns 0, 1 (escaped), 2 (in no patch), 3 (lost). */
+typedef struct
+{
+    long long key;
+    int lag;
+    int i;
+} orc_rkey;
+
+static int cmp_rkey(const void* a, const void* b)
+{
+    const orc_rkey* x = (const orc_rkey*)a;
+    const orc_rkey* y = (const orc_rkey*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    if (x->lag != y->lag) return x->lag < y->lag ? -1 : 1;
+    return (x->i > y->i) - (x->i < y->i);
+}
+
+int orc_redistribute(int ndim, int n, double* X, const int* lag, const double* x_lower, const double* x_upper,
+                     const double* dx, const int* dom_lo, const int* dom_hi, const int* periodic, int error_if_leave,
+                     int max_wraps, int npatches, const int* plo, const int* phi, int* new_order,
+                     int* patch_offsets, int* offset)
+{
+    const double tol = sqrt(DBL_EPSILON); /* LDataManager.cpp:150 */
+    long long* pcell = (long long*)malloc(sizeof(long long) * (size_t)(npatches + 1));
+    orc_rkey* keys = (orc_rkey*)malloc(sizeof(orc_rkey) * (size_t)(n > 0 ? n : 1));
+    int rc = 0;
+    pcell[0] = 0;
+    for (int q = 0; q < npatches; ++q)
+    {
+        long long c = 1;
+        for (int d = 0; d < ndim; ++d) c *= (long long)(phi[ndim * q + d] - plo[ndim * q + d] + 1);
+        pcell[q + 1] = pcell[q] + c;
+    }
+    for (int p = 0; p < n && rc == 0; ++p)
+    {
+        double* x = &X[(long)ndim * p];
+        int cell[3] = {0, 0, 0};
+        for (int d = 0; d < ndim; ++d)
+        {
+            const double Xr = x[d];
+            double v = Xr;
+            if (!isfinite(Xr))
+            {
+                rc = 3;
+                break;
+            }
+            if (periodic[d])
+            {
+                const double L = x_upper[d] - x_lower[d];
+                int w = 0;
+                while (v < x_lower[d] && w < max_wraps)
+                {
+                    v += L;
+                    ++w;
+                }
+                while (v >= x_upper[d] && w < max_wraps)
+                {
+                    v -= L;
+                    ++w;
+                }
+                if (!(v >= x_lower[d] && v < x_upper[d])) rc = 3;
+            }
+            else if (error_if_leave)
+            {
+                if (v < x_lower[d] || v > x_upper[d]) rc = 1;
+            }
+            else
+            {
+                v = v > x_lower[d] ? v : x_lower[d];
+                const double top = x_upper[d] - (x_upper[d] - x_lower[d]) * tol;
+                v = v < top ? v : top;
+            }
+            x[d] = v;
+            if (offset) offset[(long)ndim * p + d] = (int)round((Xr - v) / dx[d]);
+        }
+        if (rc) break;
+        orc_get_cell_index(ndim, x, x_lower, x_upper, dx, dom_lo, dom_hi, cell);
+        int owner = -1;
+        for (int q = 0; q < npatches && owner < 0; ++q)
+        {
+            int in = 1;
+            for (int d = 0; d < ndim; ++d) in = in && cell[d] >= plo[ndim * q + d] && cell[d] <= phi[ndim * q + d];
+            if (in) owner = q;
+        }
+        if (owner < 0)
+        {
+            rc = 2;
+            break;
+        }
+        long long lin = 0;
+        for (int d = ndim - 1; d >= 0; --d)
+            lin = lin * (phi[ndim * owner + d] - plo[ndim * owner + d] + 1) + (cell[d] - plo[ndim * owner + d]);
+        keys[p].key = pcell[owner] + lin;
+        keys[p].lag = lag ? lag[p] : p;
+        keys[p].i = p;
+    }
+    if (rc == 0)
+    {
+        qsort(keys, (size_t)n, sizeof(orc_rkey), cmp_rkey);
+        for (int k = 0; k < n; ++k) new_order[k] = keys[k].i;
+        for (int q = 0; q <= npatches; ++q)
+        {
+            int c = 0;
+            while (c < n && keys[c].key < pcell[q]) ++c;
+            patch_offsets[q] = c;
+        }
+    }
+    free(pcell);
+    free(keys);
+    return rc;
+}

@@ -132,6 +132,14 @@ double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const do
 
 /* IBStandardForceGen::computeLagrangianForce: F += spring, beam and target
  * point forces in the reference's scatter order (src/IB/IBStandardForceGen.cpp:254-305). */
+/* LDataManager::beginDataRedistribution / computeNodeDistribution for one
+ * level: wrap/clamp, periodic offsets, owner patch, new order by (patch, cell
+ * in box order, Lagrangian index).  0 ok, 1 escaped, 2 in no patch, 3 lost. */
+int orc_redistribute(int ndim, int n, double* X, const int* lag, const double* x_lower, const double* x_upper,
+                     const double* dx, const int* dom_lo, const int* dom_hi, const int* periodic, int error_if_leave,
+                     int max_wraps, int npatches, const int* plo, const int* phi, int* new_order,
+                     int* patch_offsets, int* offset);
+
 void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
                           const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,
                           const int* bp, const double* bK, const double* bc, int nt, const int* tn, const double* tK,

@@ -319,3 +319,31 @@ def lagrangian_force(ndim, X, F=None, U=None, springs=None, beams=None, targets=
                                len(beams[0]) if beams is not None else 0, _i(bm), _i(bn), _i(bp), _d(bK), _d(bc),
                                len(targets[0]) if targets is not None else 0, _i(tn), _d(tK), _d(tE), _d(tX0))
     return out
+
+
+def redistribute(X, x_lower, x_upper, dx, dom_lo, dom_hi, periodic, boxes, lag=None, error_if_leave=False,
+                 max_wraps=64):
+    """LDataManager redistribution of one level (orc_redistribute).  boxes:
+    list of (lower, upper) patch boxes in level order.  Returns (status, X
+    moved into the domain, new_order, patch_offsets, periodic offsets)."""
+    ndim = len(dom_lo)
+    X = _dv(X).reshape(-1, ndim).copy()
+    n = X.shape[0]
+    plo = np.ascontiguousarray([b[0] for b in boxes], dtype=np.int32).reshape(-1)
+    phi = np.ascontiguousarray([b[1] for b in boxes], dtype=np.int32).reshape(-1)
+    order = np.zeros(max(n, 1), dtype=np.int32)
+    poff = np.zeros(len(boxes) + 1, dtype=np.int32)
+    offs = np.zeros((max(n, 1), ndim), dtype=np.int32)
+    lg = None if lag is None else np.ascontiguousarray(lag, dtype=np.int32)
+    L = lib()
+    if not getattr(L, "_redist_types", False):
+        L.orc_redistribute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_int)] + [ctypes.POINTER(ctypes.c_double)] * 3 + \
+            [ctypes.POINTER(ctypes.c_int)] * 3 + [ctypes.c_int, ctypes.c_int, ctypes.c_int] + \
+            [ctypes.POINTER(ctypes.c_int)] * 5
+        L._redist_types = True
+    rc = L.orc_redistribute(ndim, n, _d(X), _i(lg) if lg is not None else None, _d(_dv(x_lower)), _d(_dv(x_upper)),
+                            _d(_dv(dx)), _i(_iv(dom_lo)), _i(_iv(dom_hi)), _i(_iv([1 if p else 0 for p in periodic])),
+                            1 if error_if_leave else 0, max_wraps, len(boxes), _i(plo), _i(phi), _i(order), _i(poff),
+                            _i(offs))
+    return rc, X, order[:n], poff, offs[:n]
