@@ -122,6 +122,31 @@ def touched_faces(torch, X, N, kernel):
     return total
 
 
+def time_redistribute(torch, ia, X, F, N, kernel, reps=3):
+    """ibamr_plan_redistribute on the points (a regrid: wrap into the periodic
+    box, owner patch, new order) plus the permutation of X (float64) and F
+    (float32) into the new order; mean ms of reps runs (each ends in a sync)."""
+    level = ia.Level.uniform_periodic(N, 3)
+    n = X.shape[0]
+    plan = ia.LevelPlan(level, kernel, ia.SIDE, ia.FP32C, max_points=n)
+    Xw = X.clone()
+    F32 = F.to(torch.float32).contiguous()
+    Xp, Fp = torch.empty_like(Xw), torch.empty_like(F32)
+    lag = torch.arange(n, dtype=torch.int32, device="cuda").flip(0).contiguous()
+    plan.redistribute(Xw, lag)
+    torch.cuda.synchronize()
+    import time
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        order, _, _ = plan.redistribute(Xw, lag)
+        ia.permute_rows(Xp, Xw, order)
+        ia.permute_rows(Fp, F32, order)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    plan.close()
+    return ms
+
+
 def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
     dt = torch.float64 if precision == ia.FP64 else torch.float32
     level = ia.Level.uniform_periodic(N, 3)
@@ -454,6 +479,9 @@ def main():
                    "touched_faces": T, "bytes_step": B_spread + B_interp,
                    "step_GBs": (B_spread + B_interp) / (ms32 * 1e-3) / 1e9},
     }
+    out["detail"]["redistribute_ms"] = time_redistribute(torch, ia, X, F64, N, kernel)
+    out["detail"]["redistribute"] = ("LDataManager redistribution of all points (reversed Lagrangian indices) "
+                                     "+ permutation of X and F into the new order, not part of the step")
     if not args.no_fp64:
         print("bench: FP64 form", file=sys.stderr, flush=True)
         r64 = run_form(torch, ia, ia.FP64, X, F64, N, kernel, max(2, args.steps // 2), 1)
