@@ -42,7 +42,7 @@ C_API = [
     "ibamr_plan_create", "ibamr_plan_create_patch", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
-    "ibamr_plan_info", "ibamr_plan_redistribute", "ibamr_permute_rows",
+    "ibamr_plan_info", "ibamr_plan_redistribute", "ibamr_permute_rows", "ibamr_fe_quadrature_points",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
     "ibamr_s0_interp", "ibamr_s0_spread",
 ] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear")
@@ -114,6 +114,7 @@ def lib():
     L.ibamr_plan_fill_ghosts.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_plan_redistribute.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
     L.ibamr_permute_rows.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+    L.ibamr_fe_quadrature_points.argtypes = [ctypes.c_int] * 6 + [vp] * 7 + [ctypes.c_int, vp]
     L.ibamr_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
@@ -342,6 +343,37 @@ def permute_rows(dst, src, order, stream=None):
     depth = (src.numel() // n) if n else 1
     _check(lib().ibamr_permute_rows(_ptr(dst), _ptr(src), _ptr(order), n, depth, src.element_size(),
                                     _stream_ptr(stream)))
+
+
+def fe_quadrature_points(elem_nodes, phi, JxW, X_nodes, F_nodes=None, f_dtype=None, stream=None):
+    """Quadrature points and F*JxW of every element (FEDataManager::spread's
+    point source).  elem_nodes [nelem][nbasis] int32, phi [nbasis][nqp],
+    JxW [nelem][nqp] or None, X_nodes [nnodes][ndim], F_nodes [nnodes][nvars]
+    or None (device tensors, float64).  Returns (X_qp [nelem*nqp][ndim],
+    F_qp [nelem*nqp][nvars] or None)."""
+    import torch
+    nelem, nbasis = elem_nodes.shape
+    nqp = phi.shape[1]
+    ndim = X_nodes.shape[1]
+    if tuple(phi.shape) != (nbasis, nqp) or (JxW is not None and tuple(JxW.shape) != (nelem, nqp)) or \
+            (F_nodes is not None and F_nodes.shape[0] != X_nodes.shape[0]):
+        raise ValueError("fe_quadrature_points: phi [nbasis][nqp], JxW [nelem][nqp], F_nodes rows = X_nodes rows")
+    for t in (elem_nodes, phi, JxW, X_nodes, F_nodes):
+        if t is not None and not t.is_contiguous():
+            raise ValueError("fe_quadrature_points: contiguous tensors only")
+    dev = X_nodes.device
+    Xq = torch.empty((max(nelem * nqp, 1), ndim), dtype=torch.float64, device=dev)
+    Fq = None
+    nvars = 0
+    if F_nodes is not None:
+        nvars = F_nodes.shape[1]
+        Fq = torch.empty((max(nelem * nqp, 1), nvars), dtype=f_dtype or torch.float64, device=dev)
+    prec = FP32C if Fq is not None and Fq.dtype == torch.float32 else FP64
+    _check(lib().ibamr_fe_quadrature_points(ndim, nvars, nelem, nbasis, nqp, X_nodes.shape[0], _ptr(elem_nodes),
+                                            _ptr(phi), _ptr(JxW),
+                                            _ptr(X_nodes), _ptr(F_nodes), _ptr(Xq), _ptr(Fq), prec,
+                                            _stream_ptr(stream)))
+    return Xq[:nelem * nqp], (Fq[:nelem * nqp] if Fq is not None else None)
 
 
 def forward_euler_step(X_new, X, U, dt, stream=None):

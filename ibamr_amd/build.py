@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lib", "libibamr_amd.so")
-SOURCES = ["le_plan.hip", "le_fortran_abi.hip", "ib_forces.hip", "ibtk_facade.cpp"]
+SOURCES = ["le_plan.hip", "le_fortran_abi.hip", "ib_forces.hip", "fe_quadrature.hip", "ibtk_facade.cpp"]
 
 
 def hipcc():

@@ -206,6 +206,25 @@ int ibamr_plan_fill_ghosts(ibamr_plan* plan, int slot, void* stream);
 int ibamr_plan_redistribute(ibamr_plan* plan, double* X, const int* lag_index, int npoints, int error_if_points_leave,
                             int* new_order, int* patch_offsets, int* periodic_offset, void* stream);
 
+/* Finite-element point source (FEDataManager::spread, ibtk/src/lagrangian/
+ * FEDataManager.cpp:1169-1204 with sum_weighted_elem_solution :825-857): for
+ * every element e and quadrature point q,
+ *   X_qp[e*nqp+q][d]     = sum_k X_nodes[elem_nodes[e][k]][d] * phi[k][q]
+ *   F_JxW_qp[e*nqp+q][i] = (sum_k F_nodes[elem_nodes[e][k]][i] * phi[k][q]) * JxW[e][q]
+ * (sums in basis order from 0.0, as the reference loop).  phi ([nbasis][nqp]:
+ * the quadrature rule's shape function table) and JxW ([nelem][nqp]) come from
+ * the caller's FE library; JxW NULL means weights 1 (the positions-only pass
+ * with weights_are_unity); F_nodes NULL computes X_qp only.  All arrays are
+ * device arrays; F_qp is double (IBAMR_FP64) or float (IBAMR_FP32C, rounded
+ * once).  Spreading the nelem*nqp points with a level plan then gives what the
+ * per-patch spread after zeroExteriorValues (:2738-2783) plus the ghost
+ * accumulation gives.  nvars <= 8.  X_nodes / F_nodes hold nnodes rows; an
+ * element node index outside [0, nnodes) is never read and turns that
+ * element's outputs into NaN.  Kernel only (graph-safe). */
+int ibamr_fe_quadrature_points(int ndim, int nvars, int nelem, int nbasis, int nqp, int nnodes, const int* elem_nodes,
+                               const double* phi, const double* JxW, const double* X_nodes, const double* F_nodes,
+                               double* X_qp, void* F_qp, int f_precision, void* stream);
+
 /* The VecScatter of the LData arrays to the new order (LDataManager.cpp:1769-1840, 1900-1915):
  * row k of dst = row order[k] of src, rows of depth elements of elem_bytes
  * (4 or 8) bytes; dst and src distinct device arrays.  Kernel only (graph-safe). */

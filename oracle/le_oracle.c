@@ -826,3 +826,32 @@ int orc_redistribute(int ndim, int n, double* X, const int* lag, const double* x
     free(keys);
     return rc;
 }
+
+/* ---------------------------------------------------- FE quadrature point source */
+
+/* FEDataManager::spread's point source (FEDataManager.cpp:1169-1204,
+ * sum_weighted_elem_solution :825-857): X_qp = sum_k X_node phi[k][q],
+ * F_qp = (sum_k F_node phi[k][q]) * JxW[e][q], sums from 0.0 in basis order. */
+void orc_fe_quadrature_points(int ndim, int nvars, int nelem, int nbasis, int nqp, const int* elem_nodes,
+                              const double* phi, const double* JxW, const double* X_nodes, const double* F_nodes,
+                              double* X_qp, double* F_qp)
+{
+    for (int e = 0; e < nelem; ++e)
+        for (int q = 0; q < nqp; ++q)
+        {
+            const long t = (long)e * nqp + q;
+            for (int d = 0; d < ndim; ++d) X_qp[t * ndim + d] = 0.0;
+            if (F_nodes)
+                for (int i = 0; i < nvars; ++i) F_qp[t * nvars + i] = 0.0;
+            for (int k = 0; k < nbasis; ++k)
+            {
+                const long node = elem_nodes[(long)e * nbasis + k];
+                const double p = phi[(long)k * nqp + q];
+                for (int d = 0; d < ndim; ++d) X_qp[t * ndim + d] += X_nodes[node * ndim + d] * p;
+                if (F_nodes)
+                    for (int i = 0; i < nvars; ++i) F_qp[t * nvars + i] += F_nodes[node * nvars + i] * p;
+            }
+            if (F_nodes && JxW)
+                for (int i = 0; i < nvars; ++i) F_qp[t * nvars + i] *= JxW[(long)e * nqp + q];
+        }
+}

@@ -140,6 +140,11 @@ int orc_redistribute(int ndim, int n, double* X, const int* lag, const double* x
                      int max_wraps, int npatches, const int* plo, const int* phi, int* new_order,
                      int* patch_offsets, int* offset);
 
+/* FEDataManager::spread's quadrature point source (FEDataManager.cpp:1169-1204). */
+void orc_fe_quadrature_points(int ndim, int nvars, int nelem, int nbasis, int nqp, const int* elem_nodes,
+                              const double* phi, const double* JxW, const double* X_nodes, const double* F_nodes,
+                              double* X_qp, double* F_qp);
+
 void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
                           const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,
                           const int* bp, const double* bK, const double* bc, int nt, const int* tn, const double* tK,

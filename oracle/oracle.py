@@ -347,3 +347,26 @@ def redistribute(X, x_lower, x_upper, dx, dom_lo, dom_hi, periodic, boxes, lag=N
                             1 if error_if_leave else 0, max_wraps, len(boxes), _i(plo), _i(phi), _i(order), _i(poff),
                             _i(offs))
     return rc, X, order[:n], poff, offs[:n]
+
+
+def fe_quadrature_points(elem_nodes, phi, JxW, X_nodes, F_nodes=None):
+    """FEDataManager::spread's point source (orc_fe_quadrature_points)."""
+    en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+    nelem, nbasis = en.shape
+    phi = _dv(phi)
+    nqp = phi.shape[1]
+    Xn = _dv(X_nodes)
+    ndim = Xn.shape[1]
+    Xq = np.zeros((nelem * nqp, ndim))
+    Fn = None if F_nodes is None else _dv(F_nodes)
+    nvars = 0 if Fn is None else Fn.shape[1]
+    Fq = np.zeros((nelem * nqp, max(nvars, 1)))
+    L = lib()
+    if not getattr(L, "_feq_types", False):
+        L.orc_fe_quadrature_points.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_int)] + \
+            [ctypes.POINTER(ctypes.c_double)] * 6
+        L._feq_types = True
+    Jw = None if JxW is None else _dv(JxW)
+    L.orc_fe_quadrature_points(ndim, nvars, nelem, nbasis, nqp, _i(en), _d(phi), _d(Jw) if Jw is not None else None,
+                               _d(Xn), _d(Fn) if Fn is not None else None, _d(Xq), _d(Fq))
+    return Xq, (Fq[:, :nvars] if Fn is not None else None)

@@ -1,0 +1,75 @@
+"""GPU FE point source (ibamr_fe_quadrature_points) against the oracle's
+restatement of FEDataManager::spread's quadrature loop (bitwise in FP64), and
+the FE spread end to end: quadrature points through a level plan equal the
+oracle's spread of the same points, and the spread force sums to the integral
+of the force density."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+import ibamr_amd as ia  # noqa: E402
+from ibamr_amd import fe  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(ndim, n, seed):
+    lo, hi = [0.3, 0.35, 0.25][:ndim], [0.62, 0.6, 0.7][:ndim]
+    X, E = fe.box_mesh(ndim, n, lo, hi)
+    rng = np.random.default_rng(seed)
+    X = X + rng.uniform(-0.2, 0.2, X.shape) * (np.array(hi) - np.array(lo)) / np.array(n)  # distorted elements
+    return X, E
+
+
+@pytest.mark.parametrize("ndim,n,g", [(2, (40, 30), 3), (3, (20, 16, 24), 2)])
+def test_quadrature_points_bitwise(orc, ndim, n, g):
+    print("fe qp", ndim, n, g, flush=True)
+    X, E = _mesh(ndim, n, 11)
+    xi, w = fe.gauss_rule(ndim, g)
+    phi = fe.q1_shape(ndim, xi)
+    J = fe.q1_jxw(X, E, xi, w)
+    F = np.random.default_rng(12).standard_normal((len(X), ndim))
+    Xq, Fq = orc.fe_quadrature_points(E, phi, J, X, F)
+    c = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+    gX, gF = ia.fe_quadrature_points(c(E, torch.int32), c(phi), c(J), c(X), c(F))
+    assert np.array_equal(gX.cpu().numpy(), Xq) and np.array_equal(gF.cpu().numpy(), Fq)
+    gX2, gF2 = ia.fe_quadrature_points(c(E, torch.int32), c(phi), c(J), c(X), c(F), f_dtype=torch.float32)
+    assert np.array_equal(gX2.cpu().numpy(), Xq) and np.array_equal(gF2.cpu().numpy(), Fq.astype(np.float32))
+    gX3, none = ia.fe_quadrature_points(c(E, torch.int32), c(phi), None, c(X))
+    assert none is None and np.array_equal(gX3.cpu().numpy(), Xq)
+
+
+def test_fe_spread_end_to_end(orc):
+    N = 48
+    h = 1.0 / N
+    X, E = _mesh(3, (12, 10, 14), 13)
+    xi, w = fe.gauss_rule(3, 2)
+    phi = fe.q1_shape(3, xi)
+    J = fe.q1_jxw(X, E, xi, w)
+    F = np.random.default_rng(14).standard_normal((len(X), 3))
+    c = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+    Xq, Fq = ia.fe_quadrature_points(c(E, torch.int32), c(phi), c(J), c(X), c(F))
+    level = ia.Level.uniform_periodic(N, 3)
+    gw = ia.get_minimum_ghost_width("IB_4")
+    plan = ia.LevelPlan(level, "IB_4", ia.SIDE, ia.FP64, max_points=Xq.shape[0])
+    f = [torch.zeros(ia.array_shape(level, 0, [gw] * 3, ia.SIDE, axis=a), dtype=torch.float64, device="cuda")
+         for a in range(3)]
+    plan.bind(0, f)
+    plan.bin(Xq)
+    plan.spread(0, Fq, Xq, mode=ia.OVERWRITE)
+    torch.cuda.synchronize()
+    Xn, Fn = Xq.cpu().numpy(), Fq.cpu().numpy()
+    il, iu = [0] * 3, [N - 1] * 3
+    ref = [np.zeros(orc.side_axis_shape(il, iu, [gw] * 3, a)) for a in range(3)]
+    cells = orc.get_cell_index(Xn, [0.0] * 3, [1.0] * 3, [h] * 3, il, iu)
+    idx, sh = orc.build_patch_entries(cells, il, iu, gw, [N] * 3, il, [h] * 3, [8] * 3)
+    orc.spread_side("IB_4", [h] * 3, [0.0] * 3, il, iu, [gw] * 3, ref, Xn, Fn, indices=idx, Xshift=sh)
+    total = Fn.sum(axis=0)
+    for a in range(3):
+        got = f[a].cpu().numpy()
+        sl = tuple(slice(gw, gw + N + (1 if (2 - ax) == a else 0)) for ax in range(3))
+        assert np.max(np.abs(got[sl] - ref[a][sl])) <= 1e-13 * np.max(np.abs(ref[a][sl]))
+        # the periodic side array holds face N == face 0: count the faces 0..N-1
+        inner = tuple(slice(gw, gw + N) for _ in range(3))
+        assert abs(got[inner].sum() * h ** 3 - total[a]) <= 1e-12 * np.abs(Fn[:, a]).sum()
+    plan.close()
