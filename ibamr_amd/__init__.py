@@ -43,6 +43,7 @@ C_API = [
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
     "ibamr_plan_info", "ibamr_plan_redistribute", "ibamr_permute_rows", "ibamr_fe_quadrature_points",
+    "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
     "ibamr_s0_interp", "ibamr_s0_spread",
 ] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear")
@@ -115,6 +116,10 @@ def lib():
     L.ibamr_plan_redistribute.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
     L.ibamr_permute_rows.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     L.ibamr_fe_quadrature_points.argtypes = [ctypes.c_int] * 6 + [vp] * 7 + [ctypes.c_int, vp]
+    L.ibamr_fe_mesh_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int)]
+    L.ibamr_fe_integrate_rhs.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp]
+    L.ibamr_fe_mesh_destroy.argtypes = [vp]
     L.ibamr_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
@@ -374,6 +379,48 @@ def fe_quadrature_points(elem_nodes, phi, JxW, X_nodes, F_nodes=None, f_dtype=No
                                             _ptr(X_nodes), _ptr(F_nodes), _ptr(Xq), _ptr(Fq), prec,
                                             _stream_ptr(stream)))
     return Xq[:nelem * nqp], (Fq[:nelem * nqp] if Fq is not None else None)
+
+
+class FEMesh:
+    """Element connectivity on the GPU for FEDataManager::interpWeighted's
+    right-hand side (ibamr_fe_mesh_*).  elem_nodes: host int array [nelem][nbasis]."""
+
+    def __init__(self, elem_nodes, nnodes):
+        import numpy as np
+        en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+        self.nelem, self.nbasis = en.shape
+        self.nnodes = int(nnodes)
+        h = ctypes.c_void_p()
+        _check(lib().ibamr_fe_mesh_create(ctypes.byref(h), self.nelem, self.nbasis, self.nnodes,
+                                          en.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        self._h = h
+
+    def integrate_rhs(self, F_nodes, U_qp, phi, JxW, accumulate=False, stream=None):
+        """F_nodes [nnodes][nvars] float64 (device) =/+= sum over elements of
+        phi * JxW * U_qp (U_qp [nelem*nqp][nvars], float64 or float32)."""
+        import torch
+        nqp = phi.shape[1]
+        nvars = F_nodes.shape[1]
+        if tuple(phi.shape) != (self.nbasis, nqp) or tuple(JxW.shape) != (self.nelem, nqp) or \
+                tuple(U_qp.shape) != (self.nelem * nqp, nvars) or F_nodes.shape[0] != self.nnodes or \
+                F_nodes.dtype != torch.float64:
+            raise ValueError("integrate_rhs: shapes phi [nbasis][nqp], JxW [nelem][nqp], U_qp [nelem*nqp][nvars], "
+                             "F_nodes float64 [nnodes][nvars]")
+        prec = FP32C if U_qp.dtype == torch.float32 else FP64
+        _check(lib().ibamr_fe_integrate_rhs(self._h, nvars, nqp, _ptr(phi), _ptr(JxW), _ptr(U_qp), prec,
+                                            _ptr(F_nodes), ACCUMULATE if accumulate else OVERWRITE,
+                                            _stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ibamr_fe_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def forward_euler_step(X_new, X, U, dt, stream=None):

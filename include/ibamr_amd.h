@@ -225,6 +225,23 @@ int ibamr_fe_quadrature_points(int ndim, int nvars, int nelem, int nbasis, int n
                                const double* phi, const double* JxW, const double* X_nodes, const double* F_nodes,
                                double* X_qp, void* F_qp, int f_precision, void* stream);
 
+/* FE interpolation, the weighted part (FEDataManager::interpWeighted,
+ * FEDataManager.cpp:2009-2061 with integrate_elem_rhs :1569-1606): after the
+ * velocity has been interpolated to the quadrature points (a level plan's
+ * interp of the points from ibamr_fe_quadrature_points), every element e forms
+ *   rhs[k][i] = sum_q U_qp[e*nqp+q][i] * (phi[k][q] * JxW[e][q])
+ * and adds it to node elem_nodes[e][k]; F_nodes [nnodes][nvars] (device,
+ * double) = (IBAMR_SPREAD_OVERWRITE) or += (IBAMR_SPREAD_ACCUMULATE) the sum,
+ * elements added in element order (deterministic, no atomics).  The L2
+ * projection solve that follows in the reference (a mass-matrix solve) is the
+ * caller's.  The mesh object holds the node -> (element, basis) lists, built on
+ * the host from the host connectivity; node indices are checked there. */
+typedef struct ibamr_fe_mesh ibamr_fe_mesh;
+int ibamr_fe_mesh_create(ibamr_fe_mesh** mesh, int nelem, int nbasis, int nnodes, const int* elem_nodes_host);
+int ibamr_fe_integrate_rhs(const ibamr_fe_mesh* mesh, int nvars, int nqp, const double* phi, const double* JxW,
+                           const void* U_qp, int u_precision, double* F_nodes, int mode, void* stream);
+int ibamr_fe_mesh_destroy(ibamr_fe_mesh* mesh);
+
 /* The VecScatter of the LData arrays to the new order (LDataManager.cpp:1769-1840, 1900-1915):
  * row k of dst = row order[k] of src, rows of depth elements of elem_bytes
  * (4 or 8) bytes; dst and src distinct device arrays.  Kernel only (graph-safe). */

@@ -855,3 +855,29 @@ void orc_fe_quadrature_points(int ndim, int nvars, int nelem, int nbasis, int nq
                 for (int i = 0; i < nvars; ++i) F_qp[t * nvars + i] *= JxW[(long)e * nqp + q];
         }
 }
+
+/* FEDataManager::interpWeighted's right-hand side (FEDataManager.cpp:2009-2061,
+ * integrate_elem_rhs :1569-1606): per element, rhs[i][k] += U_qp[q][i] *
+ * (phi[k][q] * JxW[q]) over q, k, i in the reference loop order, then added to
+ * the nodes element by element.  F is accumulated into. */
+void orc_fe_integrate_rhs(int nvars, int nelem, int nbasis, int nqp, const int* elem_nodes, const double* phi,
+                          const double* JxW, const double* U_qp, double* F)
+{
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)(nvars * nbasis));
+    for (int e = 0; e < nelem; ++e)
+    {
+        for (int t = 0; t < nvars * nbasis; ++t) rhs[t] = 0.0;
+        for (int q = 0; q < nqp; ++q)
+        {
+            const long idx = (long)nvars * ((long)e * nqp + q);
+            for (int k = 0; k < nbasis; ++k)
+            {
+                const double p = phi[(long)k * nqp + q] * JxW[(long)e * nqp + q];
+                for (int i = 0; i < nvars; ++i) rhs[nbasis * i + k] += U_qp[idx + i] * p;
+            }
+        }
+        for (int i = 0; i < nvars; ++i)
+            for (int k = 0; k < nbasis; ++k) F[(long)elem_nodes[(long)e * nbasis + k] * nvars + i] += rhs[nbasis * i + k];
+    }
+    free(rhs);
+}

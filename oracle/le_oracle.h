@@ -145,6 +145,10 @@ void orc_fe_quadrature_points(int ndim, int nvars, int nelem, int nbasis, int nq
                               const double* phi, const double* JxW, const double* X_nodes, const double* F_nodes,
                               double* X_qp, double* F_qp);
 
+/* FEDataManager::interpWeighted's right-hand side (FEDataManager.cpp:2009-2061). */
+void orc_fe_integrate_rhs(int nvars, int nelem, int nbasis, int nqp, const int* elem_nodes, const double* phi,
+                          const double* JxW, const double* U_qp, double* F);
+
 void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
                           const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,
                           const int* bp, const double* bK, const double* bc, int nt, const int* tn, const double* tK,

@@ -370,3 +370,21 @@ def fe_quadrature_points(elem_nodes, phi, JxW, X_nodes, F_nodes=None):
     L.orc_fe_quadrature_points(ndim, nvars, nelem, nbasis, nqp, _i(en), _d(phi), _d(Jw) if Jw is not None else None,
                                _d(Xn), _d(Fn) if Fn is not None else None, _d(Xq), _d(Fq))
     return Xq, (Fq[:, :nvars] if Fn is not None else None)
+
+
+def fe_integrate_rhs(elem_nodes, phi, JxW, U_qp, nnodes, F=None):
+    """FEDataManager::interpWeighted's right-hand side (orc_fe_integrate_rhs); returns F + rhs."""
+    en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+    nelem, nbasis = en.shape
+    phi = _dv(phi)
+    nqp = phi.shape[1]
+    U = _dv(U_qp)
+    nvars = U.shape[1]
+    out = np.zeros((nnodes, nvars)) if F is None else _dv(F).copy()
+    L = lib()
+    if not getattr(L, "_ferhs_types", False):
+        L.orc_fe_integrate_rhs.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] + \
+            [ctypes.POINTER(ctypes.c_double)] * 4
+        L._ferhs_types = True
+    L.orc_fe_integrate_rhs(nvars, nelem, nbasis, nqp, _i(en), _d(phi), _d(_dv(JxW)), _d(U), _d(out))
+    return out

@@ -73,3 +73,68 @@ def test_fe_spread_end_to_end(orc):
         inner = tuple(slice(gw, gw + N) for _ in range(3))
         assert abs(got[inner].sum() * h ** 3 - total[a]) <= 1e-12 * np.abs(Fn[:, a]).sum()
     plan.close()
+
+
+@pytest.mark.parametrize("ndim,n", [(2, (40, 30)), (3, (20, 16, 24))])
+def test_interp_rhs_bitwise(orc, ndim, n):
+    print("fe rhs", ndim, n, flush=True)
+    X, E = _mesh(ndim, n, 21)
+    xi, w = fe.gauss_rule(ndim, 2)
+    phi = fe.q1_shape(ndim, xi)
+    J = fe.q1_jxw(X, E, xi, w)
+    rng = np.random.default_rng(22)
+    U = rng.standard_normal((len(E) * len(w), ndim))
+    F0 = rng.standard_normal((len(X), ndim))
+    c = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+    mesh = ia.FEMesh(E, len(X))
+    Fd = torch.full((len(X), ndim), float("nan"), dtype=torch.float64, device="cuda")
+    mesh.integrate_rhs(Fd, c(U), c(phi), c(J))
+    assert np.array_equal(Fd.cpu().numpy(), orc.fe_integrate_rhs(E, phi, J, U, len(X)))
+    Fd = c(F0)
+    mesh.integrate_rhs(Fd, c(U, torch.float32), c(phi), c(J), accumulate=True)
+    want = orc.fe_integrate_rhs(E, phi, J, U.astype(np.float32).astype(np.float64), len(X), F=F0)
+    assert np.array_equal(Fd.cpu().numpy(), want)
+    mesh.close()
+
+
+def test_fe_mesh_rejects_bad_node():
+    with pytest.raises(ia.IBAMRError, match="element 1: bad node index"):
+        ia.FEMesh(np.array([[0, 1, 2, 3], [1, 2, 3, 9]]), 5)
+
+
+def test_fe_interp_end_to_end(orc):
+    """Velocity at the quadrature points by the level plan, then the weighted
+    right-hand side: equals the oracle's interpolation + integration."""
+    N = 48
+    h = 1.0 / N
+    X, E = _mesh(3, (10, 12, 9), 23)
+    xi, w = fe.gauss_rule(3, 2)
+    phi = fe.q1_shape(3, xi)
+    J = fe.q1_jxw(X, E, xi, w)
+    c = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+    Xq, _ = ia.fe_quadrature_points(c(E, torch.int32), c(phi), None, c(X))
+    level = ia.Level.uniform_periodic(N, 3)
+    gw = ia.get_minimum_ghost_width("BSPLINE_3")
+    plan = ia.LevelPlan(level, "BSPLINE_3", ia.SIDE, ia.FP64, max_points=Xq.shape[0])
+    g = torch.Generator(device="cuda")
+    g.manual_seed(24)
+    u = [torch.rand(ia.array_shape(level, 0, [gw] * 3, ia.SIDE, axis=a), generator=g, dtype=torch.float64,
+                    device="cuda") for a in range(3)]
+    plan.bind(1, u)
+    plan.fill_periodic_ghosts(1)
+    plan.bin(Xq)
+    Uq = torch.empty_like(Xq)
+    plan.interp(1, Uq, Xq)
+    mesh = ia.FEMesh(E, len(X))
+    Fd = torch.empty((len(X), 3), dtype=torch.float64, device="cuda")
+    mesh.integrate_rhs(Fd, Uq, c(phi), c(J))
+    torch.cuda.synchronize()
+    Xn = Xq.cpu().numpy()
+    un = [a.cpu().numpy() for a in u]
+    il, iu = [0] * 3, [N - 1] * 3
+    Uref = orc.interp_side("BSPLINE_3", [h] * 3, [0.0] * 3, il, iu, [gw] * 3, un, Xn)
+    assert np.max(np.abs(Uq.cpu().numpy() - Uref)) <= 1e-14
+    want = orc.fe_integrate_rhs(E, phi, J, Uref, len(X))
+    assert np.max(np.abs(Fd.cpu().numpy() - want)) <= 1e-13 * np.abs(want).max()
+    mesh.close()
+    plan.close()

@@ -43,3 +43,24 @@ def test_distorted_element_jacobian():
     s = rng.uniform(-1, 1, (200000, 3))
     detJ = np.linalg.det(np.einsum("jkq,kd->qdj", fe.q1_shape_grad(3, s), X[E[0]]))
     assert abs(J.sum() - 8.0 * detJ.mean()) < 5e-3
+
+
+@pytest.mark.parametrize("ndim,n", [(2, (5, 4)), (3, (3, 2, 4))])
+def test_interp_rhs_is_adjoint_of_spread_source(orc, ndim, n):
+    """<F, rhs(U)> = <F*JxW at the quadrature points, U>: the weighted
+    interpolation is the transpose of the spread's point source; and U = c
+    gives rhs summing to c * volume."""
+    X, E = fe.box_mesh(ndim, n, [0.1] * ndim, [0.9] * ndim)
+    X = X + np.random.default_rng(1).uniform(-0.02, 0.02, X.shape)
+    xi, w = fe.gauss_rule(ndim, 2)
+    phi = fe.q1_shape(ndim, xi)
+    J = fe.q1_jxw(X, E, xi, w)
+    rng = np.random.default_rng(2)
+    F = rng.standard_normal((len(X), 2))
+    U = rng.standard_normal((len(E) * len(w), 2))
+    _, Fq = orc.fe_quadrature_points(E, phi, J, X, F)
+    rhs = orc.fe_integrate_rhs(E, phi, J, U, len(X))
+    assert abs((F * rhs).sum() - (Fq * U).sum()) <= 1e-13 * np.abs(F).sum() * np.abs(U).max()
+    c = np.array([2.0, -0.5])
+    rhs1 = orc.fe_integrate_rhs(E, phi, J, np.tile(c, (len(E) * len(w), 1)), len(X))
+    assert np.allclose(rhs1.sum(axis=0), c * J.sum(), rtol=1e-13)
