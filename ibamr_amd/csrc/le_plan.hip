@@ -1160,6 +1160,7 @@ struct RedistArgs
     int* key_lag;
     int* key_cell;
     int* offset;   // [n][ndim] periodic offsets, or null
+    int* blockmax; // [gridDim.x] largest Lagrangian index of each block
     int* status;   // 0: escaped, 1: in no patch, 2: largest Lagrangian index, 3: not finite or beyond the
                    // wrap limit, 4: negative Lagrangian index
 };
@@ -1261,10 +1262,26 @@ __global__ void k_redist_prepare(RedistArgs a)
         a.key_cell[p] = cell;
         a.key_lag[p] = lag;
     }
-    // largest Lagrangian index: one atomic per wavefront
+    // largest Lagrangian index of the block (no device-wide atomic: one value per block)
+    __shared__ int s_max[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) lag = max(lag, __shfl_xor(lag, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&a.status[2], lag);
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = lag;
+    __syncthreads();
+    if (threadIdx.x == 0) a.blockmax[blockIdx.x] = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+}
+
+// status[2] = max of the per-block maxima (one block)
+__global__ __launch_bounds__(256) void k_max_reduce(const int* __restrict__ v, int n, int* __restrict__ out)
+{
+    __shared__ int s_w[4];
+    int m = 0;
+    for (int i = threadIdx.x; i < n; i += 256) m = max(m, v[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = max(max(s_w[0], s_w[1]), max(s_w[2], s_w[3]));
 }
 
 __global__ void k_gather_int(const int* __restrict__ src, const int* __restrict__ idx, int n, int* __restrict__ dst)
@@ -2134,13 +2151,16 @@ int ibamr_plan_redistribute(ibamr_plan* P, double* X, const int* lag_index, int 
     a.key_cell = reinterpret_cast<int*>(P->cell_d);  // cell_d holds 4 ints per point; the binning is void now
     a.offset = periodic_offset;
     a.status = P->status_d;
+    a.blockmax = P->rs_hist_d;  // free until the sorts below; holds >= max_points / 16 ints
     if (npoints > 0)
     {
+        static_assert(RS_TILE / RS_DIGITS <= 256, "block maxima must fit the histogram buffer");
         const int G = (npoints + B - 1) / B;
         if (P->ndim == 3)
             hipLaunchKernelGGL(k_redist_prepare<3>, dim3(G), dim3(B), 0, s, a);
         else
             hipLaunchKernelGGL(k_redist_prepare<2>, dim3(G), dim3(B), 0, s, a);
+        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, s, P->rs_hist_d, G, P->status_d + 2);
         HIP_OK(hipGetLastError());
     }
     int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
