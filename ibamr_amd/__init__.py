@@ -42,7 +42,8 @@ C_API = [
     "ibamr_plan_create", "ibamr_plan_create_patch", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
-    "ibamr_plan_info", "ibamr_plan_redistribute", "ibamr_permute_rows", "ibamr_fe_quadrature_points",
+    "ibamr_plan_info", "ibamr_plan_redistribute", "ibamr_permute_rows",
+    "ibamr_plan_set_physical_boundary", "ibamr_plan_accumulate_physical_boundary", "ibamr_plan_fill_physical_boundary", "ibamr_fe_quadrature_points",
     "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
     "ibamr_s0_interp", "ibamr_s0_spread",
@@ -67,6 +68,11 @@ class LevelGeom(ctypes.Structure):
                 ("domain_x_lower", ctypes.c_double * 3), ("domain_x_upper", ctypes.c_double * 3),
                 ("periodic_shift", ctypes.c_int * 3), ("npatches", ctypes.c_int),
                 ("patches", ctypes.POINTER(PatchGeom))]
+
+
+class BdryBox(ctypes.Structure):
+    _fields_ = [("patch", ctypes.c_int), ("codim", ctypes.c_int), ("location", ctypes.c_int),
+                ("lower", ctypes.c_int * 3), ("upper", ctypes.c_int * 3)]
 
 
 class PlanConfig(ctypes.Structure):
@@ -115,6 +121,10 @@ def lib():
     L.ibamr_plan_fill_ghosts.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_plan_redistribute.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
     L.ibamr_permute_rows.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+    L.ibamr_plan_set_physical_boundary.argtypes = [vp, ctypes.POINTER(BdryBox), ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_double)]
+    L.ibamr_plan_accumulate_physical_boundary.argtypes = [vp, ctypes.c_int, vp]
+    L.ibamr_plan_fill_physical_boundary.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_fe_quadrature_points.argtypes = [ctypes.c_int] * 6 + [vp] * 7 + [ctypes.c_int, vp]
     L.ibamr_fe_mesh_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int)]
@@ -324,6 +334,30 @@ class LevelPlan:
                                              _ptr(order), _ptr(poff), _ptr(offs), _stream_ptr(stream)))
         return order[:n], poff, (offs[:n] if offs is not None else None)
 
+    def set_physical_boundary(self, boxes, abg):
+        """boxes: [(patch, codim, location, lower, upper)] in each patch's order;
+        abg: [nboxes][arrays][3] Robin coefficients (a, b, g) of codim-1 boxes."""
+        import numpy as np
+        arr = (BdryBox * max(1, len(boxes)))()
+        for k, (p, c, l, lo, hi) in enumerate(boxes):
+            arr[k].patch, arr[k].codim, arr[k].location = p, c, l
+            for d in range(len(lo)):
+                arr[k].lower[d], arr[k].upper[d] = lo[d], hi[d]
+        ab = np.ascontiguousarray(abg, dtype=np.float64).reshape(-1)
+        if ab.size == 0:
+            ab = np.zeros(3)
+        self._bdry_keep = ab
+        _check(lib().ibamr_plan_set_physical_boundary(self._h, arr, len(boxes),
+                                                      ab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def accumulate_physical_boundary(self, slot, stream=None):
+        """accumulateFromPhysicalBoundaryData on every patch (after a spread)."""
+        _check(lib().ibamr_plan_accumulate_physical_boundary(self._h, slot, _stream_ptr(stream)))
+
+    def fill_physical_boundary(self, slot, stream=None):
+        """setPhysicalBoundaryConditions on every patch (before an interpolation)."""
+        _check(lib().ibamr_plan_fill_physical_boundary(self._h, slot, _stream_ptr(stream)))
+
     def fill_ghosts(self, slot, stream=None):
         """Same-level ghost fill between the level's patches (periodic wrap included)."""
         _check(lib().ibamr_plan_fill_ghosts(self._h, slot, _stream_ptr(stream)))
@@ -339,6 +373,57 @@ class LevelPlan:
         _check(lib().ibamr_plan_kernel_times(self._h, ctypes.byref(sm), ctypes.byref(ns), ctypes.byref(im),
                                              ctypes.byref(ni)))
         return {"spread_ms": sm.value, "nspread": ns.value, "interp_ms": im.value, "ninterp": ni.value}
+
+
+def physical_boundary_boxes(level, patch_index, gcw):
+    """Physical boundary boxes of one patch for the Robin boundary operators,
+    in SAMRAI's order (codim 1 faces, codim 2 edges, codim 3 corners, each by
+    location index), for patches whose physical boundary pieces all lie on
+    the level's non-periodic domain sides: a face of the patch on a
+    non-periodic domain side gives a codim-1 box (the cell layer outside the
+    face); an edge (corner) whose faces are all such faces gives a codim-2
+    (codim-3) box, returned as its fill box (gcw cells outward in every normal
+    direction, the patch extent along the edge).  Returns [(codim, location,
+    lower, upper)].  A caller with SAMRAI passes PatchGeometry's boundary boxes
+    instead (getCodimensionBoundaries, getBoundaryFillBox)."""
+    p = level.patches[patch_index]
+    nd = level.ndim
+    lo, hi = list(p.ilower), list(p.iupper)
+    phys = [[level.periodic_shift[d] == 0 and lo[d] == level.domain_lower[d],
+             level.periodic_shift[d] == 0 and hi[d] == level.domain_upper[d]] for d in range(nd)]
+    out = []
+    for loc in range(2 * nd):
+        d, up = loc // 2, loc % 2
+        if phys[d][up]:
+            blo, bhi = lo[:], hi[:]
+            blo[d] = bhi[d] = (hi[d] + 1) if up else (lo[d] - 1)
+            out.append((1, loc, blo, bhi))
+
+    def ghost(d, up):
+        return (hi[d] + 1, hi[d] + gcw) if up else (lo[d] - gcw, lo[d] - 1)
+    if nd == 3:
+        edges = [(0, 1, 2)] * 4 + [(1, 0, 2)] * 4 + [(2, 0, 1)] * 4
+        for loc in range(12):
+            e, n1, n2 = edges[loc]
+            if loc < 4:
+                u1, u2 = loc & 1, (loc >> 1) & 1
+            elif loc < 8:
+                u1, u2 = (loc - 4) >> 1, (loc - 4) & 1
+            else:
+                u1, u2 = (loc - 8) & 1, (loc - 8) >> 1
+            if phys[n1][u1] and phys[n2][u2]:
+                blo, bhi = lo[:], hi[:]
+                blo[n1], bhi[n1] = ghost(n1, u1)
+                blo[n2], bhi[n2] = ghost(n2, u2)
+                out.append((2, loc, blo, bhi))
+    for loc in range(1 << nd):
+        ups = [(loc >> d) & 1 for d in range(nd)]
+        if all(phys[d][ups[d]] for d in range(nd)):
+            blo, bhi = lo[:], hi[:]
+            for d in range(nd):
+                blo[d], bhi[d] = ghost(d, ups[d])
+            out.append((nd, loc, blo, bhi))
+    return out
 
 
 def permute_rows(dst, src, order, stream=None):

@@ -242,6 +242,39 @@ int ibamr_fe_integrate_rhs(const ibamr_fe_mesh* mesh, int nvars, int nqp, const 
                            const void* U_qp, int u_precision, double* F_nodes, int mode, void* stream);
 int ibamr_fe_mesh_destroy(ibamr_fe_mesh* mesh);
 
+/* Physical boundary operators after the spread / before the interpolation on
+ * non-periodic domains: CartCellRobinPhysBdryOp / CartSideRobinPhysBdryOp
+ * with LINEAR interpolation (ibtk/src/boundary/physical_boundary/
+ * CartCellRobinPhysBdryOp.cpp:415-474, CartSideRobinPhysBdryOp.cpp:553-620
+ * and :484-545, kernels cartphysbdryop{2,3}d.f.m4).  The boxes are what
+ * SAMRAI's PatchGeometry gives each patch: codim 1: the boundary box (the cell
+ * layer just outside the face; its tangential extent is the coefficient box);
+ * codim 2 and 3: getBoundaryFillBox(box, patch box, ghost width).  A patch's
+ * boxes are applied in the order given (SAMRAI's: codim 1, 2, 3 lists, each by
+ * location).  abg [nboxes][arrays][3] holds the Robin coefficients a, b, g of
+ * codim-1 boxes per array (cell data: depth component d, the reference's
+ * bc_coefs[d]; side data: axis, bc_coefs[axis]), uniform over the box.
+ * LDataManager::spread calls accumulateFromPhysicalBoundaryData after the
+ * ghost accumulation (LDataManager.cpp:647-660): ibamr_plan_accumulate_
+ * physical_boundary after ibamr_plan_spread.  The patch's out-of-domain ghost
+ * values the fold reads are, after a level spread, the contributions of all
+ * points that reach them; SAMRAI's accumulation leaves there only the patch's
+ * own points' contributions, so on levels with several patches the two agree
+ * where no other patch's points reach a patch's out-of-domain ghost cells.
+ * set_physical_boundary validates the boxes (inside the ghost box) and copies
+ * them to the device; the other two enqueue one kernel (graph-safe). */
+typedef struct
+{
+    int patch;      /* patch index in the level */
+    int codim;      /* 1 .. ndim */
+    int location;   /* SAMRAI location index of the codimension */
+    int lower[3];
+    int upper[3];
+} ibamr_bdry_box;
+int ibamr_plan_set_physical_boundary(ibamr_plan* plan, const ibamr_bdry_box* boxes, int nboxes, const double* abg);
+int ibamr_plan_accumulate_physical_boundary(ibamr_plan* plan, int slot, void* stream);
+int ibamr_plan_fill_physical_boundary(ibamr_plan* plan, int slot, void* stream);
+
 /* The VecScatter of the LData arrays to the new order (LDataManager.cpp:1769-1840, 1900-1915):
  * row k of dst = row order[k] of src, rows of depth elements of elem_bytes
  * (4 or 8) bytes; dst and src distinct device arrays.  Kernel only (graph-safe). */

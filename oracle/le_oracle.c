@@ -23,6 +23,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -880,4 +881,430 @@ void orc_fe_integrate_rhs(int nvars, int nelem, int nbasis, int nqp, const int* 
             for (int k = 0; k < nbasis; ++k) F[(long)elem_nodes[(long)e * nbasis + k] * nvars + i] += rhs[nbasis * i + k];
     }
     free(rhs);
+}
+
+/* ------------------------------------------------- physical boundary (Robin) ops */
+
+/* The LINEAR Robin ghost-cell operators of CartCellRobinPhysBdryOp and
+ * CartSideRobinPhysBdryOp (ibtk/src/boundary/physical_boundary/fortran/
+ * cartphysbdryop{2,3}d.f.m4: ccrobinphysbdryop1{x,y,z}, ccrobinphysbdryop2/3,
+ * scrobinphysbdryop1{x,y,z}, scrobinphysbdryop2/3), each with its adjoint
+ * (adjoint_op = 1: accumulateFromPhysicalBoundaryData) and forward form
+ * (setPhysicalBoundaryConditions), for one patch.  Arrays are the patch's
+ * ghosted arrays (cell: one array holding depth components; side: one array
+ * per axis), uniform ghost width gcw. */
+typedef struct
+{
+    int ndim, side, depth, gcw;
+    int ilo[3], ihi[3];
+    double dx[3];
+    double* arr[3];
+} orc_bpatch;
+
+static double* bp_at(const orc_bpatch* P, int a, int comp, const int* x)
+{
+    long off = 0, mul = 1;
+    int n[3];
+    for (int d = 0; d < P->ndim; ++d)
+        n[d] = P->ihi[d] - P->ilo[d] + 1 + 2 * P->gcw + ((P->side && d == a) ? 1 : 0);
+    for (int d = 0; d < P->ndim; ++d)
+    {
+        if (x[d] < P->ilo[d] - P->gcw || x[d] >= P->ilo[d] - P->gcw + n[d])
+        {
+            fprintf(stderr, "orc_phys_bdry: index %d of dim %d outside the ghost box\n", x[d], d);
+            abort();
+        }
+        off += (long)(x[d] - (P->ilo[d] - P->gcw)) * mul;
+        mul *= n[d];
+    }
+    return P->side ? P->arr[a] + off : P->arr[0] + (long)comp * mul + off;
+}
+
+/* the tangential dims of a face normal to dir, in the Fortran loop order (outer first) */
+static int bp_tangential(int ndim, int dir, int* t)
+{
+    int m = 0;
+    for (int d = ndim - 1; d >= 0; --d)
+        if (d != dir) t[m++] = d;
+    return m;
+}
+
+/* ccrobinphysbdryop1{x,y,z}: data box [dlo, dhi] (the patch box, or the side box of
+ * array a for the transverse use), tangential ranges from the coefficient box */
+static void bp_cc_op1(const orc_bpatch* P, int a, int comp, int loc, const int* dlo, const int* dhi, const int* blo,
+                      const int* bhi, double ca, double cb, double cg, int adjoint)
+{
+    const int ndim = P->ndim, dir = loc / 2, lower = (loc % 2) == 0;
+    const double h = P->dx[loc / ndim]; /* as the Fortran indexes it */
+    const int sgn = lower ? -1 : 1;
+    const int ig = lower ? dlo[dir] - 1 : dhi[dir] + 1, ii = lower ? dlo[dir] : dhi[dir];
+    int t[2];
+    const int nt = bp_tangential(ndim, dir, t);
+    int x[3] = {0, 0, 0};
+    const int o_lo = nt > 1 ? blo[t[0]] : 0, o_hi = nt > 1 ? bhi[t[0]] : 0;
+    for (int o = o_lo; o <= o_hi; ++o)
+        for (int q = blo[t[nt - 1]]; q <= bhi[t[nt - 1]]; ++q)
+        {
+            if (nt > 1) x[t[0]] = o;
+            x[t[nt - 1]] = q;
+            for (int i = 0; i < P->gcw; ++i)
+            {
+                const double n = 1.0 + 2.0 * i;
+                const double f_i = -(ca * n * h - 2.0 * cb) / (ca * n * h + 2.0 * cb);
+                const double f_g = 2.0 * n * h / (ca * n * h + 2.0 * cb);
+                int xg[3] = {x[0], x[1], x[2]}, xi[3] = {x[0], x[1], x[2]};
+                xg[dir] = ig + sgn * i;
+                xi[dir] = ii - sgn * i;
+                if (adjoint)
+                {
+                    const double u_g = *bp_at(P, a, comp, xg);
+                    *bp_at(P, a, comp, xi) += f_i * u_g;
+                }
+                else
+                {
+                    const double u_i = *bp_at(P, a, comp, xi);
+                    *bp_at(P, a, comp, xg) = f_i * u_i + f_g * cg;
+                }
+            }
+        }
+}
+
+/* scrobinphysbdryop1{x,y,z}: the normal component on its boundary face */
+static void bp_sc_op1(const orc_bpatch* P, int loc, const int* blo, const int* bhi, double ca, double cb, double cg,
+                      int adjoint)
+{
+    const int ndim = P->ndim, dir = loc / 2, lower = (loc % 2) == 0;
+    const double h = P->dx[loc / ndim];
+    const int sgn = lower ? -1 : 1;
+    const int ib = lower ? P->ilo[dir] : P->ihi[dir] + 1;
+    int t[2];
+    const int nt = bp_tangential(ndim, dir, t);
+    int x[3] = {0, 0, 0};
+    const int o_lo = nt > 1 ? blo[t[0]] : 0, o_hi = nt > 1 ? bhi[t[0]] : 0;
+    for (int o = o_lo; o <= o_hi; ++o)
+        for (int q = blo[t[nt - 1]]; q <= bhi[t[nt - 1]]; ++q)
+        {
+            if (nt > 1) x[t[0]] = o;
+            x[t[nt - 1]] = q;
+            int xb[3] = {x[0], x[1], x[2]};
+            xb[dir] = ib;
+            double* ub = bp_at(P, dir, 0, xb);
+            double u_b;
+            const int dirichlet = fabs(cb) < 1.0e-12;
+            if (dirichlet)
+            {
+                u_b = cg / ca;
+                *ub = u_b;
+            }
+            else
+                u_b = *ub;
+            for (int i = 1; i <= P->gcw; ++i)
+            {
+                double f_i, f_b, f_g = 0.0;
+                if (dirichlet)
+                {
+                    f_i = -1.0;
+                    f_b = 2.0;
+                }
+                else
+                {
+                    const double n = 2.0 * i;
+                    f_i = 1.0;
+                    f_b = -ca * n * h / cb;
+                    f_g = n * h / cb;
+                }
+                int xg[3] = {x[0], x[1], x[2]}, xi[3] = {x[0], x[1], x[2]};
+                xg[dir] = ib + sgn * i;
+                xi[dir] = ib - sgn * i;
+                if (adjoint)
+                {
+                    const double u_g = *bp_at(P, dir, 0, xg);
+                    *bp_at(P, dir, 0, xi) += f_i * u_g;
+                    *ub += f_b * u_g;
+                }
+                else
+                {
+                    const double u_i = *bp_at(P, dir, 0, xi);
+                    *bp_at(P, dir, 0, xg) = dirichlet ? f_i * u_i + f_b * u_b : f_i * u_i + f_b * u_b + f_g * cg;
+                }
+            }
+        }
+}
+
+/* boundary index and direction of codim-2 boxes in 3-D: edge axis, the two
+ * normal dims (n1 < n2), lower/upper per normal dim (PhysicalBoundaryUtilities
+ * location conventions) */
+static void bp_edge(int loc, int* e, int* n1, int* n2, int* up1, int* up2)
+{
+    if (loc < 4)
+    {
+        *e = 0, *n1 = 1, *n2 = 2, *up1 = loc & 1, *up2 = (loc >> 1) & 1;
+    }
+    else if (loc < 8)
+    {
+        *e = 1, *n1 = 0, *n2 = 2, *up1 = (loc - 4) >> 1, *up2 = (loc - 4) & 1;
+    }
+    else
+    {
+        *e = 2, *n1 = 0, *n2 = 1, *up1 = (loc - 8) & 1, *up2 = (loc - 8) >> 1;
+    }
+}
+
+/* ccrobinphysbdryop2 (3-D edges): linear extrapolation and its adjoint */
+static void bp_cc_edge(const orc_bpatch* P, int comp, int loc, const int* flo, const int* fhi, int adjoint)
+{
+    int e, n1, n2, u1, u2;
+    bp_edge(loc, &e, &n1, &n2, &u1, &u2);
+    const int b1 = u1 ? P->ihi[n1] : P->ilo[n1], b2 = u2 ? P->ihi[n2] : P->ilo[n2];
+    const int s1 = u1 ? 1 : -1, s2 = u2 ? 1 : -1;
+    /* Fortran loops k, j, i; every update keeps the edge index, so any loop order over
+       the edge index with (n2 outer, n1 inner) inside gives the same values */
+    for (int c2 = flo[n2]; c2 <= fhi[n2]; ++c2)
+        for (int c1 = flo[n1]; c1 <= fhi[n1]; ++c1)
+            for (int ce = flo[e]; ce <= fhi[e]; ++ce)
+            {
+                const int m1 = b1 + (b1 - c1 + s1), m2 = b2 + (b2 - c2 + s2);
+                int x[3], xm[3], xa[3], xb[3], xc[3], xd[3];
+                x[e] = xm[e] = xa[e] = xb[e] = xc[e] = xd[e] = ce;
+                x[n1] = c1, x[n2] = c2;
+                xm[n1] = m1, xm[n2] = m2;
+                xa[n1] = b1, xa[n2] = c2;
+                xb[n1] = c1, xb[n2] = b2;
+                xc[n1] = b1, xc[n2] = m2;
+                xd[n1] = m1, xd[n2] = b2;
+                if (adjoint)
+                {
+                    const double Ug = *bp_at(P, 0, comp, x);
+                    *bp_at(P, 0, comp, xm) += Ug;
+                    *bp_at(P, 0, comp, xa) += Ug;
+                    *bp_at(P, 0, comp, xb) += Ug;
+                    *bp_at(P, 0, comp, xc) -= Ug;
+                    *bp_at(P, 0, comp, xd) -= Ug;
+                }
+                else
+                    *bp_at(P, 0, comp, x) = *bp_at(P, 0, comp, xm) + (*bp_at(P, 0, comp, xa) - *bp_at(P, 0, comp, xc)) +
+                                            (*bp_at(P, 0, comp, xb) - *bp_at(P, 0, comp, xd));
+            }
+}
+
+/* ccrobinphysbdryop22d / ccrobinphysbdryop33d (corners) */
+static void bp_cc_corner(const orc_bpatch* P, int comp, int loc, const int* flo, const int* fhi, int adjoint)
+{
+    const int nd = P->ndim;
+    int b[3] = {0, 0, 0}, s[3] = {0, 0, 0};
+    for (int d = 0; d < nd; ++d)
+    {
+        const int up = (loc >> d) & 1;
+        b[d] = up ? P->ihi[d] : P->ilo[d];
+        s[d] = up ? 1 : -1;
+    }
+    const int zl = nd == 3 ? flo[2] : 0, zh = nd == 3 ? fhi[2] : 0;
+    for (int z = zl; z <= zh; ++z)
+        for (int y = flo[1]; y <= fhi[1]; ++y)
+            for (int xx = flo[0]; xx <= fhi[0]; ++xx)
+            {
+                const int x[3] = {xx, y, z};
+                int m[3] = {0, 0, 0};
+                for (int d = 0; d < nd; ++d) m[d] = b[d] + (b[d] - x[d] + s[d]);
+                int pk[3][3], mk[3][3];  /* +: keep d, others at b; -: d mirrored, others at b */
+                for (int d = 0; d < nd; ++d)
+                    for (int o = 0; o < 3; ++o)
+                    {
+                        pk[d][o] = o == d ? x[o] : b[o];
+                        mk[d][o] = o == d ? m[o] : b[o];
+                    }
+                if (adjoint)
+                {
+                    const double Ug = *bp_at(P, 0, comp, x);
+                    *bp_at(P, 0, comp, m) += Ug;
+                    for (int d = 0; d < nd; ++d) *bp_at(P, 0, comp, pk[d]) += Ug;
+                    for (int d = 0; d < nd; ++d) *bp_at(P, 0, comp, mk[d]) -= Ug;
+                }
+                else
+                {
+                    double v = *bp_at(P, 0, comp, m);
+                    for (int d = 0; d < nd; ++d) v = v + (*bp_at(P, 0, comp, pk[d]) - *bp_at(P, 0, comp, mk[d]));
+                    *bp_at(P, 0, comp, x) = v;
+                }
+            }
+}
+
+/* scrobinphysbdryop2 (3-D edges, side data): component n1 extrapolated along n2
+ * where its index along n1 is outside the side box, then component n2 along n1 */
+static void bp_sc_edge(const orc_bpatch* P, int loc, const int* flo, const int* fhi, int adjoint)
+{
+    int e, n1, n2, u1, u2;
+    bp_edge(loc, &e, &n1, &n2, &u1, &u2);
+    const int b[3] = {0, 0, 0};
+    int bd[3] = {b[0], b[1], b[2]}, sh[3] = {0, 0, 0};
+    bd[n1] = u1 ? P->ihi[n1] : P->ilo[n1];
+    bd[n2] = u2 ? P->ihi[n2] : P->ilo[n2];
+    sh[n1] = u1 ? -1 : 1;
+    sh[n2] = u2 ? -1 : 1;
+    for (int part = 0; part < 2; ++part)
+    {
+        const int comp = part == 0 ? n1 : n2, other = part == 0 ? n2 : n1;
+        int lo[3] = {flo[0], flo[1], flo[2]}, hi[3] = {fhi[0], fhi[1], fhi[2]};
+        hi[comp] += 1;
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int xx = lo[0]; xx <= hi[0]; ++xx)
+                {
+                    const int x[3] = {xx, y, z};
+                    if (!(x[comp] < P->ilo[comp] || x[comp] > P->ihi[comp] + 1)) continue;
+                    const double del = (double)abs(x[other] - bd[other]);
+                    int xb[3] = {x[0], x[1], x[2]}, xs[3] = {x[0], x[1], x[2]};
+                    xb[other] = bd[other];
+                    xs[other] = bd[other] + sh[other];
+                    if (adjoint)
+                    {
+                        const double u_g = *bp_at(P, comp, 0, x);
+                        *bp_at(P, comp, 0, xb) += (1.0 + del) * u_g;
+                        *bp_at(P, comp, 0, xs) -= del * u_g;
+                    }
+                    else
+                        *bp_at(P, comp, 0, x) = (1.0 + del) * *bp_at(P, comp, 0, xb) - del * *bp_at(P, comp, 0, xs);
+                }
+    }
+}
+
+/* scrobinphysbdryop22d / scrobinphysbdryop33d (corners, side data) */
+static void bp_sc_corner(const orc_bpatch* P, int loc, const int* flo, const int* fhi, int adjoint)
+{
+    const int nd = P->ndim;
+    int bd[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
+    for (int d = 0; d < nd; ++d)
+    {
+        const int up = (loc >> d) & 1;
+        bd[d] = up ? P->ihi[d] : P->ilo[d];
+        sh[d] = up ? -1 : 1;
+    }
+    for (int comp = 0; comp < nd; ++comp)
+    {
+        int lo[3] = {flo[0], flo[1], nd == 3 ? flo[2] : 0}, hi[3] = {fhi[0], fhi[1], nd == 3 ? fhi[2] : 0};
+        hi[comp] += 1;
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int xx = lo[0]; xx <= hi[0]; ++xx)
+                {
+                    const int x[3] = {xx, y, z};
+                    if (!(x[comp] < P->ilo[comp] || x[comp] > P->ihi[comp] + 1)) continue;
+                    double dsum = 1.0, del[3] = {0.0, 0.0, 0.0};
+                    for (int o = 0; o < nd; ++o)
+                        if (o != comp)
+                        {
+                            del[o] = (double)abs(x[o] - bd[o]);
+                            dsum = dsum + del[o];
+                        }
+                    int xb[3] = {x[0], x[1], x[2]};
+                    for (int o = 0; o < nd; ++o)
+                        if (o != comp) xb[o] = bd[o];
+                    if (adjoint)
+                    {
+                        const double u_g = *bp_at(P, comp, 0, x);
+                        *bp_at(P, comp, 0, xb) += dsum * u_g;
+                        for (int o = 0; o < nd; ++o)
+                            if (o != comp)
+                            {
+                                int xs[3] = {xb[0], xb[1], xb[2]};
+                                xs[o] += sh[o];
+                                *bp_at(P, comp, 0, xs) -= del[o] * u_g;
+                            }
+                    }
+                    else
+                    {
+                        double v = dsum * *bp_at(P, comp, 0, xb);
+                        for (int o = 0; o < nd; ++o)
+                            if (o != comp)
+                            {
+                                int xs[3] = {xb[0], xb[1], xb[2]};
+                                xs[o] += sh[o];
+                                v = v - del[o] * *bp_at(P, comp, 0, xs);
+                            }
+                        *bp_at(P, comp, 0, x) = v;
+                    }
+                }
+    }
+}
+
+/* One patch: boxes[k] = {codim, location, lo[3], hi[3]} (codim 1: the boundary
+ * box, the cell layer outside the face; codim >= 2: the fill box), abg[k][narr][3]
+ * the Robin coefficients of codim-1 boxes per array (cell: depth component;
+ * side: axis).  adjoint = 1: accumulateFromPhysicalBoundaryData (codim 3, 2,
+ * then 1; side: transverse then normal); 0: setPhysicalBoundaryConditions
+ * (codim 1 (side: normal, then transverse), 2, 3). */
+void orc_phys_bdry(int ndim, int side, int depth, int gcw, const int* ilo, const int* ihi, const double* dx,
+                   double** arrays, int nboxes, const int* boxes, const double* abg, int adjoint)
+{
+    orc_bpatch P;
+    P.ndim = ndim;
+    P.side = side;
+    P.depth = side ? 1 : depth;
+    P.gcw = gcw;
+    for (int d = 0; d < 3; ++d)
+    {
+        P.ilo[d] = d < ndim ? ilo[d] : 0;
+        P.ihi[d] = d < ndim ? ihi[d] : 0;
+        P.dx[d] = d < ndim ? dx[d] : 1.0;
+        P.arr[d] = d < (side ? ndim : 1) ? arrays[d] : 0;
+    }
+    const int narr = side ? ndim : depth;
+    /* phases: adjoint 3, 2, 1T, 1N; forward 1N, 1T, 2, 3 (cell: 1T is the cell op, 1N empty) */
+    const int phases_adj[4] = {3, 2, 10, 11}, phases_fwd[4] = {11, 10, 2, 3};
+    for (int ph = 0; ph < 4; ++ph)
+    {
+        const int phase = adjoint ? phases_adj[ph] : phases_fwd[ph];
+        for (int k = 0; k < nboxes; ++k)
+        {
+            const int* bx = boxes + 8 * k;
+            const int codim = bx[0], loc = bx[1];
+            const int* lo = bx + 2;
+            const int* hi = bx + 5;
+            if (phase == 3 && codim == 3 && ndim == 3)
+            {
+                for (int c = 0; c < (side ? 1 : depth); ++c)
+                    side ? bp_sc_corner(&P, loc, lo, hi, adjoint) : bp_cc_corner(&P, c, loc, lo, hi, adjoint);
+            }
+            else if (phase == 2 && codim == 2)
+            {
+                for (int c = 0; c < (side ? 1 : depth); ++c)
+                {
+                    if (ndim == 2)
+                        side ? bp_sc_corner(&P, loc, lo, hi, adjoint) : bp_cc_corner(&P, c, loc, lo, hi, adjoint);
+                    else
+                        side ? bp_sc_edge(&P, loc, lo, hi, adjoint) : bp_cc_edge(&P, c, loc, lo, hi, adjoint);
+                }
+            }
+            else if ((phase == 10 || phase == 11) && codim == 1)
+            {
+                const int dir = loc / 2;
+                if (!side)
+                {
+                    if (phase == 10)
+                        for (int c = 0; c < depth; ++c)
+                        {
+                            const double* q = abg + 3 * ((long)k * narr + c);
+                            bp_cc_op1(&P, 0, c, loc, P.ilo, P.ihi, lo, hi, q[0], q[1], q[2], adjoint);
+                        }
+                }
+                else if (phase == 10)
+                {
+                    for (int a = 0; a < ndim; ++a)
+                    {
+                        if (a == dir) continue;
+                        int shi[3] = {P.ihi[0], P.ihi[1], P.ihi[2]}, bhi[3] = {hi[0], hi[1], hi[2]};
+                        shi[a] += 1;  /* the side box of array a */
+                        bhi[a] += 1;  /* tangential extension of the coefficient box */
+                        const double* q = abg + 3 * ((long)k * narr + a);
+                        bp_cc_op1(&P, a, 0, loc, P.ilo, shi, lo, bhi, q[0], q[1], q[2], adjoint);
+                    }
+                }
+                else
+                {
+                    const double* q = abg + 3 * ((long)k * narr + dir);
+                    bp_sc_op1(&P, loc, lo, hi, q[0], q[1], q[2], adjoint);
+                }
+            }
+        }
+    }
 }

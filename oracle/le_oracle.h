@@ -149,6 +149,12 @@ void orc_fe_quadrature_points(int ndim, int nvars, int nelem, int nbasis, int nq
 void orc_fe_integrate_rhs(int nvars, int nelem, int nbasis, int nqp, const int* elem_nodes, const double* phi,
                           const double* JxW, const double* U_qp, double* F);
 
+/* CartCellRobinPhysBdryOp / CartSideRobinPhysBdryOp LINEAR ghost operators on one
+ * patch: adjoint = accumulateFromPhysicalBoundaryData, else setPhysicalBoundaryConditions.
+ * boxes: nboxes x {codim, location, lo[3], hi[3]}; abg: [nboxes][arrays][3]. */
+void orc_phys_bdry(int ndim, int side, int depth, int gcw, const int* ilo, const int* ihi, const double* dx,
+                   double** arrays, int nboxes, const int* boxes, const double* abg, int adjoint);
+
 void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
                           const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,
                           const int* bp, const double* bK, const double* bc, int nt, const int* tn, const double* tK,

@@ -388,3 +388,28 @@ def fe_integrate_rhs(elem_nodes, phi, JxW, U_qp, nnodes, F=None):
         L._ferhs_types = True
     L.orc_fe_integrate_rhs(nvars, nelem, nbasis, nqp, _i(en), _d(phi), _d(_dv(JxW)), _d(U), _d(out))
     return out
+
+
+def phys_bdry(arrays, ilower, iupper, gcw, dx, boxes, abg, side, adjoint, depth=1):
+    """CartCell/CartSideRobinPhysBdryOp LINEAR ops on one patch (orc_phys_bdry),
+    in place on float64 C-contiguous numpy arrays (cell: [arr]; side: one per axis).
+    boxes: list of (codim, location, lo, hi); abg: [nboxes][narrays][3]."""
+    ndim = len(ilower)
+    for a in arrays:
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    bx = np.zeros((max(len(boxes), 1), 8), dtype=np.int32)
+    for k, (c, l, lo, hi) in enumerate(boxes):
+        bx[k, 0], bx[k, 1] = c, l
+        bx[k, 2:2 + ndim] = lo
+        bx[k, 5:5 + ndim] = hi
+    ab = _dv(abg) if len(boxes) else np.zeros(3)
+    ptrs = (ctypes.POINTER(ctypes.c_double) * len(arrays))(*[_d(a) for a in arrays])
+    L = lib()
+    if not getattr(L, "_pb_types", False):
+        L.orc_phys_bdry.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 2 + \
+            [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.c_int,
+             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L._pb_types = True
+    L.orc_phys_bdry(ndim, 1 if side else 0, depth, gcw, _i(_iv(ilower)), _i(_iv(iupper)), _d(_dv(dx)), ptrs,
+                    len(boxes), _i(bx), _d(ab), 1 if adjoint else 0)
+    return arrays
