@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <sstream>
+#include <type_traits>
 
 namespace ibtk_amd
 {
@@ -227,6 +228,7 @@ template class SideData<double>;
 template class SideData<float>;
 template class LData<double>;
 template class LData<float>;
+template class LData<int>;
 
 // ----------------------------------------------------------------------------- LEInteractor
 
@@ -541,6 +543,101 @@ void LDataManager::endDataRedistribution(const LData<double>& X_data)
     if (d_impl->interp_plan != d_impl->spread_plan)
         abi_check(ibamr_plan_bin(d_impl->interp_plan, X_data.getArray(), n, nullptr), "LDataManager");
     d_impl->npoints = n;
+}
+
+std::vector<int> LDataManager::redistribute(LData<double>& X_data, const std::vector<LData<double>*>& data,
+                                            LData<int>* lag_index, bool error_if_points_leave_domain)
+{
+    const int n = X_data.getLocalNodeCount();
+    if (X_data.getDepth() != d_impl->ndim) throw Error("LDataManager::redistribute: X must have depth NDIM");
+    for (const LData<double>* d : data)
+        if (!d || d->getLocalNodeCount() != n) throw Error("LDataManager::redistribute: data sizes differ from X");
+    if (lag_index && lag_index->getLocalNodeCount() != n)
+        throw Error("LDataManager::redistribute: Lagrangian index size differs from X");
+    const int np = (int)d_impl->level.patches.size();
+    int* order = dev_alloc<int>((size_t)std::max(n, 1));
+    int* offsets = dev_alloc<int>((size_t)np + 1);
+    try
+    {
+        abi_check(ibamr_plan_redistribute(d_impl->spread_plan, X_data.getArray(),
+                                          lag_index ? lag_index->getArray() : nullptr, n,
+                                          error_if_points_leave_domain ? 1 : 0, order, offsets, nullptr, nullptr),
+                  "LDataManager::redistribute");
+        auto move = [&](auto& L) {
+            using T = typename std::remove_reference<decltype(*L.getArray())>::type;
+            LData<T> tmp(n, L.getDepth());
+            abi_check(ibamr_permute_rows(tmp.getArray(), L.getArray(), order, n, L.getDepth(), (int)sizeof(T), nullptr),
+                      "LDataManager::redistribute");
+            hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            L.swap(tmp);
+        };
+        move(X_data);
+        for (LData<double>* d : data) move(*d);
+        if (lag_index) move(*lag_index);
+    }
+    catch (...)
+    {
+        dev_free(order);
+        dev_free(offsets);
+        throw;
+    }
+    std::vector<int> h(np + 1);
+    hip_check(hipMemcpy(h.data(), offsets, sizeof(int) * (np + 1), hipMemcpyDeviceToHost), "hipMemcpy");
+    dev_free(order);
+    dev_free(offsets);
+    d_impl->npoints = -1;
+    endDataRedistribution(X_data);
+    return h;
+}
+
+IBStandardForceGen::IBStandardForceGen(int ndim, int num_nodes, const std::vector<Spring>& springs,
+                                       const std::vector<Beam>& beams, const std::vector<TargetPoint>& targets)
+    : d_ndim(ndim), d_n(num_nodes)
+{
+    std::vector<int> sm, ss, bm, bn, bp, tn;
+    std::vector<double> sk, sr, bK, bc, tK, tE, tX;
+    for (const Spring& s : springs)
+    {
+        sm.push_back(s.master);
+        ss.push_back(s.slave);
+        sk.push_back(s.stiffness);
+        sr.push_back(s.rest_length);
+    }
+    for (const Beam& b : beams)
+    {
+        bm.push_back(b.master);
+        bn.push_back(b.next);
+        bp.push_back(b.prev);
+        bK.push_back(b.rigidity);
+        for (int d = 0; d < ndim; ++d) bc.push_back(b.curvature[d]);
+    }
+    for (const TargetPoint& t : targets)
+    {
+        tn.push_back(t.node);
+        tK.push_back(t.stiffness);
+        tE.push_back(t.damping);
+        for (int d = 0; d < ndim; ++d) tX.push_back(t.X0[d]);
+    }
+    abi_check(ibamr_forces_create(&d_forces, ndim, num_nodes, (int)springs.size(), sm.data(), ss.data(), sk.data(),
+                                  sr.data(), (int)beams.size(), bm.data(), bn.data(), bp.data(), bK.data(), bc.data(),
+                                  (int)targets.size(), tn.data(), tK.data(), tE.data(), tX.data()),
+              "IBStandardForceGen");
+}
+
+IBStandardForceGen::~IBStandardForceGen()
+{
+    ibamr_forces_destroy(d_forces);
+}
+
+void IBStandardForceGen::computeLagrangianForce(LData<double>& F, const LData<double>& X, const LData<double>& U)
+{
+    if (F.getLocalNodeCount() != d_n || X.getLocalNodeCount() != d_n || U.getLocalNodeCount() != d_n ||
+        F.getDepth() != d_ndim || X.getDepth() != d_ndim || U.getDepth() != d_ndim)
+        throw Error("IBStandardForceGen::computeLagrangianForce: F, X and U must hold NDIM values per node");
+    abi_check(ibamr_forces_compute(d_forces, F.getArray(), IBAMR_FP64, X.getArray(), U.getArray(), IBAMR_FP64,
+                                   IBAMR_SPREAD_ACCUMULATE, nullptr),
+              "IBStandardForceGen::computeLagrangianForce");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
 }
 
 void LDataManager::spread(std::vector<std::shared_ptr<SideData<double>>>& f_data, const LData<double>& F_data,

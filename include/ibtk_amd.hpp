@@ -196,6 +196,13 @@ public:
     int getDepth() const { return d_depth; }
     void copyFromHost(const std::vector<T>& h);
     void copyToHost(std::vector<T>& h) const;
+    // exchange contents (and sizes) with another LData of the same type
+    void swap(LData& o)
+    {
+        std::swap(d_n, o.d_n);
+        std::swap(d_depth, o.d_depth);
+        std::swap(d_data, o.d_data);
+    }
 
 private:
     int d_n, d_depth;
@@ -215,6 +222,16 @@ public:
     // re-associates the points with cells (IndexUtilities::getCellIndex on the level)
     void endDataRedistribution(const LData<double>& X_data);
 
+    // beginDataRedistribution + endDataRedistribution (LDataManager.cpp:1349-1960) for
+    // the points this GPU holds: X is moved into the domain (periodic wrap, or clamp /
+    // error when error_if_points_leave_domain), the nodes are renumbered so that every
+    // patch's nodes are contiguous (patch order, cells in box order, Lagrangian index
+    // in a cell), and X, the arrays in data and lag_index (if given) are permuted to
+    // that order, then the points are binned again.  Returns the per-patch offsets
+    // (npatches + 1 entries).  Throws with the reference's message for escaped points.
+    std::vector<int> redistribute(LData<double>& X_data, const std::vector<LData<double>*>& data,
+                                  LData<int>* lag_index = nullptr, bool error_if_points_leave_domain = false);
+
     // LDataManager::spread (LDataManager.cpp:552-667): f <- f + S F on the patch
     // interiors, patch ghost values unchanged.  f_data[p] is patch p's data.
     void spread(std::vector<std::shared_ptr<SideData<double>>>& f_data, const LData<double>& F_data,
@@ -233,6 +250,45 @@ public:
 private:
     struct Impl;
     std::unique_ptr<Impl> d_impl;
+};
+
+// IBStandardForceGen (src/IB/IBStandardForceGen.cpp:254-305): Hookean springs
+// (default_spring_force), beams and target points of one structure, given as
+// IBStandardInitializer reads them (node indices 0-based; beam curvature and
+// target positions [element][NDIM]).
+struct Spring
+{
+    int master, slave;
+    double stiffness, rest_length;
+};
+struct Beam
+{
+    int prev, master, next;
+    double rigidity;
+    std::array<double, 3> curvature;
+};
+struct TargetPoint
+{
+    int node;
+    double stiffness, damping;
+    std::array<double, 3> X0;
+};
+
+class IBStandardForceGen
+{
+public:
+    IBStandardForceGen(int ndim, int num_nodes, const std::vector<Spring>& springs, const std::vector<Beam>& beams,
+                       const std::vector<TargetPoint>& targets);
+    ~IBStandardForceGen();
+    IBStandardForceGen(const IBStandardForceGen&) = delete;
+    IBStandardForceGen& operator=(const IBStandardForceGen&) = delete;
+    // computeLagrangianForce: F += springs + beams + target points at X (U damps
+    // the target points); IBMethod::computeLagrangianForce zeroes F first.
+    void computeLagrangianForce(LData<double>& F, const LData<double>& X, const LData<double>& U);
+
+private:
+    int d_ndim, d_n;
+    ibamr_forces* d_forces = nullptr;
 };
 
 // The IBStrategy hooks of IBMethod on one level.

@@ -10,6 +10,7 @@
 //   interpolate_01 interp <IB_4|IB_6|BSPLINE_3> <output>
 //   interpolate_01 side   <kernel> <output>     raw-array side interp + spread
 //   interpolate_01 level  <kernel> <output>     LDataManager spread/interp, forwardEulerStep
+//   interpolate_01 regrid <kernel> <output>     IBStandardForceGen, then LDataManager::redistribute
 #include "ibtk_amd.hpp"
 
 #include <cmath>
@@ -209,13 +210,80 @@ int run_level(const std::string& k, const std::string& out_name)
     }
     return 0;
 }
+int run_regrid(const std::string& k, const std::string& out_name)
+{
+    // periodic 16^3 level in two patches (x halves); 400 points, some outside the box
+    const int N = 16, n = 400;
+    Level L;
+    L.ndim = NDIM;
+    L.dx = {{1.0 / N, 1.0 / N, 1.0 / N}};
+    L.domain = Box(NDIM, Index(0, 0, 0), Index(N - 1, N - 1, N - 1));
+    L.x_lower = {{0, 0, 0}};
+    L.x_upper = {{1, 1, 1}};
+    L.periodic_shift = IntVector(N, N, N);
+    for (int q = 0; q < 2; ++q)
+    {
+        const double dx[3] = {1.0 / N, 1.0 / N, 1.0 / N}, xlo[3] = {0.5 * q, 0, 0}, xhi[3] = {0.5 * (q + 1), 1, 1};
+        L.patches.push_back(std::make_shared<Patch>(Box(NDIM, Index(8 * q, 0, 0), Index(8 * q + 7, N - 1, N - 1)),
+                                                    CartesianPatchGeometry(NDIM, dx, xlo, xhi)));
+    }
+    const int gw = LEInteractor::getMinimumGhostWidth(k);
+    auto ldm = std::make_shared<LDataManager>(L, k, k, IntVector(gw), n);
+    std::mt19937 seq(5u);
+    std::uniform_real_distribution<double> dist(-0.3, 1.3), small(-0.01, 0.01);
+    std::vector<double> Xh(NDIM * n), Uh(NDIM * n);
+    for (double& v : Xh) v = dist(seq);
+    for (double& v : Uh) v = small(seq);
+    std::vector<Spring> springs;
+    std::vector<Beam> beams;
+    std::vector<TargetPoint> targets;
+    for (int i = 0; i < n; ++i)
+    {
+        springs.push_back({i, (i + 1) % n, 1.0 + 0.01 * i, 0.05});
+        beams.push_back({(i + n - 1) % n, i, (i + 1) % n, 0.5, {{0.001, -0.002, 0.0}}});
+        if (i % 7 == 0) targets.push_back({i, 3.0, 0.5, {{0.5, 0.5, 0.5}}});
+    }
+    IBStandardForceGen gen(NDIM, n, springs, beams, targets);
+    LData<double> X(n, NDIM), F(n, NDIM), U(n, NDIM);
+    LData<int> lag(n, 1);
+    std::vector<int> lh(n);
+    for (int i = 0; i < n; ++i) lh[i] = n - 1 - i;
+    X.copyFromHost(Xh);
+    U.copyFromHost(Uh);
+    F.copyFromHost(std::vector<double>(NDIM * n, 0.0));
+    lag.copyFromHost(lh);
+    gen.computeLagrangianForce(F, X, U);
+    const std::vector<int> offsets = ldm->redistribute(X, {&F, &U}, &lag);
+    std::vector<double> Xa, Fa, Ua;
+    std::vector<int> la;
+    X.copyToHost(Xa);
+    F.copyToHost(Fa);
+    U.copyToHost(Ua);
+    lag.copyToHost(la);
+    std::ofstream out(out_name);
+    out.precision(17);
+    out << "# X_before U_before, then after redistribution: X F U lag; offsets\n";
+    for (int p = 0; p < n; ++p)
+    {
+        for (int d = 0; d < 3; ++d) out << Xh[3 * p + d] << ' ';
+        for (int d = 0; d < 3; ++d) out << Uh[3 * p + d] << ' ';
+        for (int d = 0; d < 3; ++d) out << Xa[3 * p + d] << ' ';
+        for (int d = 0; d < 3; ++d) out << Fa[3 * p + d] << ' ';
+        for (int d = 0; d < 3; ++d) out << Ua[3 * p + d] << ' ';
+        out << la[p] << '\n';
+    }
+    out << "offsets";
+    for (int v : offsets) out << ' ' << v;
+    out << '\n';
+    return 0;
+}
 } // namespace
 
 int main(int argc, char** argv)
 {
     if (argc != 4)
     {
-        std::cerr << "usage: " << argv[0] << " interp|side|level <kernel> <output>\n";
+        std::cerr << "usage: " << argv[0] << " interp|side|level|regrid <kernel> <output>\n";
         return 2;
     }
     const std::string mode = argv[1], k = argv[2], out = argv[3];
@@ -224,6 +292,7 @@ int main(int argc, char** argv)
         if (mode == "interp") return run_interp(k, out);
         if (mode == "side") return run_side(k, out);
         if (mode == "level") return run_level(k, out);
+        if (mode == "regrid") return run_regrid(k, out);
         std::cerr << "unknown mode " << mode << "\n";
         return 2;
     }

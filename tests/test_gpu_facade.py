@@ -11,6 +11,10 @@ raw-array plans (ibamr_plan_create_patch) behind it.
   * ... level <kernel>: LDataManager / IBMethod on a periodic level: f holds 1
     before the spread and keeps 1 in its ghost cells; U after the same-level
     ghost fill; X_new = X + dt U bitwise.
+  * ... regrid <kernel>: IBStandardForceGen then LDataManager::redistribute on a
+    two-patch periodic level: forces bitwise with the oracle at the old
+    positions, positions / order / Lagrangian indices / patch offsets exactly
+    as the oracle's redistribution, and F, U carried to the new order.
   * LevelPlan(box=...) in Python for cell and side data and several boxes.
 """
 import json
@@ -205,3 +209,27 @@ def test_patch_plan_spread_and_interior_mode(orc, ndim, grow):
                 inner[tuple(sl)] = True
                 assert np.array_equal(got[~inner], base[a][~inner])
                 assert np.max(np.abs(got[inner] - ref[a][inner])) <= 1e-13 * scale
+
+
+def test_cpp_forcegen_and_redistribute(tmp_path, orc):
+    lines = _run("regrid", "IB_4", tmp_path)
+    n = 400
+    rows = np.array([[float(v) for v in l.split()] for l in lines[1:1 + n]])
+    Xb, Ub, Xa, Fa, Ua, la = rows[:, 0:3], rows[:, 3:6], rows[:, 6:9], rows[:, 9:12], rows[:, 12:15], rows[:, 15]
+    offsets = [int(v) for v in lines[1 + n].split()[1:]]
+    i = np.arange(n)
+    springs = (i, (i + 1) % n, 1.0 + 0.01 * i, np.full(n, 0.05))
+    beams = ((i - 1) % n, i, (i + 1) % n, np.full(n, 0.5), np.tile([0.001, -0.002, 0.0], (n, 1)))
+    tn = np.arange(0, n, 7)
+    targets = (tn, np.full(len(tn), 3.0), np.full(len(tn), 0.5), np.full((len(tn), 3), 0.5))
+    F = orc.lagrangian_force(3, Xb, U=Ub, springs=springs, beams=beams, targets=targets)
+    lag = n - 1 - i
+    boxes = [([0, 0, 0], [7, 15, 15]), ([8, 0, 0], [15, 15, 15])]
+    rc, Xw, order, poff, _ = orc.redistribute(Xb, [0.0] * 3, [1.0] * 3, [1.0 / 16] * 3, [0] * 3, [15] * 3,
+                                              [True] * 3, boxes, lag=lag)
+    assert rc == 0
+    assert np.array_equal(Xa, Xw[order])
+    assert np.array_equal(Fa, F[order])
+    assert np.array_equal(Ua, Ub[order])
+    assert np.array_equal(la, lag[order])
+    assert offsets == list(poff)
