@@ -20,8 +20,8 @@
 //          sequential adds, over chunks cut per bin, so the result equals the
 //          reference loop run over the oracle's z-bucketed entry order.
 //
-// Interpolation (k_interp3): one 256-thread workgroup per superbin (2x2x2
-// bins = 16^3 cells).  For each array it finds the exact stencil extent of its
+// Interpolation (k_interp3): one IBAMR_I3_THREADS-thread (512) workgroup per
+// superbin (2x2x2 bins = 16^3 cells).  For each array it finds the exact stencil extent of its
 // points, stages that block of the ghosted array into LDS with coalesced
 // loads, then each thread sums the W^3 stencil of its points from LDS.
 #pragma once

@@ -8,19 +8,21 @@
 //
 // Design (MI355X-first, not a translation of the per-point Fortran loops):
 //  * binning: IndexUtilities::getCellIndex per point (FP64, bit-exact), then a
-//    stable radix sort of points by spread bin (a cube of cells); CSR offsets.
-//  * spread: owner-computes gather.  One 64-lane wavefront owns one tile of
-//    grid values of one array (side axis or depth component) of one patch, in
-//    LDS.  It visits the bins whose points can reach the tile in a fixed order
-//    (bin z, image z, bin y, image y, bin x, image x) and the points of each
-//    bin in index order, and adds each point's clipped stencil to the tile.
-//    A tile is touched by one wavefront only, so every grid value sums its
-//    contributions sequentially in that fixed order: no atomics, bitwise
-//    reproducible, and identical to the reference loop run over the same
-//    entry order.  Periodic images (LIndexSetData.cpp:80-117) are generated on
-//    the fly from the bin offsets.
-//  * interp: one thread per point in bin order (neighbouring threads read
-//    neighbouring grid values), all axes per thread, overwriting U.
+//    stable LSD radix sort of the points by spread bin (a cube of cells; no
+//    device-wide atomics); CSR offsets by a binary search per bin.
+//  * spread: owner-computes gather.  A tile of grid values of one array (side
+//    axis or depth component) of one patch belongs to one wavefront (2-D here:
+//    a 16x16 tile in LDS; 3-D in le_kernels3d.h: 8x8 columns of z values in
+//    registers).  It visits the bins whose points can reach the tile in a fixed
+//    order (bin z, image z, bin y, image y, bin x, image x) and the points of
+//    each bin in index order, so every grid value sums its contributions in a
+//    fixed order: no atomics, bitwise reproducible.  Periodic images
+//    (LIndexSetData.cpp:80-117) are generated on the fly from the bin offsets.
+//  * interp: 2-D one thread per point in bin order; 3-D one workgroup per
+//    superbin with the grid block staged in LDS (le_kernels3d.h).
+//  * around the path: redistribution at a regrid (radix sorts by patch, cell and
+//    Lagrangian index), same-level and periodic ghost fills, the physical
+//    boundary (Robin) operators, the forward-Euler update.
 #include "ibamr_amd.h"
 #include "le_weights.h"
 
