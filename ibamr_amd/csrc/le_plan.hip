@@ -956,21 +956,53 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a)
 
 // ----------------------------------------------------------------------------- periodic ghost fill
 
+// Periodic ghost fill of one array of a single-patch, fully periodic level:
+// every value outside the core [g, g+N) of each axis takes the value of its
+// periodic image, which lies in the core (the ghost width never exceeds the
+// period).  Only the ghost shell is enumerated: z slabs (whole planes), then y
+// slabs inside the core z range, then x slabs inside the core y and z ranges.
 template <typename T>
 __global__ void k_fill_periodic(T* __restrict__ arr, int n0, int n1, int n2, int g0, int g1, int g2, int N0, int N1,
-                                int N2, long total)
+                                int N2, long sz, long sy, long sx)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= total) return;
-    const int i0 = (int)(e % n0);
-    const int i1 = (int)((e / n0) % n1);
-    const int i2 = (int)(e / ((long)n0 * n1));
-    // periodic image inside the first N entries of the interior along every axis
-    const int j0 = ((i0 - g0) % N0 + N0) % N0 + g0;
-    const int j1 = ((i1 - g1) % N1 + N1) % N1 + g1;
-    const int j2 = ((i2 - g2) % N2 + N2) % N2 + g2;
-    if (j0 == i0 && j1 == i1 && j2 == i2) return;
-    arr[e] = arr[(long)j0 + (long)n0 * (j1 + (long)n1 * j2)];
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i0, i1, i2;
+    if (t < sz)
+    {
+        const long plane = (long)n0 * n1;
+        const int q = (int)(t / plane);                       // which ghost plane
+        const long r = t - (long)q * plane;
+        i2 = q < g2 ? q : q - g2 + g2 + N2;                   // lower planes, then upper planes
+        i1 = (int)(r / n0);
+        i0 = (int)(r - (long)i1 * n0);
+    }
+    else if ((t -= sz) < sy)
+    {
+        const long slab = (long)(n1 - N1) * n0;              // per core z plane
+        const int z = (int)(t / slab);
+        const long r = t - (long)z * slab;
+        const int q = (int)(r / n0);
+        i0 = (int)(r - (long)q * n0);
+        i1 = q < g1 ? q : q - g1 + g1 + N1;
+        i2 = g2 + z;
+    }
+    else if ((t -= sy) < sx)
+    {
+        const int w = n0 - N0;                               // per core (y, z) row
+        const long row = t / w;
+        const int q = (int)(t - row * w);
+        i0 = q < g0 ? q : q - g0 + g0 + N0;
+        i1 = g1 + (int)(row % N1);
+        i2 = g2 + (int)(row / N1);
+    }
+    else
+        return;
+    int j0 = i0 - g0, j1 = i1 - g1, j2 = i2 - g2;
+    j0 += j0 < 0 ? N0 : (j0 >= N0 ? -N0 : 0);
+    j1 += j1 < 0 ? N1 : (j1 >= N1 ? -N1 : 0);
+    j2 += j2 < 0 ? N2 : (j2 >= N2 ? -N2 : 0);
+    arr[(long)i0 + (long)n0 * (i1 + (long)n1 * i2)] =
+        arr[(long)(j0 + g0) + (long)n0 * ((j1 + g1) + (long)n1 * (j2 + g2))];
 }
 
 // Same-level ghost fill (the RefineSchedule::fillData step before interp,
@@ -2546,18 +2578,24 @@ int ibamr_plan_fill_periodic_ghosts(ibamr_plan* P, int slot, void* stream)
             N[d] = pd.ihi[d] - pd.ilo[d] + 1;
         }
         const long total = (long)n[0] * n[1] * n[2];
+        // the core along a side axis is the N faces [g, g+N): face N is the image of face 0
+        const long sz = (long)(n[2] - N[2]) * n[0] * n[1];
+        const long sy = (long)N[2] * (n[1] - N[1]) * n[0];
+        const long sx = (long)N[2] * N[1] * (n[0] - N[0]);
+        const long shell = sz + sy + sx;
+        if (shell == 0) continue;
         for (int c = 0; c < ncomps; ++c)
         {
             const int B = 256;
-            const long G = (total + B - 1) / B;
+            const long G = (shell + B - 1) / B;
             if (P->precision == IBAMR_FP64)
                 hipLaunchKernelGGL(k_fill_periodic<double>, dim3((unsigned)G), dim3(B), 0, s,
                                    static_cast<double*>(ptrs[a]) + c * total, n[0], n[1], n[2], g[0], g[1], g[2],
-                                   N[0], N[1], N[2], total);
+                                   N[0], N[1], N[2], sz, sy, sx);
             else
                 hipLaunchKernelGGL(k_fill_periodic<float>, dim3((unsigned)G), dim3(B), 0, s,
                                    static_cast<float*>(ptrs[a]) + c * total, n[0], n[1], n[2], g[0], g[1], g[2],
-                                   N[0], N[1], N[2], total);
+                                   N[0], N[1], N[2], sz, sy, sx);
             HIP_OK(hipGetLastError());
         }
     }
