@@ -502,6 +502,32 @@ __global__ void k_scatter_points(const int* __restrict__ inv, int n, int ndim, c
         for (int c = 0; c < fdepth; ++c) Fs[(size_t)fdepth * k + c] = F[(size_t)fdepth * p + c];
 }
 
+// The same permutation driven by the sorted order, for records of D positions and
+// D force components (side data): each thread gathers one point's record (all
+// loads first) and writes it to its slot, so the writes are coalesced.  Measured
+// faster than the scattered-write form for these 24 + 12-byte records.
+template <int D, typename T>
+__global__ void k_gather_points(const int* __restrict__ perm, int n, const double* __restrict__ X,
+                                double* __restrict__ Xs, const T* __restrict__ F, T* __restrict__ Fs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t p = (size_t)perm[i];
+    double x[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = X[D * p + d];
+#pragma unroll
+    for (int d = 0; d < D; ++d) Xs[D * (size_t)i + d] = x[d];
+    if (F)
+    {
+        T f[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) f[d] = F[D * p + d];
+#pragma unroll
+        for (int d = 0; d < D; ++d) Fs[D * (size_t)i + d] = f[d];
+    }
+}
+
 // ----------------------------------------------------------------------------- spread
 
 template <typename T>
@@ -2433,9 +2459,14 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         if (P->npoints > 0)
         {
             const int B = 256;
-            hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
-                               P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
-                               static_cast<T*>(P->Fs_d));
+            if (P->fdepth == NDIM)
+                hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
+                                   P->perm_d, P->npoints, X, P->Xs_d, static_cast<const T*>(F),
+                                   static_cast<T*>(P->Fs_d));
+            else
+                hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
+                                   P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
+                                   static_cast<T*>(P->Fs_d));
             HIP_OK(hipGetLastError());
             P->xs_valid = true;
         }
@@ -2483,9 +2514,8 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
         if (X)  // else: reuse the positions the last spread/interp put in bin order
         {
             const int B = 256;
-            hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
-                               P->npoints, NDIM, X, P->Xs_d, 0, static_cast<const T*>(nullptr),
-                               static_cast<T*>(nullptr));
+            hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
+                               P->npoints, X, P->Xs_d, static_cast<const T*>(nullptr), static_cast<T*>(nullptr));
             HIP_OK(hipGetLastError());
             P->xs_valid = true;
         }
