@@ -268,38 +268,39 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
         int4 nc4 = make_int4(0, 0, 0, 0);
         double nX[3] = {0.0, 0.0, 0.0};
         T nF = T(0);
-        lane_point(nk, noc);
-        if (nk >= 0)
-        {
+        int nst[3] = {0, 0, 0};   // FP32C: precomputed stencil starts and weights
+        float nw[3][W];
+        // the lane's point of the current chunk: position (FP64) or precomputed
+        // starts and weights (FP32C, rows of the point's axis variants), F, cell
+        auto load_point = [&]() {
+            if (nk < 0) return;
             nc4 = a.cells[nk];
+            if (FP64)
+            {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
+                for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
+            }
+            else
+            {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    const int v = (a.nvar == 2 && d == axis) ? 1 : 0;
+                    const size_t row = (size_t)(d * a.nvar + v) * a.npts + nk;
+                    nst[d] = a.st[row];
+#pragma unroll
+                    for (int j = 0; j < W; ++j) nw[d][j] = a.wt[row * W + j];
+                }
+            }
             nF = a.F[(size_t)a.fdepth * nk + fcomp];
-        }
+        };
+        lane_point(nk, noc);
+        load_point();
         while (cvalid)
         {
             const int k = nk, oc = noc;
             const int4 c4 = nc4;
-            const double Xk[3] = {nX[0], nX[1], nX[2]};
             const T Fv = nF;
-            // advance and prefetch the next chunk
-            if (MERGE)
-                cq += 64;
-            else
-            {
-                crp += 64;
-                normalize();
-            }
-            cvalid = MERGE ? cq < gtotal : cr < ngc;
-            nk = -1;
-            if (cvalid) lane_point(nk, noc);
-            if (nk >= 0)
-            {
-                nc4 = a.cells[nk];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
-                nF = a.F[(size_t)a.fdepth * nk + fcomp];
-            }
 
             int my_zz = 1 << 20;
             if (k >= 0)
@@ -317,12 +318,27 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                 {
                     T w[3][W];
                     int bse[3];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d)
+                    if (FP64)
                     {
-                        // X + Xshift, Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
-                        const double Xs = Xk[d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
-                        bse[d] = P.ilo[d] + weights_1d<KER>(Xs, xl[d], a.dx[d], w[d]);
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                        {
+                            // X + Xshift, Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
+                            const double Xs = nX[d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
+                            bse[d] = P.ilo[d] + weights_1d<KER>(Xs, xl[d], a.dx[d], w[d]);
+                        }
+                    }
+                    else
+                    {
+                        // precomputed in the level's frame (k_point_weights); an image
+                        // moves the stencil by whole periods
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                        {
+                            bse[d] = nst[d] + o[d] * a.pshift[d];
+#pragma unroll
+                            for (int j = 0; j < W; ++j) w[d][j] = (T)nw[d][j];
+                        }
                     }
                     const int dx0 = bse[0] - flo[0], dy0 = bse[1] - flo[1], zz = bse[2] - flo[2];
                     if (dx0 > -W && dx0 < 8 && dy0 > -W && dy0 < 8 && zz > -W && zz < ZC)
@@ -354,6 +370,18 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                 }
             }
             __syncthreads();
+            // advance, and load the next chunk's points while this one is applied
+            if (MERGE)
+                cq += 64;
+            else
+            {
+                crp += 64;
+                normalize();
+            }
+            cvalid = MERGE ? cq < gtotal : cr < ngc;
+            nk = -1;
+            if (cvalid) lane_point(nk, noc);
+            load_point();
             const bool chunk_any = __ballot(my_zz < (1 << 20)) != 0;
             if (chunk_any)
             {
@@ -497,7 +525,7 @@ __device__ __forceinline__ int interp_owner(const InterpArgs<T>& a, int k, const
 }
 
 template <int KER, typename T>
-__global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds_elems, int nsb, int per)
+__global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds_elems, int nsb, int nsx)
 {
     using KT = KernelTraits<KER>;
     constexpr int W = KT::W;
@@ -509,9 +537,12 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
     // extents: [0] bases with the patch x_lower, [1] with x_lower - dx/2 (the side axis)
     __shared__ int s_lo[2][3], s_hi[2][3], s_pid, s_flag;
 
-    // XCD-aware: workgroup i runs on XCD i % 8, which takes a contiguous run
-    // of superbins (a z-slab), so neighbours share their halo in one L2
-    const int sb = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    // XCD-aware: workgroup i runs on XCD i % 8.  The rows of superbins along x
+    // are dealt to the XCDs in turn (row r to XCD r % 8), each XCD taking its
+    // rows in order: a row's superbins share their x halo in one L2, and every
+    // XCD gets a share of every region of the point cloud.
+    const int xi = (int)(blockIdx.x % 8), qi = (int)(blockIdx.x / 8);
+    const int sb = (xi + 8 * (qi / nsx)) * nsx + qi % nsx;
     if (sb >= nsb) return;
     const int s0 = a.bin_start[sb * 8], s1 = a.bin_start[sb * 8 + 8];
     if (s0 == s1) return;

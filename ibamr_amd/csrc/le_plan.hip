@@ -207,6 +207,11 @@ struct ibamr_plan
     double* Xs_d = nullptr;            // positions in bin order
     bool xs_valid = false;             // Xs holds the positions of the last spread/interp (same binning)
     void* Fs_d = nullptr;              // forces in bin order
+    // FP32C 3-D: per point in bin order, axis d and variant v, the first stencil
+    // index (st_d[(d*nvar+v)*n + k]) and the W weights (wt_d[((d*nvar+v)*n + k)*W + j])
+    float* wt_d = nullptr;
+    int* st_d = nullptr;
+    int nvar = 1;                      // variants per axis: 2 for side data (x_lower, x_lower - dx/2)
     int* bin_start_d = nullptr;        // [nmem+2]
 
     // optional timing of the main spread / interpolation kernels: event pairs
@@ -502,6 +507,49 @@ __global__ void k_scatter_points(const int* __restrict__ inv, int n, int ndim, c
         for (int c = 0; c < fdepth; ++c) Fs[(size_t)fdepth * k + c] = F[(size_t)fdepth * p + c];
 }
 
+// FP32C 3-D spread set-up: for each point (bin order), axis d and variant v
+// (v = 0: the level's x_lower; v = 1: x_lower - dx/2, the normal axis of side
+// data, LEInteractor.cpp:3689) the first stencil index in the level's index
+// space and the W weights, in the level's frame.  The spread's tiles then stage
+// a candidate with loads instead of recomputing its weights for every tile it
+// reaches; a patch or periodic image shifts the start by whole cells.
+template <int KER>
+__global__ void k_point_weights(int n, const double* __restrict__ Xs, int nvar, double3 xlo, double3 dx, int3 lo,
+                                float* __restrict__ wt, int* __restrict__ st)
+{
+    constexpr int W = KernelTraits<KER>::W;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double xl[3] = {xlo.x, xlo.y, xlo.z}, h[3] = {dx.x, dx.y, dx.z};
+    const int il[3] = {lo.x, lo.y, lo.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const double X = Xs[3 * (size_t)k + d];
+        for (int v = 0; v < nvar; ++v)
+        {
+            float w[W];
+            const int s0 = weights_1d<KER>(X, v ? xl[d] - 0.5 * h[d] : xl[d], h[d], w);
+            const size_t row = (size_t)(d * nvar + v) * n + k;
+            st[row] = il[d] + s0;
+#pragma unroll
+            for (int j = 0; j < W; ++j) wt[row * W + j] = w[j];
+        }
+    }
+}
+
+int kernel_width(int k)
+{
+    switch (k)
+    {
+    case K_IB4: return KernelTraits<K_IB4>::W;
+    case K_IB6: return KernelTraits<K_IB6>::W;
+    case K_BS3: return KernelTraits<K_BS3>::W;
+    case K_PWL: return KernelTraits<K_PWL>::W;
+    default: return 6;
+    }
+}
+
 // The same permutation driven by the sorted order, for records of D positions and
 // D force components (side data): each thread gathers one point's record (all
 // loads first) and writes it to its slot, so the writes are coalesced.  Measured
@@ -554,6 +602,11 @@ struct SpreadArgs
     int reach;
     int side;
     int mode;
+    // FP32C 3-D: the points' stencil starts and weights in the level's frame
+    const float* wt;
+    const int* st;
+    int nvar;
+    int npts;
 };
 
 template <int NDIM, int TILE>
@@ -1190,6 +1243,10 @@ void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double
     a.reach = P->reach;
     a.side = P->centering == IBAMR_SIDE;
     a.mode = mode;
+    a.wt = P->wt_d;
+    a.st = P->st_d;
+    a.nvar = P->nvar;
+    a.npts = P->npoints;
 }
 } // namespace
 
@@ -1956,6 +2013,8 @@ int ibamr_plan_destroy(ibamr_plan* P)
     dfree(P->cell_sorted_d);
     dfree(P->Xs_d);
     dfree(P->Fs_d);
+    dfree(P->wt_d);
+    dfree(P->st_d);
     dfree(P->bin_start_d);
     dfree(P->keys2_d);
     dfree(P->pcell_off_d);
@@ -2137,18 +2196,31 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
                     for (int tx = 0; tx < nt[0]; ++tx) tiles.push_back(TileDesc{i, c, {tx, ty, tz}});
         }
     }
-    // XCD-aware launch order: workgroup i runs on XCD i % 8, so give every XCD a
-    // contiguous run of the natural (patch, array, z, y, x) order; neighbouring
-    // tiles then share point data and output lines in the same L2.
+    // XCD-aware launch order: workgroup i runs on XCD i % 8.  The rows of tiles
+    // along x (one (patch, array, z, y) each) are dealt to the XCDs in turn, and
+    // each XCD runs its rows in the natural (patch, array, z, y) order: a row's
+    // tiles share their x halo in one L2, every XCD gets a share of every region
+    // (point clouds are rarely uniform, so contiguous thirds or eighths of the
+    // domain would leave some XCDs idle), and all XCDs sweep the domain together,
+    // so the point data they re-read is recent in the shared last-level cache.
     {
-        const size_t n = tiles.size();
-        const size_t per = (n + 7) / 8;
-        std::vector<TileDesc> ord(8 * per, TileDesc{-1, 0, {0, 0, 0}});
-        for (size_t i = 0; i < ord.size(); ++i)
+        std::vector<std::vector<TileDesc>> q(8);
+        size_t g = 0;
+        for (size_t i = 0; i < tiles.size(); ++i)
         {
-            const size_t nat = (i % 8) * per + i / 8;
-            if (nat < n) ord[i] = tiles[nat];
+            const TileDesc& t = tiles[i];
+            if (i > 0)
+            {
+                const TileDesc& u = tiles[i - 1];
+                if (t.patch != u.patch || t.comp != u.comp || t.t[1] != u.t[1] || t.t[2] != u.t[2]) ++g;
+            }
+            q[g % 8].push_back(t);
         }
+        size_t per = 0;
+        for (const auto& v : q) per = std::max(per, v.size());
+        std::vector<TileDesc> ord(8 * per, TileDesc{-1, 0, {0, 0, 0}});
+        for (int x = 0; x < 8; ++x)
+            for (size_t i = 0; i < q[x].size(); ++i) ord[8 * i + x] = q[x][i];
         tiles.swap(ord);
     }
     P->ntiles = (int)tiles.size();
@@ -2207,6 +2279,13 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     PLAN_HIP(hipMalloc(&P->cell_sorted_d, sizeof(int4) * np));
     PLAN_HIP(hipMalloc(&P->Xs_d, sizeof(double) * ndim * np));
     PLAN_HIP(hipMalloc(&P->Fs_d, (C->precision == IBAMR_FP64 ? 8 : 4) * (size_t)P->fdepth * np));
+    if (ndim == 3 && C->precision != IBAMR_FP64)
+    {
+        P->nvar = C->centering == IBAMR_SIDE ? 2 : 1;
+        const size_t rows = (size_t)3 * P->nvar * np;
+        PLAN_HIP(hipMalloc(&P->wt_d, sizeof(float) * kernel_width(k) * rows));
+        PLAN_HIP(hipMalloc(&P->st_d, sizeof(int) * rows));
+    }
     PLAN_HIP(hipMalloc(&P->bin_start_d, sizeof(int) * (P->nmem + 2)));
     {
         const size_t nrs = (size_t)RS_DIGITS * ((np + RS_TILE - 1) / RS_TILE);
@@ -2469,6 +2548,16 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
                                    static_cast<T*>(P->Fs_d));
             HIP_OK(hipGetLastError());
             P->xs_valid = true;
+            if (NDIM == 3 && sizeof(T) == 4)
+            {
+                if (!P->wt_d || !P->st_d) return fail(IBAMR_ERR_INVALID, "internal: no weight buffers");
+                hipLaunchKernelGGL((k_point_weights<KER>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
+                                   P->npoints, P->Xs_d, P->nvar,
+                                   make_double3(P->dom_xlo[0], P->dom_xlo[1], P->dom_xlo[2]),
+                                   make_double3(P->dx[0], P->dx[1], P->dx[2]),
+                                   make_int3(P->dom_lo[0], P->dom_lo[1], P->dom_lo[2]), P->wt_d, P->st_d);
+                HIP_OK(hipGetLastError());
+            }
         }
         a.X = P->Xs_d;
         a.F = static_cast<const T*>(P->Fs_d);
@@ -2547,8 +2636,10 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
             const int trc = time_begin(P, 1, s, &t_end);
             if (trc != IBAMR_OK) return trc;
             const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
-            const int per = (nsb + 7) / 8;  // superbins per XCD (see k_interp3)
-            hipLaunchKernelGGL((k_interp3<KER, T>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems, nsb, per);
+            const int nrow = P->nsb[1] * P->nsb[2];
+            const int per = (nrow + 7) / 8 * P->nsb[0];  // superbins per XCD (see k_interp3)
+            hipLaunchKernelGGL((k_interp3<KER, T>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems, nsb,
+                               P->nsb[0]);
         }
         else
         {
