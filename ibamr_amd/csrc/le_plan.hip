@@ -206,6 +206,7 @@ struct ibamr_plan
     int4* cell_sorted_d = nullptr;
     double* Xs_d = nullptr;            // positions in bin order
     bool xs_valid = false;             // Xs holds the positions of the last spread/interp (same binning)
+    bool ws_valid = false;             // wt_d/st_d hold the stencils of Xs (FP32C 3-D)
     void* Fs_d = nullptr;              // forces in bin order
     // FP32C 3-D: per point in bin order, axis d and variant v, the first stencil
     // index (st_d[(d*nvar+v)*n + k]) and the W weights (wt_d[((d*nvar+v)*n + k)*W + j])
@@ -536,6 +537,21 @@ __global__ void k_point_weights(int n, const double* __restrict__ Xs, int nvar, 
             for (int j = 0; j < W; ++j) wt[row * W + j] = w[j];
         }
     }
+}
+
+// FP32C 3-D: stencil starts and weights of the bin-ordered positions, used by
+// the spread and the interpolation of the same positions
+template <int KER>
+int launch_point_weights(ibamr_plan* P, hipStream_t s)
+{
+    if (!P->wt_d || !P->st_d) return fail(IBAMR_ERR_INVALID, "internal: no weight buffers");
+    const int B = 256;
+    hipLaunchKernelGGL((k_point_weights<KER>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->npoints, P->Xs_d,
+                       P->nvar, make_double3(P->dom_xlo[0], P->dom_xlo[1], P->dom_xlo[2]),
+                       make_double3(P->dx[0], P->dx[1], P->dx[2]),
+                       make_int3(P->dom_lo[0], P->dom_lo[1], P->dom_lo[2]), P->wt_d, P->st_d);
+    HIP_OK(hipGetLastError());
+    return IBAMR_OK;
 }
 
 int kernel_width(int k)
@@ -897,6 +913,13 @@ struct InterpArgs
     int fdepth;
     int depth;
     int side;
+    // FP32C 3-D: the points' stencil starts and weights in the level's frame
+    // (null: evaluate them from X and the level's x_lower / lower index)
+    const float* wt;
+    const int* st;
+    int nvar;
+    double lev_xlo[3];
+    int lev_lo[3];
 };
 
 // Compensated (FP32) or plain (FP64, same order as the reference) sum.
@@ -2317,9 +2340,10 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
         const int rc = dispatch(3, k, C->precision, [&](auto, auto ker, auto prec) -> int {
             using T = decltype(prec);
             constexpr int KER = decltype(ker)::value;
-            HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp3<KER, T>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       interp3_lds_elems<KER, T>() * (int)sizeof(T)));
+            for (const void* f : {reinterpret_cast<const void*>(&k_interp3<KER, T, false>),
+                                  reinterpret_cast<const void*>(&k_interp3<KER, T, true>)})
+                HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           interp3_lds_elems<KER, T>() * (int)sizeof(T)));
             return IBAMR_OK;
         });
         if (rc != IBAMR_OK) return cleanup_fail(rc);
@@ -2392,6 +2416,7 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     hipStream_t s = static_cast<hipStream_t>(stream);
     P->npoints = npoints;
     P->xs_valid = false;
+    P->ws_valid = false;
     const int B = 256;
     const int m = P->nmem + 1;  // bins plus the out-of-domain sentinel
     if (npoints == 0)
@@ -2550,13 +2575,9 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
             P->xs_valid = true;
             if (NDIM == 3 && sizeof(T) == 4)
             {
-                if (!P->wt_d || !P->st_d) return fail(IBAMR_ERR_INVALID, "internal: no weight buffers");
-                hipLaunchKernelGGL((k_point_weights<KER>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
-                                   P->npoints, P->Xs_d, P->nvar,
-                                   make_double3(P->dom_xlo[0], P->dom_xlo[1], P->dom_xlo[2]),
-                                   make_double3(P->dx[0], P->dx[1], P->dx[2]),
-                                   make_int3(P->dom_lo[0], P->dom_lo[1], P->dom_lo[2]), P->wt_d, P->st_d);
-                HIP_OK(hipGetLastError());
+                const int rc = launch_point_weights<KER>(P, s);
+                if (rc != IBAMR_OK) return rc;
+                P->ws_valid = true;
             }
         }
         a.X = P->Xs_d;
@@ -2607,6 +2628,17 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
                                P->npoints, X, P->Xs_d, static_cast<const T*>(nullptr), static_cast<T*>(nullptr));
             HIP_OK(hipGetLastError());
             P->xs_valid = true;
+            P->ws_valid = false;
+        }
+        // FP32C: the spread set-up's stencils of these positions when it computed
+        // them, else the kernel evaluates the same ones (level frame)
+        a.wt = P->ws_valid ? P->wt_d : nullptr;
+        a.st = P->ws_valid ? P->st_d : nullptr;
+        a.nvar = P->nvar;
+        for (int d = 0; d < 3; ++d)
+        {
+            a.lev_xlo[d] = P->dom_xlo[d];
+            a.lev_lo[d] = P->dom_lo[d];
         }
         a.X = P->Xs_d;
         a.U = static_cast<T*>(U);
@@ -2638,8 +2670,12 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
             const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
             const int nrow = P->nsb[1] * P->nsb[2];
             const int per = (nrow + 7) / 8 * P->nsb[0];  // superbins per XCD (see k_interp3)
-            hipLaunchKernelGGL((k_interp3<KER, T>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems, nsb,
-                               P->nsb[0]);
+            if (a.st)  // FP32C with the spread set-up's stencils of these positions
+                hipLaunchKernelGGL((k_interp3<KER, T, true>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems,
+                                   nsb, P->nsb[0]);
+            else
+                hipLaunchKernelGGL((k_interp3<KER, T, false>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems,
+                                   nsb, P->nsb[0]);
         }
         else
         {
@@ -2771,6 +2807,7 @@ int ibamr_plan_redistribute(ibamr_plan* P, double* X, const int* lag_index, int 
     const int B = 256;
     P->npoints = -1;  // the binning referred to the old order
     P->xs_valid = false;
+    P->ws_valid = false;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, P->status_d, 8, 0);
     RedistArgs a{};
     a.X = X;

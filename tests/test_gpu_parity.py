@@ -270,7 +270,9 @@ def test_out_of_domain_reported(orc):
 @pytest.mark.parametrize("precision", [ia.FP64, ia.FP32C])
 def test_spread_interp_bitwise_reproducible(precision):
     """No floating-point atomics: two runs of the same spread and interp give
-    identical bits, also after re-binning (a fresh binning gives the same order)."""
+    identical bits, also after re-binning (a fresh binning gives the same order).
+    The interpolation gives the same bits whether it evaluates the stencils
+    itself (X passed) or reads those the spread set-up computed (X = None)."""
     n, npts, kernel = 32, 5000, "IB_6"
     dt = torch.float64 if precision == ia.FP64 else torch.float32
     level = ia.Level.uniform_periodic(n, 3)
@@ -291,7 +293,11 @@ def test_spread_interp_bitwise_reproducible(precision):
         plan.spread(0, F, X, mode=ia.OVERWRITE)
         U = torch.zeros((npts, 3), dtype=dt, device="cuda")
         plan.interp(1, U, X)
+        plan.spread(0, F, X, mode=ia.OVERWRITE)
+        U2 = torch.zeros((npts, 3), dtype=dt, device="cuda")
+        plan.interp(1, U2, None)
         torch.cuda.synchronize()
+        assert torch.equal(U2, U)
         outs.append((f, U))
     for a in range(3):
         assert torch.equal(outs[0][0][a], outs[1][0][a]), a
