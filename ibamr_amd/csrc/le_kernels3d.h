@@ -144,20 +144,16 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     const int nx = s_ln[0], ny = s_ln[1], nz = s_ln[2];
     const int ncomb = nx * ny * nz;
 
-    // registers: tile values (and the FP32 partial / compensation)
-    T acc[ZC];
+    // registers: tile values, accumulated in FP64 (FP32C: plus the chunk's
+    // FP32 partial sums)
+    double acc[ZC];
     T part[FP64 ? 1 : ZC];
-    T comp[FP64 ? 1 : ZC];
     const bool overwrite = a.mode == IBAMR_SPREAD_OVERWRITE;
 #pragma unroll
     for (int z = 0; z < ZC; ++z)
     {
-        acc[z] = T(0);
-        if (!FP64)
-        {
-            part[z] = T(0);
-            comp[z] = T(0);
-        }
+        acc[z] = 0.0;
+        if (!FP64) part[z] = T(0);
     }
 
     // total number of points in the visited bins; an empty tile keeps its values.
@@ -183,7 +179,7 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     {
 #pragma unroll
         for (int z = 0; z < ZC; ++z)
-            if (flo[2] + z <= fhi[2]) acc[z] = arr[col_off + plane * (size_t)(flo[2] + z - alo[2])];
+            if (flo[2] + z <= fhi[2]) acc[z] = (double)arr[col_off + plane * (size_t)(flo[2] + z - alo[2])];
     }
 
     const T inv_vol = T(1.0 / a.vol);
@@ -450,14 +446,11 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                 }
                 if (!FP64)
                 {
-                    // fold the chunk's partial sums into the compensated totals
+                    // fold the chunk's partial sums into the FP64 totals
 #pragma unroll
                     for (int z = 0; z < ZC; ++z)
                     {
-                        const T yv = part[z] - comp[z];
-                        const T t = acc[z] + yv;
-                        comp[z] = (t - acc[z]) - yv;
-                        acc[z] = t;
+                        acc[z] = acc[z] + (double)part[z];
                         part[z] = T(0);
                     }
                 }
@@ -472,9 +465,7 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
         for (int z = 0; z < ZC; ++z)
             if (flo[2] + z <= fhi[2])
             {
-                T v = acc[z];
-                if (!FP64) v = v - comp[z];
-                arr[col_off + plane * (size_t)(flo[2] + z - alo[2])] = v;
+                arr[col_off + plane * (size_t)(flo[2] + z - alo[2])] = (T)acc[z];
             }
     }
 }
