@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -2199,52 +2200,46 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     bps[P->nmem + 1] = (int)bpl.size();
     if (bpl.empty()) bpl.push_back(0);
 
-    // work list: every tile of every array of every patch
+    // work list: every tile of every array of every patch.  XCD-aware launch
+    // order: workgroup i runs on XCD i % 8.  Tiles are grouped by (patch, z row,
+    // y row), each group holding that row of every array, and the groups are
+    // dealt to the XCDs in turn; each XCD runs its groups in (patch, z, y) order.
+    // A group's tiles read the same point data through one L2 (all arrays, x
+    // neighbours), every XCD gets a share of every region (point clouds are
+    // rarely uniform, so contiguous eighths of the domain would leave some XCDs
+    // idle), and all XCDs sweep the domain together, so the data they re-read is
+    // recent in the shared last-level cache.
     std::vector<TileDesc> tiles;
-    const int ncomp_tiles = C->centering == IBAMR_SIDE ? ndim : C->depth;
-    for (int i = 0; i < L->npatches; ++i)
     {
-        const PatchDev& pd = P->patches_h[i];
-        for (int c = 0; c < ncomp_tiles; ++c)
-        {
-            const int axis = C->centering == IBAMR_SIDE ? c : -1;
-            int nt[3] = {1, 1, 1};
-            for (int d = 0; d < ndim; ++d)
-            {
-                const int n = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
-                nt[d] = (n + P->tsz[d] - 1) / P->tsz[d];
-            }
-            for (int tz = 0; tz < nt[2]; ++tz)
-                for (int ty = 0; ty < nt[1]; ++ty)
-                    for (int tx = 0; tx < nt[0]; ++tx) tiles.push_back(TileDesc{i, c, {tx, ty, tz}});
-        }
-    }
-    // XCD-aware launch order: workgroup i runs on XCD i % 8.  The rows of tiles
-    // along x (one (patch, array, z, y) each) are dealt to the XCDs in turn, and
-    // each XCD runs its rows in the natural (patch, array, z, y) order: a row's
-    // tiles share their x halo in one L2, every XCD gets a share of every region
-    // (point clouds are rarely uniform, so contiguous thirds or eighths of the
-    // domain would leave some XCDs idle), and all XCDs sweep the domain together,
-    // so the point data they re-read is recent in the shared last-level cache.
-    {
+        const int ncomp_tiles = C->centering == IBAMR_SIDE ? ndim : C->depth;
         std::vector<std::vector<TileDesc>> q(8);
         size_t g = 0;
-        for (size_t i = 0; i < tiles.size(); ++i)
+        for (int i = 0; i < L->npatches; ++i)
         {
-            const TileDesc& t = tiles[i];
-            if (i > 0)
+            const PatchDev& pd = P->patches_h[i];
+            std::vector<std::array<int, 3>> nt(ncomp_tiles, std::array<int, 3>{1, 1, 1});
+            int mt[3] = {1, 1, 1};
+            for (int c = 0; c < ncomp_tiles; ++c)
             {
-                const TileDesc& u = tiles[i - 1];
-                if (t.patch != u.patch || t.comp != u.comp || t.t[1] != u.t[1] || t.t[2] != u.t[2]) ++g;
+                const int axis = C->centering == IBAMR_SIDE ? c : -1;
+                for (int d = 0; d < ndim; ++d)
+                {
+                    const int n = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
+                    nt[c][d] = (n + P->tsz[d] - 1) / P->tsz[d];
+                    mt[d] = std::max(mt[d], nt[c][d]);
+                }
             }
-            q[g % 8].push_back(t);
+            for (int tz = 0; tz < mt[2]; ++tz)
+                for (int ty = 0; ty < mt[1]; ++ty, ++g)
+                    for (int c = 0; c < ncomp_tiles; ++c)
+                        if (ty < nt[c][1] && tz < nt[c][2])
+                            for (int tx = 0; tx < nt[c][0]; ++tx) q[g % 8].push_back(TileDesc{i, c, {tx, ty, tz}});
         }
         size_t per = 0;
         for (const auto& v : q) per = std::max(per, v.size());
-        std::vector<TileDesc> ord(8 * per, TileDesc{-1, 0, {0, 0, 0}});
+        tiles.assign(8 * per, TileDesc{-1, 0, {0, 0, 0}});
         for (int x = 0; x < 8; ++x)
-            for (size_t i = 0; i < q[x].size(); ++i) ord[8 * i + x] = q[x][i];
-        tiles.swap(ord);
+            for (size_t k = 0; k < q[x].size(); ++k) tiles[8 * k + x] = q[x][k];
     }
     P->ntiles = (int)tiles.size();
 
