@@ -514,20 +514,45 @@ __global__ void k_scatter_points(const int* __restrict__ inv, int n, int ndim, c
 // data, LEInteractor.cpp:3689) the first stencil index in the level's index
 // space and the W weights, in the level's frame.  The spread's tiles then stage
 // a candidate with loads instead of recomputing its weights for every tile it
-// reaches; a patch or periodic image shifts the start by whole cells.
+// reaches; a patch or periodic image shifts the start by whole cells.  With
+// perm, the kernel also performs the bin-order gather of X and F (side data:
+// records of 3 positions and 3 force components), so Xs is not read back.
 template <int KER>
-__global__ void k_point_weights(int n, const double* __restrict__ Xs, int nvar, double3 xlo, double3 dx, int3 lo,
-                                float* __restrict__ wt, int* __restrict__ st)
+__global__ void k_point_weights(int n, double* __restrict__ Xs, int nvar, double3 xlo, double3 dx, int3 lo,
+                                float* __restrict__ wt, int* __restrict__ st, const int* __restrict__ perm,
+                                const double* __restrict__ Xin, const float* __restrict__ Fin,
+                                float* __restrict__ Fs)
 {
     constexpr int W = KernelTraits<KER>::W;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double xl[3] = {xlo.x, xlo.y, xlo.z}, h[3] = {dx.x, dx.y, dx.z};
     const int il[3] = {lo.x, lo.y, lo.z};
+    double x[3];
+    if (perm)
+    {
+        const size_t p = (size_t)perm[k];
+        float f[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            x[d] = Xin[3 * p + d];
+            f[d] = Fin[3 * p + d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            Xs[3 * (size_t)k + d] = x[d];
+            Fs[3 * (size_t)k + d] = f[d];
+        }
+    }
+    else
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[d] = Xs[3 * (size_t)k + d];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        const double X = Xs[3 * (size_t)k + d];
+        const double X = x[d];
         for (int v = 0; v < nvar; ++v)
         {
             float w[W];
@@ -542,15 +567,17 @@ __global__ void k_point_weights(int n, const double* __restrict__ Xs, int nvar, 
 
 // FP32C 3-D: stencil starts and weights of the bin-ordered positions, used by
 // the spread and the interpolation of the same positions
+// (with X and F: the gather into bin order of side data's X and F as well)
 template <int KER>
-int launch_point_weights(ibamr_plan* P, hipStream_t s)
+int launch_point_weights(ibamr_plan* P, hipStream_t s, const double* X = nullptr, const float* F = nullptr)
 {
     if (!P->wt_d || !P->st_d) return fail(IBAMR_ERR_INVALID, "internal: no weight buffers");
     const int B = 256;
     hipLaunchKernelGGL((k_point_weights<KER>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->npoints, P->Xs_d,
                        P->nvar, make_double3(P->dom_xlo[0], P->dom_xlo[1], P->dom_xlo[2]),
                        make_double3(P->dx[0], P->dx[1], P->dx[2]),
-                       make_int3(P->dom_lo[0], P->dom_lo[1], P->dom_lo[2]), P->wt_d, P->st_d);
+                       make_int3(P->dom_lo[0], P->dom_lo[1], P->dom_lo[2]), P->wt_d, P->st_d,
+                       X ? P->perm_d : nullptr, X, F, static_cast<float*>(P->Fs_d));
     HIP_OK(hipGetLastError());
     return IBAMR_OK;
 }
@@ -2558,22 +2585,32 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         if (P->npoints > 0)
         {
             const int B = 256;
-            if (P->fdepth == NDIM)
-                hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
-                                   P->perm_d, P->npoints, X, P->Xs_d, static_cast<const T*>(F),
-                                   static_cast<T*>(P->Fs_d));
-            else
-                hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
-                                   P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
-                                   static_cast<T*>(P->Fs_d));
-            HIP_OK(hipGetLastError());
-            P->xs_valid = true;
-            if (NDIM == 3 && sizeof(T) == 4)
+            constexpr bool WTS = NDIM == 3 && sizeof(T) == 4;  // FP32C 3-D: stencils set up here
+            if (WTS && P->fdepth == NDIM)
             {
-                const int rc = launch_point_weights<KER>(P, s);
+                // gather and stencils in one pass
+                const int rc = launch_point_weights<KER>(P, s, X, reinterpret_cast<const float*>(F));
                 if (rc != IBAMR_OK) return rc;
-                P->ws_valid = true;
             }
+            else
+            {
+                if (P->fdepth == NDIM)
+                    hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
+                                       P->perm_d, P->npoints, X, P->Xs_d, static_cast<const T*>(F),
+                                       static_cast<T*>(P->Fs_d));
+                else
+                    hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
+                                       P->idx_d, P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
+                                       static_cast<T*>(P->Fs_d));
+                HIP_OK(hipGetLastError());
+                if (WTS)
+                {
+                    const int rc = launch_point_weights<KER>(P, s);
+                    if (rc != IBAMR_OK) return rc;
+                }
+            }
+            P->xs_valid = true;
+            P->ws_valid = WTS;
         }
         a.X = P->Xs_d;
         a.F = static_cast<const T*>(P->Fs_d);
