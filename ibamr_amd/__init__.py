@@ -42,7 +42,8 @@ C_API = [
     "ibamr_plan_create", "ibamr_plan_create_patch", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
-    "ibamr_plan_info", "ibamr_plan_redistribute", "ibamr_permute_rows",
+    "ibamr_plan_info", "ibamr_plan_set_tile_height", "ibamr_plan_get_tile_height", "ibamr_plan_autotune",
+    "ibamr_plan_redistribute", "ibamr_permute_rows",
     "ibamr_plan_set_physical_boundary", "ibamr_plan_accumulate_physical_boundary", "ibamr_plan_fill_physical_boundary", "ibamr_fe_quadrature_points",
     "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
@@ -132,6 +133,10 @@ def lib():
     L.ibamr_fe_mesh_destroy.argtypes = [vp]
     L.ibamr_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
+    L.ibamr_plan_set_tile_height.argtypes = [vp, ctypes.c_int]
+    L.ibamr_plan_get_tile_height.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.ibamr_plan_autotune.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), vp]
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
     L.ibamr_s0_interp.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, dp, ctypes.c_int, ip, ip, ip, dp, ip, dp,
                                   ctypes.c_int, dp, dp]
@@ -293,6 +298,27 @@ class LevelPlan:
         nt, ts, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib().ibamr_plan_info(self._h, ctypes.byref(nt), ctypes.byref(ts), ctypes.byref(nb)))
         return {"ntiles": nt.value, "tile": ts.value, "nbins": nb.value}
+
+    @property
+    def tile_height(self):
+        """z values per spread tile (3-D; the tile is 8 x 8 grid lines of one array)."""
+        h = ctypes.c_int()
+        _check(lib().ibamr_plan_get_tile_height(self._h, ctypes.byref(h)))
+        return h.value
+
+    @tile_height.setter
+    def tile_height(self, h):
+        _check(lib().ibamr_plan_set_tile_height(self._h, int(h)))
+
+    def autotune(self, slot, F, X, reps=3, stream=None):
+        """Time the spread of F at X (binned) into the arrays bound to `slot`
+        (overwritten) with every built tile height and keep the fastest.
+        Returns {height: ms per spread call} (-1: height not admissible)."""
+        n = 8  # IBAMR_MAX_TILE_HEIGHTS
+        hs, ms, nc = (ctypes.c_int * n)(), (ctypes.c_float * n)(), ctypes.c_int()
+        _check(lib().ibamr_plan_autotune(self._h, slot, _ptr(F), _ptr(X), reps, hs, ms, ctypes.byref(nc),
+                                         _stream_ptr(stream)))
+        return {hs[i]: ms[i] for i in range(nc.value)}
 
     def bind(self, slot, arrays):
         """arrays: flat list [patch0 axis0, patch0 axis1, ..., patch1 axis0, ...] of device tensors."""

@@ -3,7 +3,7 @@
 //
 // Spread (k_spread3): owner-computes, register-resident.  One wavefront owns a
 // column tile of one array: 8 x 8 (x, y) grid lines, one per lane, times
-// s3_zc<T>() consecutive z values held in registers.  Points that can reach the
+// ZC consecutive z values held in registers.  Points that can reach the
 // tile are staged 64 at a time: each lane computes one point's weights (FP64
 // index selection, weights in T) and writes the x and y weights, aligned to
 // the tile's lanes and zero-padded, plus the z weights to LDS.  The staged
@@ -28,14 +28,13 @@
 
 namespace
 {
-// z values per spread tile (registers per lane): FP32C keeps three register
-// sets per z (total, partial, compensation), so its column is shorter, which
-// leaves room for more waves per SIMD
-#ifndef IBAMR_S3_ZC_FP32
-#define IBAMR_S3_ZC_FP32 16
-#endif
+// z values per spread tile (registers per lane), the default of each form:
+// FP32C keeps two register sets per z (the FP64 total, two registers, and the
+// chunk's FP32 partial), so its column is shorter, which leaves room for more
+// waves per SIMD.  The autotuner (ibamr_plan_autotune) picks among the heights
+// instantiated for each form (s3_zc_ok).
 constexpr int S3_ZC_FP64 = 32;
-constexpr int S3_ZC_FP32 = IBAMR_S3_ZC_FP32;
+constexpr int S3_ZC_FP32 = 16;
 template <typename T>
 constexpr int s3_zc()
 {
@@ -44,13 +43,12 @@ constexpr int s3_zc()
 constexpr int S3_PAD = 9;    // staged weight row: the 8 tile lanes (odd stride: no bank conflicts)
 constexpr int S3_MAXCOMB = 64;
 
-template <int KER, typename T, bool MERGE>
+template <int KER, typename T, bool MERGE, int ZC>
 __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
 {
     using KT = KernelTraits<KER>;
     constexpr int W = KT::W;
     constexpr bool FP64 = sizeof(T) == 8;
-    constexpr int ZC = s3_zc<T>();
     __shared__ T s_rx[64 * S3_PAD];
     __shared__ T s_ry[64 * S3_PAD];
     __shared__ T s_wz[64 * 8];

@@ -1307,6 +1307,67 @@ void dfree(void* p)
 {
     if (p) (void)hipFree(p);
 }
+
+// The spread's work list for the plan's tile shape, uploaded to tiles_d
+// (replacing any previous list).
+int build_tiles(ibamr_plan* P)
+{
+    // work list: every tile of every array of every patch.  XCD-aware launch
+    // order: workgroup i runs on XCD i % 8.  Tiles are grouped by (patch, z row,
+    // y row), each group holding that row of every array, and the groups are
+    // dealt to the XCDs in turn; each XCD runs its groups in (patch, z, y) order.
+    // A group's tiles read the same point data through one L2 (all arrays, x
+    // neighbours), every XCD gets a share of every region (point clouds are
+    // rarely uniform, so contiguous eighths of the domain would leave some XCDs
+    // idle), and all XCDs sweep the domain together, so the data they re-read is
+    // recent in the shared last-level cache.
+    std::vector<TileDesc> tiles;
+    {
+        const int ncomp_tiles = P->centering == IBAMR_SIDE ? P->ndim : P->depth;
+        std::vector<std::vector<TileDesc>> q(8);
+        size_t g = 0;
+        for (int i = 0; i < P->npatches; ++i)
+        {
+            const PatchDev& pd = P->patches_h[i];
+            std::vector<std::array<int, 3>> nt(ncomp_tiles, std::array<int, 3>{1, 1, 1});
+            int mt[3] = {1, 1, 1};
+            for (int c = 0; c < ncomp_tiles; ++c)
+            {
+                const int axis = P->centering == IBAMR_SIDE ? c : -1;
+                for (int d = 0; d < P->ndim; ++d)
+                {
+                    const int n = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
+                    nt[c][d] = (n + P->tsz[d] - 1) / P->tsz[d];
+                    mt[d] = std::max(mt[d], nt[c][d]);
+                }
+            }
+            for (int tz = 0; tz < mt[2]; ++tz)
+                for (int ty = 0; ty < mt[1]; ++ty, ++g)
+                    for (int c = 0; c < ncomp_tiles; ++c)
+                        if (ty < nt[c][1] && tz < nt[c][2])
+                            for (int tx = 0; tx < nt[c][0]; ++tx) q[g % 8].push_back(TileDesc{i, c, {tx, ty, tz}});
+        }
+        size_t per = 0;
+        for (const auto& v : q) per = std::max(per, v.size());
+        tiles.assign(8 * per, TileDesc{-1, 0, {0, 0, 0}});
+        for (int x = 0; x < 8; ++x)
+            for (size_t k = 0; k < q[x].size(); ++k) tiles[8 * k + x] = q[x][k];
+    }
+    dfree(P->tiles_d);
+    P->tiles_d = nullptr;
+    P->ntiles = (int)tiles.size();
+    HIP_OK(hipMalloc(&P->tiles_d, sizeof(TileDesc) * std::max<size_t>(1, tiles.size())));
+    if (!tiles.empty()) HIP_OK(hipMemcpy(P->tiles_d, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
+    return IBAMR_OK;
+}
+
+// Spread tile heights instantiated for each form (3-D); the first is the default.
+const std::vector<int>& tile_heights(int precision)
+{
+    static const std::vector<int> fp64 = {32, 16}, fp32 = {16, 8, 24};
+    return precision == IBAMR_FP64 ? fp64 : fp32;
+}
+
 } // namespace
 
 // ----------------------------------------------------------------------------- redistribution
@@ -2227,48 +2288,6 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     bps[P->nmem + 1] = (int)bpl.size();
     if (bpl.empty()) bpl.push_back(0);
 
-    // work list: every tile of every array of every patch.  XCD-aware launch
-    // order: workgroup i runs on XCD i % 8.  Tiles are grouped by (patch, z row,
-    // y row), each group holding that row of every array, and the groups are
-    // dealt to the XCDs in turn; each XCD runs its groups in (patch, z, y) order.
-    // A group's tiles read the same point data through one L2 (all arrays, x
-    // neighbours), every XCD gets a share of every region (point clouds are
-    // rarely uniform, so contiguous eighths of the domain would leave some XCDs
-    // idle), and all XCDs sweep the domain together, so the data they re-read is
-    // recent in the shared last-level cache.
-    std::vector<TileDesc> tiles;
-    {
-        const int ncomp_tiles = C->centering == IBAMR_SIDE ? ndim : C->depth;
-        std::vector<std::vector<TileDesc>> q(8);
-        size_t g = 0;
-        for (int i = 0; i < L->npatches; ++i)
-        {
-            const PatchDev& pd = P->patches_h[i];
-            std::vector<std::array<int, 3>> nt(ncomp_tiles, std::array<int, 3>{1, 1, 1});
-            int mt[3] = {1, 1, 1};
-            for (int c = 0; c < ncomp_tiles; ++c)
-            {
-                const int axis = C->centering == IBAMR_SIDE ? c : -1;
-                for (int d = 0; d < ndim; ++d)
-                {
-                    const int n = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
-                    nt[c][d] = (n + P->tsz[d] - 1) / P->tsz[d];
-                    mt[d] = std::max(mt[d], nt[c][d]);
-                }
-            }
-            for (int tz = 0; tz < mt[2]; ++tz)
-                for (int ty = 0; ty < mt[1]; ++ty, ++g)
-                    for (int c = 0; c < ncomp_tiles; ++c)
-                        if (ty < nt[c][1] && tz < nt[c][2])
-                            for (int tx = 0; tx < nt[c][0]; ++tx) q[g % 8].push_back(TileDesc{i, c, {tx, ty, tz}});
-        }
-        size_t per = 0;
-        for (const auto& v : q) per = std::max(per, v.size());
-        tiles.assign(8 * per, TileDesc{-1, 0, {0, 0, 0}});
-        for (int x = 0; x < 8; ++x)
-            for (size_t k = 0; k < q[x].size(); ++k) tiles[8 * k + x] = q[x][k];
-    }
-    P->ntiles = (int)tiles.size();
 
     // segments of the same-level ghost fill: every element of every patch array
     std::vector<int4> fsegs;
@@ -2305,9 +2324,10 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
 
     PLAN_HIP(hipMalloc(&P->patches_d, sizeof(PatchDev) * P->patches_h.size()));
     PLAN_HIP(hipMemcpy(P->patches_d, P->patches_h.data(), sizeof(PatchDev) * P->patches_h.size(), hipMemcpyHostToDevice));
-    PLAN_HIP(hipMalloc(&P->tiles_d, sizeof(TileDesc) * std::max<size_t>(1, tiles.size())));
-    if (!tiles.empty())
-        PLAN_HIP(hipMemcpy(P->tiles_d, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
+    {
+        const int rc = build_tiles(P);
+        if (rc != IBAMR_OK) return cleanup_fail(rc);
+    }
     PLAN_HIP(hipMalloc(&P->bin_patch_start_d, sizeof(int) * bps.size()));
     PLAN_HIP(hipMemcpy(P->bin_patch_start_d, bps.data(), sizeof(int) * bps.size(), hipMemcpyHostToDevice));
     PLAN_HIP(hipMalloc(&P->fill_segs_d, sizeof(int4) * std::max<size_t>(1, fsegs.size())));
@@ -2622,7 +2642,15 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         if (NDIM == 3)
         {
             // FP32C packs chunks across bins; FP64 cuts them per bin (the oracle's fixed order)
-            hipLaunchKernelGGL((k_spread3<KER, T, sizeof(T) == 4>), dim3(P->ntiles), dim3(64), 0, s, a);
+            constexpr bool MERGE = sizeof(T) == 4;
+            switch (P->tsz[2])
+            {
+            case 8: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 8>), dim3(P->ntiles), dim3(64), 0, s, a); break;
+            case 16: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 16>), dim3(P->ntiles), dim3(64), 0, s, a); break;
+            case 24: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 24>), dim3(P->ntiles), dim3(64), 0, s, a); break;
+            case 32: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 32>), dim3(P->ntiles), dim3(64), 0, s, a); break;
+            default: return fail(IBAMR_ERR_INVALID, "internal: tile height without a kernel");
+            }
         }
         else
             hipLaunchKernelGGL((k_spread_tiles<2, KER, T, 16>), dim3(P->ntiles), dim3(WAVE), 0, s, a);
@@ -2630,6 +2658,99 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         if (t_end) HIP_OK(hipEventRecord(t_end, s));
         return IBAMR_OK;
     });
+}
+
+int ibamr_plan_set_tile_height(ibamr_plan* P, int height)
+{
+    if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
+    if (P->ndim != 3) return height == P->tsz[2] ? IBAMR_OK : fail(IBAMR_ERR_INVALID, "2-D plans have one tile shape");
+    const std::vector<int>& hs = tile_heights(P->precision);
+    if (std::find(hs.begin(), hs.end(), height) == hs.end())
+        return fail(IBAMR_ERR_INVALID, "tile height " + std::to_string(height) + " not built for this precision");
+    if ((height + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16)
+        return fail(IBAMR_ERR_INVALID, "drift_cells too large for this tile height");
+    if (height == P->tsz[2]) return IBAMR_OK;
+    const int old = P->tsz[2];
+    P->tsz[2] = height;
+    const int rc = build_tiles(P);
+    if (rc != IBAMR_OK)
+    {
+        const std::string m = g_err;
+        P->tsz[2] = old;
+        (void)build_tiles(P);
+        g_err = m;
+        return rc;
+    }
+    return IBAMR_OK;
+}
+
+int ibamr_plan_get_tile_height(const ibamr_plan* P, int* height)
+{
+    if (!P || !height) return fail(IBAMR_ERR_INVALID, "null argument");
+    *height = P->tsz[2];
+    return IBAMR_OK;
+}
+
+int ibamr_plan_autotune(ibamr_plan* P, int slot, const void* F, const double* X, int reps, int* heights, float* ms,
+                        int* ncand, void* stream)
+{
+    if (!P || !heights || !ms || !ncand) return fail(IBAMR_ERR_INVALID, "null argument");
+    if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
+    *ncand = 0;
+    if (P->ndim != 3)
+    {
+        heights[0] = P->tsz[2];
+        ms[0] = 0.0f;
+        *ncand = 1;
+        return IBAMR_OK;
+    }
+    reps = std::max(1, reps);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const std::vector<int> hs = tile_heights(P->precision);
+    if ((int)hs.size() > IBAMR_MAX_TILE_HEIGHTS) return fail(IBAMR_ERR_INVALID, "internal: too many tile heights");
+    const bool timing = P->timing;
+    P->timing = false;  // the trials are not the caller's launches
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = IBAMR_OK, best = P->tsz[2];
+    float best_ms = std::numeric_limits<float>::infinity();
+    auto run = [&]() -> int {
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        for (size_t i = 0; i < hs.size(); ++i)
+        {
+            heights[i] = hs[i];
+            ms[i] = -1.0f;  // not admissible (drift too large for this height)
+            if ((hs[i] + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16) continue;
+            int r = ibamr_plan_set_tile_height(P, hs[i]);
+            if (r != IBAMR_OK) return r;
+            r = ibamr_plan_spread(P, slot, F, X, IBAMR_SPREAD_OVERWRITE, stream);
+            if (r != IBAMR_OK) return r;
+            HIP_OK(hipEventRecord(e0, s));
+            for (int k = 0; k < reps; ++k)
+            {
+                r = ibamr_plan_spread(P, slot, F, X, IBAMR_SPREAD_OVERWRITE, stream);
+                if (r != IBAMR_OK) return r;
+            }
+            HIP_OK(hipEventRecord(e1, s));
+            HIP_OK(hipEventSynchronize(e1));
+            float t = 0.0f;
+            HIP_OK(hipEventElapsedTime(&t, e0, e1));
+            ms[i] = t / reps;
+            if (ms[i] < best_ms)
+            {
+                best_ms = ms[i];
+                best = hs[i];
+            }
+        }
+        return IBAMR_OK;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    P->timing = timing;
+    if (rc != IBAMR_OK) return rc;
+    *ncand = (int)hs.size();
+    return ibamr_plan_set_tile_height(P, best);
 }
 
 int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* stream)

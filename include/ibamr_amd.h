@@ -325,6 +325,26 @@ int ibamr_plan_kernel_times(ibamr_plan* plan, double* spread_ms, int* nspread, d
 /* Plan facts for reports: number of spread tiles, tile edge, bins. */
 int ibamr_plan_info(const ibamr_plan* plan, int* ntiles, int* tile, int* nbins);
 
+/* Spread tile shape (3-D plans).  A spread tile is 8 x 8 grid lines of one
+ * array times `height` z values held in registers; the heights built for a
+ * form are 32, 16 (FP64) and 16, 8, 24 (FP32C), the first being the default.
+ * FP64 results do not depend on the height (the summation order per value is
+ * fixed by the bins); FP32C results do only by rounding.  No counterpart in the
+ * reference, whose CPU loops have no tiles: these tune this implementation's
+ * work decomposition to the caller's point distribution and hardware.
+ *   set: choose a height (rebuilds the work list; not graph-safe).
+ *   autotune: for each height, one untimed and `reps` timed spreads of F at X
+ *     into the arrays bound to `slot` (IBAMR_SPREAD_OVERWRITE: their values are
+ *     replaced), then keeps the fastest.  heights[] and ms[] (capacity
+ *     IBAMR_MAX_TILE_HEIGHTS) receive each candidate and its mean time per
+ *     spread call, *ncand their number; synchronises the stream.  Points must be
+ *     binned.  2-D plans report their one shape. */
+#define IBAMR_MAX_TILE_HEIGHTS 8
+int ibamr_plan_set_tile_height(ibamr_plan* plan, int height);
+int ibamr_plan_get_tile_height(const ibamr_plan* plan, int* height);
+int ibamr_plan_autotune(ibamr_plan* plan, int slot, const void* F, const double* X, int reps, int* heights,
+                        float* ms, int* ncand, void* stream);
+
 const char* ibamr_last_error(void);
 
 #ifdef __cplusplus
