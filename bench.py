@@ -172,6 +172,8 @@ def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
     plan.check(stream=s)
     torch.cuda.synchronize()
     bin_ms = eb0.elapsed_time(eb1)
+    # spread tile height chosen by the library's autotuner on these points (untimed)
+    tune = plan.autotune(0, Fd, X, reps=3, stream=s)
 
     def step(ev=None):
         if ev:
@@ -197,7 +199,7 @@ def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
     plan.set_timing(False)
     sp = [e[0].elapsed_time(e[1]) for e in evs]
     it = [e[1].elapsed_time(e[2]) for e in evs]
-    return {"plan": plan, "f": f, "u": u, "U": U, "wall_s": wall, "spread_ms": sum(sp) / len(sp),
+    return {"plan": plan, "tune": tune, "f": f, "u": u, "U": U, "wall_s": wall, "spread_ms": sum(sp) / len(sp),
             "interp_ms": sum(it) / len(it), "bin_ms": bin_ms, "info": plan.info(),
             "spread_kernel_ms": kt["spread_ms"], "interp_kernel_ms": kt["interp_ms"]}
 
@@ -254,6 +256,7 @@ def run_amr_form(torch, ia, precision, level, X, F, kernel, steps, warmup, seed=
     Fd = F.to(dt).contiguous()
     U = torch.zeros((n, 3), dtype=dt, device="cuda")
     plan.bin(X, stream=s)
+    tune = plan.autotune(0, Fd, X, reps=3, stream=s)  # spread tile height for these patches and points
     for _ in range(warmup):
         plan.spread(0, Fd, X, mode=ia.OVERWRITE, stream=s)
         plan.interp(1, U, X, stream=s)
@@ -268,7 +271,8 @@ def run_amr_form(torch, ia, precision, level, X, F, kernel, steps, warmup, seed=
     torch.cuda.synchronize()
     kt = plan.kernel_times()
     plan.set_timing(False)
-    return {"f": f, "U": U, "ms": e0.elapsed_time(e1) / steps, "kt": kt, "info": plan.info()}
+    return {"f": f, "U": U, "ms": e0.elapsed_time(e1) / steps, "kt": kt, "info": plan.info(), "tune": tune,
+            "height": plan.tile_height}
 
 
 def main_amr(args, torch, ia):
@@ -298,7 +302,10 @@ def main_amr(args, torch, ia):
                                "%d points on 4 ellipsoidal shells, side-centred BSPLINE_3, FP32 compensated"
                                % (len(level.patches), cells, n),
                    "patches": len(level.patches), "fine_cells": cells, "points": n, "kernel": kernel,
-                   "spread_tiles": r32["info"]["ntiles"]},
+                   "spread_tiles": r32["info"]["ntiles"],
+                   "tile": [r32["info"]["tile"], r32["info"]["tile"], r32["height"]],
+                   "tile_autotune_ms": {str(h): t for h, t in r32["tune"].items()},
+                   "bins": "%d^3 cells, sorted (CSR: no capacity limit)" % r32["info"]["tile"]},
         "detail": {"spread_kernel_ms": r32["kt"]["spread_ms"], "interp_kernel_ms": r32["kt"]["interp_ms"],
                    "fp64_points_per_s": n / (r64["ms"] * 1e-3), "fp64_spread_kernel_ms": r64["kt"]["spread_ms"],
                    "fp64_interp_kernel_ms": r64["kt"]["interp_ms"], "fp32c_over_fp64": r64["ms"] / r32["ms"],
@@ -469,7 +476,10 @@ def main():
         "config": {"workload": "3-D periodic %d^3 side-centred %s, %d points (half uniform, half in a 2h shell "
                                "r=0.3), FP32 compensated spread+interp" % (N, kernel, n),
                    "grid": N, "points": n, "kernel": kernel, "centering": "side", "precision": "fp32c",
-                   "tile": r32["info"]["tile"], "spread_tiles": r32["info"]["ntiles"]},
+                   "tile": [r32["info"]["tile"], r32["info"]["tile"], r32["plan"].tile_height],
+                   "tile_autotune_ms": {str(h): t for h, t in r32["tune"].items()},
+                   "bins": "%d^3 cells, sorted (CSR: no capacity limit)" % r32["info"]["tile"],
+                   "spread_tiles": r32["info"]["ntiles"]},
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tr["bytes_per_dispatch"] if tr else None, "algorithmic_bytes": B_dom,
