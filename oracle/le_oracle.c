@@ -36,6 +36,9 @@ int orc_kernel_from_name(const char* name)
     if (strcmp(name, "IB_4") == 0) return ORC_IB_4;
     if (strcmp(name, "IB_6") == 0) return ORC_IB_6;
     if (strcmp(name, "BSPLINE_3") == 0) return ORC_BSPLINE_3;
+    if (strcmp(name, "BSPLINE_4") == 0) return ORC_BSPLINE_4;
+    if (strcmp(name, "BSPLINE_5") == 0) return ORC_BSPLINE_5;
+    if (strcmp(name, "BSPLINE_6") == 0) return ORC_BSPLINE_6;
     return -1;
 }
 
@@ -48,6 +51,9 @@ int orc_stencil_size(int kernel)
     case ORC_IB_4: return 4;
     case ORC_IB_6: return 6;
     case ORC_BSPLINE_3: return 4;
+    case ORC_BSPLINE_4: return 4;
+    case ORC_BSPLINE_5: return 6;
+    case ORC_BSPLINE_6: return 6;
     default: return -1;
     }
 }
@@ -68,6 +74,9 @@ int orc_kernel_width(int kernel)
     case ORC_IB_4: return 4;
     case ORC_IB_6: return 6;
     case ORC_BSPLINE_3: return 3;
+    case ORC_BSPLINE_4: return 4;
+    case ORC_BSPLINE_5: return 5;
+    case ORC_BSPLINE_6: return 6;
     default: return -1;
     }
 }
@@ -102,6 +111,63 @@ double orc_bspline_3_delta(double x)
     if (modx <= 0.5) return 0.5 * (-2.0 * r2 + 6.0 * r - 3.0);
     if (modx <= 1.5) return 0.5 * (r2 - 6.0 * r + 9.0);
     return 0.0;
+}
+
+/* lagrangian_delta.f.m4:268-288 */
+double orc_bspline_4_delta(double x)
+{
+    const double modx = fabs(x);
+    const double r = modx + 2.0;
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    if (modx <= 1.0) return (1.0 / 6.0) * (((3.0 * r3 - 24.0 * r2) + 60.0 * r) - 44.0);
+    if (modx <= 2.0) return (1.0 / 6.0) * (((-r3 + 12.0 * r2) - 48.0 * r) + 64.0);
+    return 0.0;
+}
+
+/* lagrangian_delta.f.m4:296-320 */
+double orc_bspline_5_delta(double x)
+{
+    const double modx = fabs(x);
+    const double r = modx + 2.5;
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    const double r4 = r3 * r;
+    if (modx <= 0.5) return (1.0 / 24.0) * ((((6.0 * r4 - 60.0 * r3) + 210.0 * r2) - 300.0 * r) + 155.0);
+    if (modx <= 1.5) return (1.0 / 24.0) * ((((-4.0 * r4 + 60.0 * r3) - 330.0 * r2) + 780.0 * r) - 655.0);
+    if (modx <= 2.5) return (1.0 / 24.0) * ((((r4 - 20.0 * r3) + 150.0 * r2) - 500.0 * r) + 625.0);
+    return 0.0;
+}
+
+/* lagrangian_delta.f.m4:328-354 */
+double orc_bspline_6_delta(double x)
+{
+    const double modx = fabs(x);
+    const double r = modx + 3.0;
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    const double r4 = r3 * r;
+    const double r5 = r4 * r;
+    if (modx <= 1.0)
+        return (1.0 / 60.0) * (((((2193.0 - 3465.0 * r) + 2130.0 * r2) - 630.0 * r3) + 90.0 * r4) - 5.0 * r5);
+    if (modx <= 2.0)
+        return (1.0 / 120.0) * (((((-10974.0 + 12270.0 * r) - 5340.0 * r2) + 1140.0 * r3) - 120.0 * r4) + 5.0 * r5);
+    if (modx <= 3.0)
+        return (1.0 / 120.0) * (((((7776.0 - 6480.0 * r) + 2160.0 * r2) - 360.0 * r3) + 30.0 * r4) - r5);
+    return 0.0;
+}
+
+double orc_delta(int kernel, double r)
+{
+    switch (kernel)
+    {
+    case ORC_IB_4: return orc_ib_4_delta(r);
+    case ORC_BSPLINE_3: return orc_bspline_3_delta(r);
+    case ORC_BSPLINE_4: return orc_bspline_4_delta(r);
+    case ORC_BSPLINE_5: return orc_bspline_5_delta(r);
+    case ORC_BSPLINE_6: return orc_bspline_6_delta(r);
+    default: return 0.0;
+    }
 }
 
 /* ------------------------------------------------------------- 1-D weights */
@@ -147,6 +213,11 @@ static void ib6_weights(double r, double* w)
  * already-shifted coordinate X(d,s)+Xshift(d,l). */
 int orc_weights_1d(int kernel, double Xs, double x_lower, double dx, double* w)
 {
+    return orc_weights_1d_u(kernel, Xs, Xs, x_lower, dx, w);
+}
+
+int orc_weights_1d_u(int kernel, double Xs, double Xu, double x_lower, double dx, double* w)
+{
     switch (kernel)
     {
     case ORC_IB_4:
@@ -186,6 +257,28 @@ int orc_weights_1d(int kernel, double Xs, double x_lower, double dx, double* w)
         }
         return lo;
     }
+    case ORC_BSPLINE_4:
+    case ORC_BSPLINE_5:
+    case ORC_BSPLINE_6:
+    {
+        /* lagrangian_interaction3d.f.m4:2882-2908 (BSPLINE_4), :3230-3248
+         * (BSPLINE_5: centre -2..+2), :3460-3480 (BSPLINE_6); the even orders
+         * pick the side by the unshifted X against the cell centre */
+        const int W = orc_kernel_width(kernel);
+        const int c = (int)floor((Xs - x_lower) / dx);
+        const double X_c = x_lower + ((double)c + 0.5) * dx;
+        int lo;
+        if (kernel == ORC_BSPLINE_5)
+            lo = c - 2;
+        else
+            lo = (Xu < X_c) ? c - W / 2 : c - W / 2 + 1;
+        for (int k = 0; k < W; ++k)
+        {
+            const double X_cell = x_lower + ((double)(lo + k) + 0.5) * dx;
+            w[k] = orc_delta(kernel, (Xs - X_cell) / dx);
+        }
+        return lo;
+    }
     case ORC_PIECEWISE_LINEAR:
     {
         /* lagrangian_interaction3d.f.m4:680-699 */
@@ -217,13 +310,13 @@ typedef struct
     double w[3][8];
 } stencil_t;
 
-static void make_stencil(int kernel, int ndim, const double* Xs, const double* dx, const double* x_lower,
-                         const int* ilower, const int* iupper, const int* nugc, stencil_t* st)
+static void make_stencil(int kernel, int ndim, const double* Xs, const double* Xu, const double* dx,
+                         const double* x_lower, const int* ilower, const int* iupper, const int* nugc, stencil_t* st)
 {
     const int W = orc_kernel_width(kernel);
     for (int d = 0; d < ndim; ++d)
     {
-        const int rel = orc_weights_1d(kernel, Xs[d], x_lower[d], dx[d], st->w[d]);
+        const int rel = orc_weights_1d_u(kernel, Xs[d], Xu[d], x_lower[d], dx[d], st->w[d]);
         const int b = ilower[d] + rel;
         const int glo = ilower[d] - nugc[d], ghi = iupper[d] + nugc[d];
         st->base[d] = b;
@@ -246,7 +339,7 @@ void orc_interp(int kernel, int ndim, const double* dx, const double* x_lower, i
         double Xs[3];
         for (int d = 0; d < ndim; ++d) Xs[d] = X[ndim * s + d] + Xshift[ndim * l + d];
         stencil_t st;
-        make_stencil(kernel, ndim, Xs, dx, x_lower, ilower, iupper, nugc, &st);
+        make_stencil(kernel, ndim, Xs, X + ndim * s, dx, x_lower, ilower, iupper, nugc, &st);
         for (int c = 0; c < depth; ++c)
         {
             const double* uc = u + c * ncomp;
@@ -305,7 +398,7 @@ void orc_spread(int kernel, int ndim, const double* dx, const double* x_lower, i
         double Xs[3];
         for (int d = 0; d < ndim; ++d) Xs[d] = X[ndim * s + d] + Xshift[ndim * l + d];
         stencil_t st;
-        make_stencil(kernel, ndim, Xs, dx, x_lower, ilower, iupper, nugc, &st);
+        make_stencil(kernel, ndim, Xs, X + ndim * s, dx, x_lower, ilower, iupper, nugc, &st);
         for (int c = 0; c < depth; ++c)
         {
             double* uc = u + c * ncomp;
@@ -618,7 +711,7 @@ void orc_order_entries_z(int kernel, const int* cell, const int* indices, const 
         const double Xs = X[3 * p + 2] + Xshift[3 * l + 2];
         z[l].key[0] = group;
         z[l].key[1] = rank / chunk;
-        z[l].key[2] = orc_weights_1d(kernel, Xs, x_lower[2], dx[2], w);
+        z[l].key[2] = orc_weights_1d_u(kernel, Xs, X[3 * p + 2], x_lower[2], dx[2], w);
         z[l].key[3] = p;
         z[l].pos = l;
     }
@@ -647,7 +740,7 @@ double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const do
             for (int o = -nimg; o <= nimg; ++o)
             {
                 const double r = (xg[d] - (X[ndim * p + d] + o * period[d])) / dx[d];
-                wsum += kernel == ORC_IB_4 ? orc_ib_4_delta(r) : orc_bspline_3_delta(r);
+                wsum += orc_delta(kernel, r);
             }
             wprod *= wsum;
         }

@@ -38,24 +38,37 @@ enum orc_kernel
     ORC_PIECEWISE_LINEAR = 0,
     ORC_IB_4 = 1,
     ORC_IB_6 = 2,
-    ORC_BSPLINE_3 = 3
+    ORC_BSPLINE_3 = 3,
+    ORC_BSPLINE_4 = 4,
+    ORC_BSPLINE_5 = 5,
+    ORC_BSPLINE_6 = 6
 };
 
 /* Registry (LEInteractor.cpp:2035-2114). Returns -1 for unknown names. */
 int orc_kernel_from_name(const char* name);
-int orc_stencil_size(int kernel);      /* getStencilSize: PWL 2, IB_4 4, IB_6 6, BSPLINE_3 4 */
+int orc_stencil_size(int kernel);      /* getStencilSize: PWL 2, IB_4 4, IB_6 6, BSPLINE_3 4, _4 4, _5 6, _6 6 */
 int orc_min_ghost_width(int kernel);   /* floor(stencil/2)+1 */
 
 /* Scalar delta functions (lagrangian_delta.f.m4:174-196 and :243-260). */
 double orc_ib_4_delta(double r);
+/* lagrangian_delta.f.m4:268-354 (BSPLINE_4, _5, _6) and any kernel's scalar delta. */
+double orc_bspline_4_delta(double x);
+double orc_bspline_5_delta(double x);
+double orc_bspline_6_delta(double x);
+double orc_delta(int kernel, double r);
 double orc_bspline_3_delta(double x);
 
 /* 1-D weights of one coordinate exactly as the kernel computes them.
  * X_o_dx_num is (X+Xshift)-x_lower; returns the first stencil index relative
  * to ilower (unclipped) and writes the stencil weights into w[] (length
  * orc_kernel_width(kernel)). */
-int orc_kernel_width(int kernel);      /* PWL 2, IB_4 4, IB_6 6, BSPLINE_3 3 */
+int orc_kernel_width(int kernel);      /* PWL 2, IB_4 4, IB_6 6, BSPLINE_3 3, _4 4, _5 5, _6 6 */
 int orc_weights_1d(int kernel, double Xs, double x_lower, double dx, double* w);
+/* The same with the unshifted X as well: BSPLINE_4 and BSPLINE_6 choose their
+ * stencil by comparing the unshifted X(d,s) with the centre of the cell of
+ * X(d,s)+Xshift (lagrangian_interaction3d.f.m4:2891, :3469; 2d.f.m4:2708,
+ * :3156), which for a periodic image forces the choice by the shift's sign. */
+int orc_weights_1d_u(int kernel, double Xs, double Xu, double x_lower, double dx, double* w);
 
 /* Fortran-equivalent kernels.  Arrays follow the Fortran declarations:
  *   u(ilower0-nugc0:iupper0+nugc0, ..., 0:depth-1)  column-major, depth slowest

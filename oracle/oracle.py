@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-KERNELS = {"PIECEWISE_LINEAR": 0, "IB_4": 1, "IB_6": 2, "BSPLINE_3": 3}
+KERNELS = {"PIECEWISE_LINEAR": 0, "IB_4": 1, "IB_6": 2, "BSPLINE_3": 3, "BSPLINE_4": 4, "BSPLINE_5": 5, "BSPLINE_6": 6}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -40,6 +40,10 @@ def lib():
         L.orc_bspline_3_delta.argtypes = [ctypes.c_double]
         L.orc_bspline_3_delta.restype = ctypes.c_double
         L.orc_weights_1d.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_weights_1d_u.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, _dp]
+        L.orc_delta.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_delta.restype = ctypes.c_double
         L.orc_interp.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _dp,
                                  _ip, _dp, ctypes.c_int, _dp, _dp]
         L.orc_spread.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _dp, ctypes.c_int,
@@ -104,11 +108,17 @@ def bspline_3_delta(x):
     return lib().orc_bspline_3_delta(float(x))
 
 
-def weights_1d(kernel, Xs, x_lower, dx):
-    """Returns (first stencil index relative to ilower, weights)."""
+def delta(kernel, r):
+    """The kernel's scalar delta function (IB_4 and the B-splines)."""
+    return lib().orc_delta(kernel_id(kernel), float(r))
+
+
+def weights_1d(kernel, Xs, x_lower, dx, Xu=None):
+    """Returns (first stencil index relative to ilower, weights).  Xu: the
+    unshifted X, which BSPLINE_4/6 compare with the cell centre (default Xs)."""
     k = kernel_id(kernel)
     w = np.zeros(8)
-    lo = lib().orc_weights_1d(k, float(Xs), float(x_lower), float(dx), _d(w))
+    lo = lib().orc_weights_1d_u(k, float(Xs), float(Xs if Xu is None else Xu), float(x_lower), float(dx), _d(w))
     return lo, w[: kernel_width(k)].copy()
 
 
