@@ -48,7 +48,8 @@ C_API = [
     "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
     "ibamr_s0_interp", "ibamr_s0_spread",
-] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear")
+] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear",
+                                                        "bspline_4", "bspline_5", "bspline_6")
      for op in ("interp", "spread") for d in ("2d", "3d")]
 
 

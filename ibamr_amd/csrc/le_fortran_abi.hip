@@ -49,11 +49,13 @@ struct S0Geom
     int n[3];     // ghost box extents
 };
 
+// Xs = X(d,s) + Xshift(d,l); Xu = X(d,s) (BSPLINE_4/6 choose their stencil side with it)
 template <int NDIM, int KER>
-__device__ __forceinline__ void s0_stencil(const S0Geom& g, const double* Xs, int* bse, double (*w)[KernelTraits<KER>::W])
+__device__ __forceinline__ void s0_stencil(const S0Geom& g, const double* Xs, const double* Xu, int* bse,
+                                           double (*w)[KernelTraits<KER>::W])
 {
 #pragma unroll
-    for (int d = 0; d < NDIM; ++d) bse[d] = g.ilo[d] + weights_1d<KER>(Xs[d], g.x_lower[d], g.dx[d], w[d]);
+    for (int d = 0; d < NDIM; ++d) bse[d] = g.ilo[d] + weights_1d_u<KER>(Xs[d], Xu[d], g.x_lower[d], g.dx[d], w[d]);
 }
 
 // last[s] = largest list position with index s
@@ -79,7 +81,7 @@ __global__ void k_s0_interp(S0Geom g, const double* __restrict__ u, const int* _
     for (int d = 0; d < NDIM; ++d) Xs[d] = X[NDIM * (size_t)s + d] + Xshift[NDIM * (size_t)l + d];
     int bse[3];
     double w[3][W];
-    s0_stencil<NDIM, KER>(g, Xs, bse, w);
+    s0_stencil<NDIM, KER>(g, Xs, X + NDIM * (size_t)s, bse, w);
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < NDIM; ++d)
@@ -139,7 +141,7 @@ __global__ void k_s0_contrib(S0Geom g, const int* __restrict__ indices, const do
     for (int d = 0; d < NDIM; ++d) Xs[d] = X[NDIM * (size_t)s + d] + Xshift[NDIM * (size_t)l + d];
     int bse[3];
     double w[3][W];
-    s0_stencil<NDIM, KER>(g, Xs, bse, w);
+    s0_stencil<NDIM, KER>(g, Xs, X + NDIM * (size_t)s, bse, w);
     const double vol = NDIM == 3 ? (g.dx[0] * g.dx[1]) * g.dx[2] : g.dx[0] * g.dx[1];
     const double Vv = V[(size_t)g.depth * s + c];
     const size_t ncomp = (size_t)g.n[0] * g.n[1] * (NDIM == 3 ? g.n[2] : 1);
@@ -195,10 +197,25 @@ int kid(const char* k)
     if (!std::strcmp(k, "IB_6")) return K_IB6;
     if (!std::strcmp(k, "BSPLINE_3")) return K_BS3;
     if (!std::strcmp(k, "PIECEWISE_LINEAR")) return K_PWL;
+    if (!std::strcmp(k, "BSPLINE_4")) return K_BS4;
+    if (!std::strcmp(k, "BSPLINE_5")) return K_BS5;
+    if (!std::strcmp(k, "BSPLINE_6")) return K_BS6;
     return -1;
 }
 
-int width(int k) { return k == K_IB4 ? 4 : (k == K_IB6 ? 6 : (k == K_BS3 ? 3 : 2)); }
+int width(int k)
+{
+    switch (k)
+    {
+    case K_IB4: return KernelTraits<K_IB4>::W;
+    case K_IB6: return KernelTraits<K_IB6>::W;
+    case K_BS3: return KernelTraits<K_BS3>::W;
+    case K_BS4: return KernelTraits<K_BS4>::W;
+    case K_BS5: return KernelTraits<K_BS5>::W;
+    case K_BS6: return KernelTraits<K_BS6>::W;
+    default: return KernelTraits<K_PWL>::W;
+    }
+}
 
 struct DevBuf
 {
@@ -251,6 +268,9 @@ int by_kernel(int ndim, int k, F&& f)
         case K_IB6: return f(nd, std::integral_constant<int, K_IB6>{});
         case K_BS3: return f(nd, std::integral_constant<int, K_BS3>{});
         case K_PWL: return f(nd, std::integral_constant<int, K_PWL>{});
+        case K_BS4: return f(nd, std::integral_constant<int, K_BS4>{});
+        case K_BS5: return f(nd, std::integral_constant<int, K_BS5>{});
+        case K_BS6: return f(nd, std::integral_constant<int, K_BS6>{});
         default: return set_error(IBAMR_ERR_BAD_KERNEL, "unknown kernel function");
         }
     };
@@ -445,3 +465,15 @@ S0_INTERP3(lagrangian_piecewise_linear_interp3d_, "PIECEWISE_LINEAR")
 S0_SPREAD3(lagrangian_piecewise_linear_spread3d_, "PIECEWISE_LINEAR")
 S0_INTERP2(lagrangian_piecewise_linear_interp2d_, "PIECEWISE_LINEAR")
 S0_SPREAD2(lagrangian_piecewise_linear_spread2d_, "PIECEWISE_LINEAR")
+S0_INTERP3(lagrangian_bspline_4_interp3d_, "BSPLINE_4")
+S0_SPREAD3(lagrangian_bspline_4_spread3d_, "BSPLINE_4")
+S0_INTERP3(lagrangian_bspline_5_interp3d_, "BSPLINE_5")
+S0_SPREAD3(lagrangian_bspline_5_spread3d_, "BSPLINE_5")
+S0_INTERP3(lagrangian_bspline_6_interp3d_, "BSPLINE_6")
+S0_SPREAD3(lagrangian_bspline_6_spread3d_, "BSPLINE_6")
+S0_INTERP2(lagrangian_bspline_4_interp2d_, "BSPLINE_4")
+S0_SPREAD2(lagrangian_bspline_4_spread2d_, "BSPLINE_4")
+S0_INTERP2(lagrangian_bspline_5_interp2d_, "BSPLINE_5")
+S0_SPREAD2(lagrangian_bspline_5_spread2d_, "BSPLINE_5")
+S0_INTERP2(lagrangian_bspline_6_interp2d_, "BSPLINE_6")
+S0_SPREAD2(lagrangian_bspline_6_spread2d_, "BSPLINE_6")

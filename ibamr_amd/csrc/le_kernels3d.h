@@ -49,6 +49,9 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     using KT = KernelTraits<KER>;
     constexpr int W = KT::W;
     constexpr bool FP64 = sizeof(T) == 8;
+    // FP32C stages the set-up pass's stencils, except for kernels whose
+    // stencil side depends on the unshifted X (an image's differs)
+    constexpr bool PRECOMP = !FP64 && !side_by_unshifted_x<KER>();
     __shared__ T s_rx[64 * S3_PAD];
     __shared__ T s_ry[64 * S3_PAD];
     __shared__ T s_wz[64 * 8];
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
         auto load_point = [&]() {
             if (nk < 0) return;
             nc4 = a.cells[nk];
-            if (FP64)
+            if (!PRECOMP)
             {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) nX[d] = a.X[3 * (size_t)nk + d];
@@ -312,14 +315,14 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                 {
                     T w[3][W];
                     int bse[3];
-                    if (FP64)
+                    if (!PRECOMP)
                     {
 #pragma unroll
                         for (int d = 0; d < 3; ++d)
                         {
                             // X + Xshift, Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
                             const double Xs = nX[d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
-                            bse[d] = P.ilo[d] + weights_1d<KER>(Xs, xl[d], a.dx[d], w[d]);
+                            bse[d] = P.ilo[d] + weights_1d_u<KER>(Xs, nX[d], xl[d], a.dx[d], w[d]);
                         }
                     }
                     else

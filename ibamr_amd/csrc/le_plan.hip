@@ -77,8 +77,9 @@ struct TileDesc
 };
 
 // The reference's kernel registry (LEInteractor.cpp:2035-2108): every known
-// name and its stencil size.  Only IB_4, IB_6 and BSPLINE_3 have GPU kernels;
-// the other names are known (isKnownKernel, getStencilSize) but plans reject them.
+// name and its stencil size.  IB_4, IB_6, BSPLINE_3..6 and PIECEWISE_LINEAR
+// have GPU kernels; the other names are known (isKnownKernel, getStencilSize)
+// but plans reject them.
 struct KernelName
 {
     const char* name;
@@ -86,8 +87,8 @@ struct KernelName
     int id;   // K_* of the GPU kernel, or -2 (registry only)
 };
 static const KernelName k_registry[] = {
-    {"BSPLINE_3", 4, K_BS3},          {"BSPLINE_4", 4, -2},
-    {"BSPLINE_5", 6, -2},             {"BSPLINE_6", 6, -2},
+    {"BSPLINE_3", 4, K_BS3},          {"BSPLINE_4", 4, K_BS4},
+    {"BSPLINE_5", 6, K_BS5},          {"BSPLINE_6", 6, K_BS6},
     {"COMPOSITE_BSPLINE_23", 4, -2},  {"COMPOSITE_BSPLINE_32", 4, -2},
     {"COMPOSITE_BSPLINE_34", 4, -2},  {"COMPOSITE_BSPLINE_43", 4, -2},
     {"COMPOSITE_BSPLINE_45", 5, -2},  {"COMPOSITE_BSPLINE_54", 5, -2},
@@ -122,6 +123,9 @@ static int stencil_size(int k)
     case K_IB4: return 4;
     case K_IB6: return 6;
     case K_BS3: return 4;  // LEInteractor.cpp:2057-2058
+    case K_BS4: return 4;
+    case K_BS5: return 6;  // LEInteractor.cpp:2061-2062
+    case K_BS6: return 6;
     default: return -1;
     }
 }
@@ -590,6 +594,9 @@ int kernel_width(int k)
     case K_IB6: return KernelTraits<K_IB6>::W;
     case K_BS3: return KernelTraits<K_BS3>::W;
     case K_PWL: return KernelTraits<K_PWL>::W;
+    case K_BS4: return KernelTraits<K_BS4>::W;
+    case K_BS5: return KernelTraits<K_BS5>::W;
+    case K_BS6: return KernelTraits<K_BS6>::W;
     default: return 6;
     }
 }
@@ -831,9 +838,10 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
                             for (int d = 0; d < NDIM; ++d)
                             {
                                 // X + Xshift with Xshift = (double)(offset) * dx  (LIndexSetData.cpp:116)
-                                const double Xs = a.X[NDIM * (size_t)k + d] + (double)(o[d] * a.pshift[d]) * a.dx[d];
+                                const double Xu = a.X[NDIM * (size_t)k + d];
+                                const double Xs = Xu + (double)(o[d] * a.pshift[d]) * a.dx[d];
                                 T w[W];
-                                const int rel = weights_1d<KER>(Xs, xl[d], a.dx[d], w);
+                                const int rel = weights_1d_u<KER>(Xs, Xu, xl[d], a.dx[d], w);
                                 const int bse = P.ilo[d] + rel;
                                 // clip to the array's ghost box (istart/istop, f.m4:1452-1457), then to the tile
                                 const int l = max(max(bse, alo[d]), flo[d]);
@@ -1255,6 +1263,9 @@ int dispatch(int ndim, int kernel, int precision, F&& f)
         case K_IB6: return f(nd, std::integral_constant<int, K_IB6>{}, prec);
         case K_BS3: return f(nd, std::integral_constant<int, K_BS3>{}, prec);
         case K_PWL: return f(nd, std::integral_constant<int, K_PWL>{}, prec);
+        case K_BS4: return f(nd, std::integral_constant<int, K_BS4>{}, prec);
+        case K_BS5: return f(nd, std::integral_constant<int, K_BS5>{}, prec);
+        case K_BS6: return f(nd, std::integral_constant<int, K_BS6>{}, prec);
         default: return fail(IBAMR_ERR_BAD_KERNEL, "kernel has no GPU spread/interp");
         }
     };
@@ -2605,7 +2616,8 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         if (P->npoints > 0)
         {
             const int B = 256;
-            constexpr bool WTS = NDIM == 3 && sizeof(T) == 4;  // FP32C 3-D: stencils set up here
+            // FP32C 3-D: stencils set up here (not for kernels whose side depends on the unshifted X)
+            constexpr bool WTS = NDIM == 3 && sizeof(T) == 4 && !side_by_unshifted_x<KER>();
             if (WTS && P->fdepth == NDIM)
             {
                 // gather and stencils in one pass

@@ -9,7 +9,8 @@
 //
 // Reference: ibtk/src/lagrangian/fortran/lagrangian_interaction3d.f.m4
 //   IB_4 :1265-1273, IB_6 :2238-2272 (K at :2227), BSPLINE_3 :2655-2676,
-//   and lagrangian_delta.f.m4:243-260 (B-spline delta).
+//   BSPLINE_4 :2882-2908, BSPLINE_5 :3230-3248, BSPLINE_6 :3460-3480,
+//   and lagrangian_delta.f.m4:243-354 (B-spline deltas).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,7 +22,10 @@ enum KernelId : int
     K_PWL = 0,
     K_IB4 = 1,
     K_IB6 = 2,
-    K_BS3 = 3
+    K_BS3 = 3,
+    K_BS4 = 4,
+    K_BS5 = 5,
+    K_BS6 = 6
 };
 
 template <int K>
@@ -47,6 +51,33 @@ struct KernelTraits<K_BS3>
     static constexpr bool IB = false;
     static constexpr int GHOST = 3;
 };
+template <>
+struct KernelTraits<K_BS4>
+{
+    static constexpr int W = 4;
+    static constexpr bool IB = false;
+    static constexpr int GHOST = 3;  // stencil size 4
+};
+template <>
+struct KernelTraits<K_BS5>
+{
+    static constexpr int W = 5;
+    static constexpr bool IB = false;
+    static constexpr int GHOST = 4;  // stencil size 6 (LEInteractor.cpp getStencilSize)
+};
+template <>
+struct KernelTraits<K_BS6>
+{
+    static constexpr int W = 6;
+    static constexpr bool IB = false;
+    static constexpr int GHOST = 4;
+};
+// Kernels whose stencil side depends on the unshifted X (weights_1d_u).
+template <int K>
+constexpr bool side_by_unshifted_x()
+{
+    return K == K_BS4 || K == K_BS6;
+}
 template <>
 struct KernelTraits<K_PWL>
 {
@@ -81,6 +112,72 @@ __device__ __forceinline__ float bspline3_delta_f(float x)
         return 0.5f * t * t;
     }
     return 0.0f;
+}
+
+// lagrangian_delta.f.m4:268-354, FP64, the reference's operation order: each
+// piece a polynomial in r = |x| + n/2, evaluated left to right.
+__device__ __forceinline__ double bspline4_delta_d(double x)
+{
+    const double modx = fabs(x);
+    const double r = modx + 2.0;
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    if (modx <= 1.0) return (1.0 / 6.0) * (((3.0 * r3 - 24.0 * r2) + 60.0 * r) - 44.0);
+    if (modx <= 2.0) return (1.0 / 6.0) * (((-r3 + 12.0 * r2) - 48.0 * r) + 64.0);
+    return 0.0;
+}
+__device__ __forceinline__ double bspline5_delta_d(double x)
+{
+    const double modx = fabs(x);
+    const double r = modx + 2.5;
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    const double r4 = r3 * r;
+    if (modx <= 0.5) return (1.0 / 24.0) * ((((6.0 * r4 - 60.0 * r3) + 210.0 * r2) - 300.0 * r) + 155.0);
+    if (modx <= 1.5) return (1.0 / 24.0) * ((((-4.0 * r4 + 60.0 * r3) - 330.0 * r2) + 780.0 * r) - 655.0);
+    if (modx <= 2.5) return (1.0 / 24.0) * ((((r4 - 20.0 * r3) + 150.0 * r2) - 500.0 * r) + 625.0);
+    return 0.0;
+}
+__device__ __forceinline__ double bspline6_delta_d(double x)
+{
+    const double modx = fabs(x);
+    const double r = modx + 3.0;
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    const double r4 = r3 * r;
+    const double r5 = r4 * r;
+    if (modx <= 1.0)
+        return (1.0 / 60.0) * (((((2193.0 - 3465.0 * r) + 2130.0 * r2) - 630.0 * r3) + 90.0 * r4) - 5.0 * r5);
+    if (modx <= 2.0)
+        return (1.0 / 120.0) * (((((-10974.0 + 12270.0 * r) - 5340.0 * r2) + 1140.0 * r3) - 120.0 * r4) + 5.0 * r5);
+    if (modx <= 3.0) return (1.0 / 120.0) * (((((7776.0 - 6480.0 * r) + 2160.0 * r2) - 360.0 * r3) + 30.0 * r4) - r5);
+    return 0.0;
+}
+
+// Shared stencil of BSPLINE_4/5/6: centre cell of X+Xshift, then the odd order
+// takes centre -2..+2 and the even orders one side chosen by comparing the
+// unshifted X with that cell's centre (Xu; Xu = Xs without a shift).
+template <int K>
+__device__ __forceinline__ int bspline_hi_weights(double Xs, double Xu, double x_lower, double dx, double* w)
+{
+    constexpr int W = KernelTraits<K>::W;
+    const int c = (int)floor((Xs - x_lower) / dx);
+    int lo;
+    if (K == K_BS5)
+        lo = c - 2;
+    else
+    {
+        const double X_c = x_lower + ((double)c + 0.5) * dx;
+        lo = Xu < X_c ? c - W / 2 : c - W / 2 + 1;
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k)
+    {
+        const double X_cell = x_lower + ((double)(lo + k) + 0.5) * dx;
+        const double t = (Xs - X_cell) / dx;
+        w[k] = K == K_BS4 ? bspline4_delta_d(t) : (K == K_BS5 ? bspline5_delta_d(t) : bspline6_delta_d(t));
+    }
+    return lo;
 }
 
 // K of the 6-point kernel (f.m4:2227) and the constant sub-expressions of
@@ -196,6 +293,22 @@ __device__ __forceinline__ int weights_1d<K_BS3>(double Xs, double x_lower, doub
     return lo;
 }
 
+template <>
+__device__ __forceinline__ int weights_1d<K_BS4>(double Xs, double x_lower, double dx, double* w)
+{
+    return bspline_hi_weights<K_BS4>(Xs, Xs, x_lower, dx, w);
+}
+template <>
+__device__ __forceinline__ int weights_1d<K_BS5>(double Xs, double x_lower, double dx, double* w)
+{
+    return bspline_hi_weights<K_BS5>(Xs, Xs, x_lower, dx, w);
+}
+template <>
+__device__ __forceinline__ int weights_1d<K_BS6>(double Xs, double x_lower, double dx, double* w)
+{
+    return bspline_hi_weights<K_BS6>(Xs, Xs, x_lower, dx, w);
+}
+
 // PIECEWISE_LINEAR, lagrangian_interaction3d.f.m4:680-699 (spread) / 556-575
 // (interp): the cell whose centre is nearest, then the linear hat on the two
 // cells straddling X.
@@ -264,6 +377,41 @@ __device__ __forceinline__ int weights_1d<K_PWL>(double Xs, double x_lower, doub
     w[0] = (float)wd[0];
     w[1] = (float)wd[1];
     return lo;
+}
+
+// BSPLINE_4/5/6 in FP32 forms: the FP64 weights rounded once (their pieces are
+// polynomials in |x| + n/2 whose terms cancel, which FP32 would not survive).
+#define IBAMR_BS_HI_F(K)                                                                     \
+    template <>                                                                              \
+    __device__ __forceinline__ int weights_1d<K>(double Xs, double x_lower, double dx, float* w) \
+    {                                                                                        \
+        double wd[KernelTraits<K>::W];                                                       \
+        const int lo = weights_1d<K>(Xs, x_lower, dx, wd);                                   \
+        for (int k = 0; k < KernelTraits<K>::W; ++k) w[k] = (float)wd[k];                    \
+        return lo;                                                                           \
+    }
+IBAMR_BS_HI_F(K_BS4)
+IBAMR_BS_HI_F(K_BS5)
+IBAMR_BS_HI_F(K_BS6)
+#undef IBAMR_BS_HI_F
+
+// Weights of X(d,s)+Xshift with the unshifted X(d,s) as well: only BSPLINE_4/6
+// use it (their stencil side); for the others this is weights_1d.
+template <int K, typename T>
+__device__ __forceinline__ int weights_1d_u(double Xs, double Xu, double x_lower, double dx, T* w)
+{
+    if constexpr (side_by_unshifted_x<K>())
+    {
+        double wd[KernelTraits<K>::W];
+        const int lo = bspline_hi_weights<K>(Xs, Xu, x_lower, dx, wd);
+        for (int k = 0; k < KernelTraits<K>::W; ++k) w[k] = (T)wd[k];
+        return lo;
+    }
+    else
+    {
+        (void)Xu;
+        return weights_1d<K>(Xs, x_lower, dx, w);
+    }
 }
 
 } // namespace ibamr

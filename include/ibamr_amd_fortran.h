@@ -80,6 +80,21 @@ void lagrangian_piecewise_linear_spread3d_(IBAMR_S0_SPREAD3D_ARGS);
 /* lagrangian_interaction2d.f.m4:469 / :587 */
 void lagrangian_piecewise_linear_interp2d_(IBAMR_S0_INTERP2D_ARGS);
 void lagrangian_piecewise_linear_spread2d_(IBAMR_S0_SPREAD2D_ARGS);
+/* lagrangian_interaction3d.f.m4:2814 / :2933 (BSPLINE_4), :3050 / :3162
+ * (BSPLINE_5), :3273 / :3392 (BSPLINE_6) */
+void lagrangian_bspline_4_interp3d_(IBAMR_S0_INTERP3D_ARGS);
+void lagrangian_bspline_4_spread3d_(IBAMR_S0_SPREAD3D_ARGS);
+void lagrangian_bspline_5_interp3d_(IBAMR_S0_INTERP3D_ARGS);
+void lagrangian_bspline_5_spread3d_(IBAMR_S0_SPREAD3D_ARGS);
+void lagrangian_bspline_6_interp3d_(IBAMR_S0_INTERP3D_ARGS);
+void lagrangian_bspline_6_spread3d_(IBAMR_S0_SPREAD3D_ARGS);
+/* lagrangian_interaction2d.f.m4:2519 / :2634, :2748 / :2858, :2967 / :3082 */
+void lagrangian_bspline_4_interp2d_(IBAMR_S0_INTERP2D_ARGS);
+void lagrangian_bspline_4_spread2d_(IBAMR_S0_SPREAD2D_ARGS);
+void lagrangian_bspline_5_interp2d_(IBAMR_S0_INTERP2D_ARGS);
+void lagrangian_bspline_5_spread2d_(IBAMR_S0_SPREAD2D_ARGS);
+void lagrangian_bspline_6_interp2d_(IBAMR_S0_INTERP2D_ARGS);
+void lagrangian_bspline_6_spread2d_(IBAMR_S0_SPREAD2D_ARGS);
 
 /* Status-returning forms of the same operations (no abort), for callers that
  * want an error code: kernel is "IB_4" | "IB_6" | "BSPLINE_3", ndim 2 or 3,

@@ -2,7 +2,7 @@
 outputs (data files only; run in a container where the IBAMR checkout is
 mounted at /root/reference).
 
-  interpolate_01_{2d,3d}.{ib_4,ib_6,bspline_3}.json
+  interpolate_01_{2d,3d}.{ib_4,ib_6,bspline_3,bspline_4,bspline_5,bspline_6}.json
       from tests/interpolate/interpolate_01_<dim>.<kernel>.output: the printed
       point coordinates and interpolated values (cell-centred, depth NDIM).
   curve2d_64.vertex, curve2d_64.spring
@@ -120,7 +120,7 @@ def patches_fixture(ref, case):
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     for dim in ("2d", "3d"):
-        for kernel in ("ib_4", "ib_6", "bspline_3"):
+        for kernel in ("ib_4", "ib_6", "bspline_3", "bspline_4", "bspline_5", "bspline_6"):
             fx = interp_fixture(ref, dim, kernel)
             with open(os.path.join(HERE, "interpolate_01_%s.%s.json" % (dim, kernel)), "w") as f:
                 json.dump(fx, f)

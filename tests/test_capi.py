@@ -107,7 +107,7 @@ def test_registry_full_reference_table(L):
         assert ia.get_stencil_size(k) == s, k
         assert ia.get_minimum_ghost_width(k) == s // 2 + 1, k
     assert ia.is_known_kernel("USER_DEFINED")
-    for k in ("IB_3", "BSPLINE_4", "PIECEWISE_CUBIC"):
+    for k in ("IB_3", "COMPOSITE_BSPLINE_43", "PIECEWISE_CUBIC"):
         with pytest.raises(ia.IBAMRError) as e:
             ia.LevelPlan(_level3(), k, centering=ia.SIDE, max_points=10)
         assert e.value.code == ia.IBAMR_ERR_BAD_KERNEL

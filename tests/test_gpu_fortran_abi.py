@@ -32,7 +32,8 @@ def _case(ndim, n, gw, npts, seed):
 
 
 @pytest.mark.parametrize("ndim", [2, 3])
-@pytest.mark.parametrize("kernel", ["IB_4", "IB_6", "BSPLINE_3", "PIECEWISE_LINEAR"])
+@pytest.mark.parametrize("kernel", ["IB_4", "IB_6", "BSPLINE_3", "PIECEWISE_LINEAR", "BSPLINE_4", "BSPLINE_5",
+                                    "BSPLINE_6"])
 @pytest.mark.parametrize("depth", [1, 2])
 def test_s0_bitwise(orc, ndim, kernel, depth):
     print("s0", ndim, kernel, depth, flush=True)
@@ -59,7 +60,9 @@ def test_s0_bitwise(orc, ndim, kernel, depth):
     assert np.array_equal(ug, uo)
 
 
-@pytest.mark.parametrize("kernel,tag", [("IB_4", "ib_4"), ("IB_6", "ib_6"), ("BSPLINE_3", "bspline_3")])
+@pytest.mark.parametrize("kernel,tag", [("IB_4", "ib_4"), ("IB_6", "ib_6"), ("BSPLINE_3", "bspline_3"),
+                                        ("BSPLINE_4", "bspline_4"), ("BSPLINE_5", "bspline_5"),
+                                        ("BSPLINE_6", "bspline_6")])
 def test_s0_interpolate_01_3d_golden(kernel, tag):
     """The reference test interpolate_01_3d.<kernel> through the Fortran-ABI
     drop-in on the GPU: 16-digit agreement with the printed values."""
@@ -80,7 +83,8 @@ def test_s0_interpolate_01_3d_golden(kernel, tag):
                  depth=3)
     for c in range(3):
         err = np.abs(V[:, c] - rows[:, 3 + c]) / np.abs(rows[:, 3 + c])
-        assert err.max() < 2e-15
+        # print precision; BSPLINE_4/5/6: the conditioning of their polynomial form (test_oracle_golden)
+        assert err.max() < {"BSPLINE_4": 5e-14, "BSPLINE_5": 5e-13, "BSPLINE_6": 3e-12}.get(kernel, 2e-15)
 
 
 def test_s0_fortran_symbol_call():

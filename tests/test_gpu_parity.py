@@ -113,6 +113,12 @@ CASES = [
     (2, 32, "IB_4", ia.SIDE, 1, 500),
     (2, 32, "IB_6", ia.SIDE, 1, 500),
     (2, 24, "BSPLINE_3", ia.CELL, 3, 400),
+    (3, 16, "BSPLINE_4", ia.SIDE, 1, 600),
+    (3, 16, "BSPLINE_5", ia.SIDE, 1, 600),
+    (3, 16, "BSPLINE_6", ia.SIDE, 1, 600),
+    (3, 12, "BSPLINE_5", ia.CELL, 1, 300),
+    (2, 32, "BSPLINE_4", ia.SIDE, 1, 500),
+    (2, 24, "BSPLINE_6", ia.CELL, 2, 400),
 ]
 
 
@@ -181,7 +187,10 @@ def test_fp64_periodic_bitwise(orc, case):
     assert np.array_equal(U.cpu().numpy(), Uref), np.nanmax(np.abs(U.cpu().numpy() - Uref))
 
 
-@pytest.mark.parametrize("case", CASES[:3] + CASES[5:7], ids=[_ids(c) for c in CASES[:3] + CASES[5:7]])
+FP32_CASES = CASES[:3] + CASES[5:7] + CASES[8:11] + CASES[12:13]
+
+
+@pytest.mark.parametrize("case", FP32_CASES, ids=[_ids(c) for c in FP32_CASES])
 def test_fp32c_error(orc, case):
     ndim, n, kernel, centering, depth, npts = case
     print("case", _ids(case), flush=True)

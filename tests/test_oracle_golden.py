@@ -7,7 +7,8 @@ Fixtures (tests/golden/*.json) hold the data printed by the reference tests:
   spread with the raw-array overload, :137-156).
 The point sets are regenerated bit-for-bit (tests/golden/mt19937.py), so the
 comparison is against exact inputs.  Tolerances are print precision (16
-significant digits): 2e-15 relative.
+significant digits): 2e-15 relative, except for the BSPLINE_4/5/6 kernels,
+whose polynomial form bounds the agreement between two compilations (COND_TOL).
 """
 import json
 import math
@@ -19,8 +20,16 @@ import pytest
 from mt19937 import MT19937, uniform_real
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-KERNELS = [("IB_4", "ib_4"), ("IB_6", "ib_6"), ("BSPLINE_3", "bspline_3")]
+KERNELS = [("IB_4", "ib_4"), ("IB_6", "ib_6"), ("BSPLINE_3", "bspline_3"), ("BSPLINE_4", "bspline_4"),
+           ("BSPLINE_5", "bspline_5"), ("BSPLINE_6", "bspline_6")]
 PRINT_TOL = 2e-15
+# BSPLINE_4/5/6 evaluate each piece as a polynomial in r = |x| + n/2 whose terms
+# (up to 64/6, 625/24 and 7776/120) cancel to weights of order 1e-2, so one
+# rounding difference in a term (the reference's compiler may contract or
+# reorder; the oracle follows the source order with no contraction) moves a
+# weight by ~1e-14 and V by up to ~1e-12 relative.  Any slip in a coefficient
+# or stencil is >= 1e-4.
+COND_TOL = {"BSPLINE_4": 5e-14, "BSPLINE_5": 5e-13, "BSPLINE_6": 3e-12}
 
 
 def _load(name):
@@ -59,7 +68,7 @@ def test_interpolate_01_3d_golden(orc, kernel, tag):
     V = orc.interp(kernel, dx, xl, il, iu, [gw] * 3, u, X, indices=idx, depth=3)
     for c in range(3):
         err = np.abs(V[:, c] - rows[:, 3 + c]) / np.abs(rows[:, 3 + c])
-        assert err.max() < PRINT_TOL, (kernel, c, err.max())
+        assert err.max() < COND_TOL.get(kernel, PRINT_TOL), (kernel, c, err.max())
 
 
 @pytest.mark.parametrize("kernel,tag", KERNELS)
@@ -86,7 +95,7 @@ def test_interpolate_01_2d_golden_interior(orc, kernel, tag):
     assert ok.sum() >= 40
     for c in range(2):
         err = np.abs(V[ok, c] - rows[ok, 3 + c]) / np.abs(rows[ok, 3 + c])
-        assert err.max() < PRINT_TOL, (kernel, c, err.max())
+        assert err.max() < COND_TOL.get(kernel, PRINT_TOL), (kernel, c, err.max())
 
 
 def test_ghost_accumulation_3d_side_spread_golden(orc):
