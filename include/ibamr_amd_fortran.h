@@ -5,7 +5,7 @@
  * extern "C" block of ibtk/src/lagrangian/LEInteractor.cpp:237-1516 (symbols
  * mangled by FortranCInterface, IBTK_FC_FUNC_, CMakeLists.txt:66-83; gfortran
  * gives lower case plus a trailing underscore).  This library exports the same
- * symbols for IB_4, IB_6 and BSPLINE_3 in 2-D and 3-D, with the same argument
+ * symbols for IB_4, IB_6, BSPLINE_3..6 and PIECEWISE_LINEAR in 2-D and 3-D, with the same argument
  * lists (every scalar by reference: C++ `const int&` == `const int*`) and the
  * same semantics:
  *   - interp overwrites V(0:depth-1, indices(l)) for every listed entry and
