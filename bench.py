@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--cpu-sample-cores", type=int, default=2000000, help="points of the all-cores CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--point-order", choices=["regrid", "generated"], default="regrid",
+                    help="regrid: the points in LDataManager's order after a redistribution (the reference's "
+                         "steady state); generated: the generator's order, random in space")
     ap.add_argument("--workload", choices=["shell512", "amr", "multi"], default="shell512",
                     help="shell512: periodic 512^3 IB_6 (default); amr: multi-patch fine level, BSPLINE_3; "
                          "multi: 64 shells on 384^3 (IB_4 and BSPLINE_3), 100 graph-captured steps")
@@ -148,6 +151,22 @@ def time_redistribute(torch, ia, X, F, N, kernel, reps=3):
     ms = (time.perf_counter() - t0) * 1e3 / reps
     plan.close()
     return ms
+
+
+def regrid_order(torch, ia, X, F, N, kernel):
+    """The points in the order LDataManager keeps its local Lagrangian data after
+    a regrid (LDataManager::endDataRedistribution: patch, cell in box order,
+    Lagrangian index), computed by ibamr_plan_redistribute; the generator's
+    order is random in space."""
+    level = ia.Level.uniform_periodic(N, 3)
+    n = X.shape[0]
+    plan = ia.LevelPlan(level, kernel, ia.SIDE, ia.FP64, max_points=n)
+    Xw = X.clone()
+    order, _, _ = plan.redistribute(Xw, torch.arange(n, dtype=torch.int32, device="cuda"))
+    o = order.long()
+    torch.cuda.synchronize()
+    plan.close()
+    return Xw[o].contiguous(), F[o].contiguous()
 
 
 def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
@@ -491,7 +510,10 @@ def main():
         return
     N, n, kernel = args.grid, args.points, args.kernel
     X, F64 = make_points(torch, n, N, seed=2026)
-    print("bench: %d^3 %s side, %d points; FP32C form" % (N, kernel, n), file=sys.stderr, flush=True)
+    if args.point_order == "regrid":
+        X, F64 = regrid_order(torch, ia, X, F64, N, kernel)
+    print("bench: %d^3 %s side, %d points (%s order); FP32C form" % (N, kernel, n, args.point_order),
+          file=sys.stderr, flush=True)
     r32 = run_form(torch, ia, ia.FP32C, X, F64, N, kernel, args.steps, args.warmup)
     ms32 = r32["spread_ms"] + r32["interp_ms"]
     T = touched_faces(torch, X, N, kernel)
@@ -522,6 +544,7 @@ def main():
         "config": {"workload": "3-D periodic %d^3 side-centred %s, %d points (half uniform, half in a 2h shell "
                                "r=0.3), FP32 compensated spread+interp" % (N, kernel, n),
                    "grid": N, "points": n, "kernel": kernel, "centering": "side", "precision": "fp32c",
+                   "point_order": args.point_order,
                    "tile": [r32["info"]["tile"], r32["info"]["tile"], r32["plan"].tile_height],
                    "tile_autotune_ms": {str(h): t for h, t in r32["tune"].items()},
                    "bins": "%d^3 cells, sorted (CSR: no capacity limit)" % r32["info"]["tile"],
