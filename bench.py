@@ -352,8 +352,17 @@ def main_amr(args, torch, ia):
     g = torch.Generator(device="cuda")
     g.manual_seed(99)
     F = torch.randn((n, 3), generator=g, device="cuda", dtype=torch.float64)
-    print("bench amr: %d patches, %d fine cells, %d points" % (len(level.patches), cells, n), file=sys.stderr,
-          flush=True)
+    if args.point_order == "regrid":
+        # LDataManager's local order after a regrid of this level (patch, cell, Lagrangian index)
+        p = ia.LevelPlan(level, kernel, ia.SIDE, ia.FP64, max_points=n)
+        Xw = X.clone()
+        order, _, _ = p.redistribute(Xw, torch.arange(n, dtype=torch.int32, device="cuda"))
+        o = order.long()
+        torch.cuda.synchronize()
+        p.close()
+        X, F = Xw[o].contiguous(), F[o].contiguous()
+    print("bench amr: %d patches, %d fine cells, %d points (%s order)" % (len(level.patches), cells, n,
+                                                                       args.point_order), file=sys.stderr, flush=True)
     r32 = run_amr_form(torch, ia, ia.FP32C, level, X, F, kernel, args.steps, args.warmup)
     r64 = run_amr_form(torch, ia, ia.FP64, level, X, F, kernel, max(2, args.steps // 2), 1)
     err_f = max(float((a.double() - b).abs().max()) for a, b in zip(r32["f"], r64["f"])) / \
@@ -367,7 +376,7 @@ def main_amr(args, torch, ia):
                                "%d points on 4 ellipsoidal shells, side-centred BSPLINE_3, FP32 compensated"
                                % (len(level.patches), cells, n),
                    "patches": len(level.patches), "fine_cells": cells, "points": n, "kernel": kernel,
-                   "spread_tiles": r32["info"]["ntiles"],
+                   "point_order": args.point_order, "spread_tiles": r32["info"]["ntiles"],
                    "tile": [r32["info"]["tile"], r32["info"]["tile"], r32["height"]],
                    "tile_autotune_ms": {str(h): t for h, t in r32["tune"].items()},
                    "bins": "%d^3 cells, sorted (CSR: no capacity limit)" % r32["info"]["tile"]},
