@@ -1,6 +1,6 @@
 """ibamr_amd -- MI355X (gfx950) Eulerian-Lagrangian coupling for IBAMR-style
 immersed-boundary codes: regularized-delta spreading and interpolation
-(IB_4, IB_6, BSPLINE_3) on cell- and side-centred patch data.
+(IB_4, IB_6, BSPLINE_3..6, PIECEWISE_LINEAR) on cell- and side-centred patch data.
 
 Python host layer over the C ABI of libibamr_amd.so (include/ibamr_amd.h).
 Device memory comes from torch tensors (plumbing only); every computation runs

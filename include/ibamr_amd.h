@@ -98,7 +98,7 @@ typedef struct ibamr_level_geom
 
 typedef struct ibamr_plan_config
 {
-    const char* kernel;   /* "IB_4" | "IB_6" | "BSPLINE_3" */
+    const char* kernel;   /* "IB_4" | "IB_6" | "BSPLINE_3" | "BSPLINE_4" | "BSPLINE_5" | "BSPLINE_6" | "PIECEWISE_LINEAR" */
     int centering;        /* IBAMR_CELL | IBAMR_SIDE */
     int precision;        /* IBAMR_FP64 | IBAMR_FP32C */
     int depth;            /* cell data depth; side data: must be 1 */

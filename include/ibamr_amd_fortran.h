@@ -97,7 +97,7 @@ void lagrangian_bspline_6_interp2d_(IBAMR_S0_INTERP2D_ARGS);
 void lagrangian_bspline_6_spread2d_(IBAMR_S0_SPREAD2D_ARGS);
 
 /* Status-returning forms of the same operations (no abort), for callers that
- * want an error code: kernel is "IB_4" | "IB_6" | "BSPLINE_3", ndim 2 or 3,
+ * want an error code: kernel is "IB_4" | "IB_6" | "BSPLINE_3".."BSPLINE_6" | "PIECEWISE_LINEAR", ndim 2 or 3,
  * ilower/iupper/nugc arrays of length ndim.  Returns an ibamr_status. */
 int ibamr_s0_interp(const char* kernel, int ndim, const double* dx, const double* x_lower, int depth,
                     const int* ilower, const int* iupper, const int* nugc, const double* u, const int* indices,
