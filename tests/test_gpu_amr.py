@@ -16,7 +16,7 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
 
 
-@pytest.mark.parametrize("kernel", ["BSPLINE_3", "IB_4"])
+@pytest.mark.parametrize("kernel", ["BSPLINE_3", "IB_4", "BSPLINE_5", "BSPLINE_6"])
 def test_amr_level_fp64_bitwise(orc, kernel):
     level, X = synthetic.amr_level(coarse=16, ratio=4, npoints=4000, nshell=4, seed=5, block=16)
     assert len(level.patches) > 8
