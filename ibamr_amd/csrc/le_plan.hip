@@ -2393,10 +2393,13 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
         const int rc = dispatch(3, k, C->precision, [&](auto, auto ker, auto prec) -> int {
             using T = decltype(prec);
             constexpr int KER = decltype(ker)::value;
-            for (const void* f : {reinterpret_cast<const void*>(&k_interp3<KER, T, false>),
-                                  reinterpret_cast<const void*>(&k_interp3<KER, T, true>)})
-                HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           interp3_lds_elems<KER, T>() * (int)sizeof(T)));
+            // the variant reading the set-up pass's stencils exists for FP32C only
+            const void* fs[2] = {reinterpret_cast<const void*>(&k_interp3<KER, T, false>), nullptr};
+            if constexpr (sizeof(T) == 4) fs[1] = reinterpret_cast<const void*>(&k_interp3<KER, T, true>);
+            for (const void* f : fs)
+                if (f)
+                    HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               interp3_lds_elems<KER, T>() * (int)sizeof(T)));
             return IBAMR_OK;
         });
         if (rc != IBAMR_OK) return cleanup_fail(rc);
@@ -2835,12 +2838,19 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
             const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
             const int nrow = P->nsb[1] * P->nsb[2];
             const int per = (nrow + 7) / 8 * P->nsb[0];  // superbins per XCD (see k_interp3)
-            if (a.st)  // FP32C with the spread set-up's stencils of these positions
-                hipLaunchKernelGGL((k_interp3<KER, T, true>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems,
-                                   nsb, P->nsb[0]);
-            else
-                hipLaunchKernelGGL((k_interp3<KER, T, false>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems,
-                                   nsb, P->nsb[0]);
+            if constexpr (sizeof(T) == 4)
+            {
+                if (a.st)  // the spread set-up's stencils of these positions
+                {
+                    hipLaunchKernelGGL((k_interp3<KER, T, true>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a,
+                                       elems, nsb, P->nsb[0]);
+                    HIP_OK(hipGetLastError());
+                    if (t_end) HIP_OK(hipEventRecord(t_end, s));
+                    return IBAMR_OK;
+                }
+            }
+            hipLaunchKernelGGL((k_interp3<KER, T, false>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems, nsb,
+                               P->nsb[0]);
         }
         else
         {
