@@ -311,3 +311,41 @@ def test_spread_interp_bitwise_reproducible(precision):
     for a in range(3):
         assert torch.equal(outs[0][0][a], outs[1][0][a]), a
     assert torch.equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("precision", [ia.FP64, ia.FP32C])
+def test_shifted_index_space(precision):
+    """The same periodic level with its index space starting at (5, -3, 7)
+    instead of 0 (SAMRAI boxes may start anywhere): spread and interp give the
+    same bits, in both forms (FP32C places stencils in the level's frame, lower
+    index included)."""
+    n, npts, kernel = 24, 4000, "IB_6"
+    lo = [5, -3, 7]
+    level0 = ia.Level.uniform_periodic(n, 3)
+    p = ia.Patch(lo, [l + n - 1 for l in lo], [0.0] * 3, [1.0] * 3)
+    level1 = ia.Level([1.0 / n] * 3, lo, [l + n - 1 for l in lo], [0.0] * 3, [1.0] * 3, [n] * 3, [p])
+    dt = torch.float64 if precision == ia.FP64 else torch.float32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    X = torch.rand((npts, 3), generator=g, device=DEV, dtype=torch.float64)
+    X[:500] = 1.0 - 0.05 * torch.rand((500, 3), generator=g, device=DEV, dtype=torch.float64)  # periodic images
+    F = torch.randn((npts, 3), generator=g, device=DEV, dtype=torch.float64)
+    out = []
+    for level in (level0, level1):
+        plan = ia.LevelPlan(level, kernel, ia.SIDE, precision, max_points=npts)
+        shapes = [ia.array_shape(level, 0, plan.data_gcw, ia.SIDE, axis=a) for a in range(3)]
+        f = [torch.zeros(s, dtype=dt, device=DEV) for s in shapes]
+        u = [torch.sin(torch.arange(int(np.prod(s)), device=DEV, dtype=torch.float64) * 0.01).reshape(s).to(dt)
+             for s in shapes]
+        plan.bin(X)
+        plan.bind(0, f)
+        plan.bind(1, u)
+        plan.spread(0, F.to(dt), X, mode=ia.OVERWRITE)
+        U = torch.zeros((npts, 3), dtype=dt, device=DEV)
+        plan.interp(1, U, None)
+        torch.cuda.synchronize()
+        out.append(([a.cpu().numpy() for a in f], U.cpu().numpy()))
+        plan.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][1], out[1][1])
