@@ -67,6 +67,24 @@ def copy_probe(torch):
     print(json.dumps({"copy_probe_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
+def counters_from_profile(kernel):
+    """VALU-busy and LDS bank-conflict fractions of a kernel from the committed
+    rocprofv3 counter summary of the 512^3 workload ("Derived" table), or None."""
+    path = os.path.join(ROOT, "profiles", "counters_spread3_interp3_512c_side_ib6_8M_fp32c.md")
+    if not os.path.exists(path):
+        return None
+    col = 0 if kernel == "k_spread3" else 1
+    out = {}
+    with open(path) as fh:
+        for line in fh:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 3 and cells[0].startswith("VALU busy"):
+                out["valu_busy_frac"] = float(cells[1 + col].rstrip(" %")) / 100.0
+            elif len(cells) == 3 and cells[0].startswith("LDS bank-conflict rate"):
+                out["lds_bank_conflict_frac"] = float(cells[1 + col].rstrip(" %")) / 100.0
+    return out or None
+
+
 def traffic_from_profile(workload_tag):
     """HBM bytes per dispatch of the dominant kernel, from the committed
     rocprofv3 --pmc summary of this workload (FETCH_SIZE and WRITE_SIZE, each
@@ -568,7 +586,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tr["bytes_per_dispatch"] if tr else None, "algorithmic_bytes": B_dom,
-                     "traffic_source": ("profiles/hbm_traffic_%s.txt" % tag) if tr else None},
+                     "traffic_source": ("profiles/hbm_traffic_%s.txt" % tag) if tr else None,
+                     # what bounds it instead: instruction issue (rocprofv3 counters of this workload)
+                     "counters": counters_from_profile(kname) if N == 512 and kernel == "IB_6" else None},
         "detail": {"spread_ms": r32["spread_ms"], "interp_ms": r32["interp_ms"], "bin_ms": r32["bin_ms"],
                    "spread_kernel_ms": r32["spread_kernel_ms"], "interp_kernel_ms": r32["interp_kernel_ms"],
                    "touched_faces": T, "bytes_step": B_spread + B_interp,
