@@ -93,3 +93,38 @@ def test_height_validation():
     assert p2.tile_height == 1
     with pytest.raises(ia.IBAMRError):
         p2.tile_height = 16
+
+
+def test_graph_after_autotune():
+    """The autotuner rebuilds the work list (a new device array): a HIP graph
+    captured after it replays the spread and interpolation with the chosen tile,
+    equal to the eager calls on every replay."""
+    plan, X, F, shapes, dt = _setup(ia.FP32C, "IB_4", ia.SIDE)
+    f = [torch.zeros(s, dtype=dt, device=DEV) for s in shapes]
+    u = [torch.rand(s, dtype=torch.float64, device=DEV).to(dt) for s in shapes]
+    plan.bind(0, f)
+    plan.bind(1, u)
+    plan.autotune(0, F, X, reps=1)
+    U = torch.zeros((X.shape[0], 3), dtype=dt, device=DEV)
+    plan.spread(0, F, X, mode=ia.OVERWRITE)
+    plan.interp(1, U, None)
+    torch.cuda.synchronize()
+    f_ref, U_ref = [a.clone() for a in f], U.clone()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            plan.spread(0, F, X, mode=ia.OVERWRITE, stream=side)
+            plan.interp(1, U, None, stream=side)
+    torch.cuda.current_stream().wait_stream(side)
+    for rep in range(3):
+        for a in f:
+            a.fill_(7.0)
+        U.fill_(7.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(f, f_ref):
+            assert torch.equal(a, b), rep
+        assert torch.equal(U, U_ref), rep
