@@ -505,7 +505,7 @@ __device__ __forceinline__ int interp_owner(const InterpArgs<T>& a, int k, const
                    ? 0
                    : -1;
     }
-    const int key = a.keys_sorted[k];
+    const int key = a.keys_sorted[k] >> a.ksub;
     for (int j = a.bin_patch_start[key]; j < a.bin_patch_start[key + 1]; ++j)
     {
         const PatchDev& Q = a.patches[a.bin_patches[j]];

@@ -175,6 +175,7 @@ struct ibamr_plan
     int nsb[3] = {1, 1, 1};
     int nbins = 1;
     int nmem = 1;           // bin memory slots (>= nbins)
+    int ksub = 0;           // bits of the cell-in-bin part of the binning keys
     int npatches = 0;
     std::vector<PatchDev> patches_h;
     int max_points = 0;
@@ -243,6 +244,7 @@ struct BinArgs
     int nsb[3];
     int bin;
     int nmem;
+    int ksub;   // bits of the cell-in-bin part of a key (0: bins only)
     int* keys;
     int4* cell;
 };
@@ -270,13 +272,17 @@ __global__ void k_bin_keys(BinArgs a)
         inside = inside && (v >= (double)a.sel_lo[d]) && (v <= (double)a.sel_hi[d]);
         c[d] = inside ? (int)v : a.dom_lo[d];
     }
-    int key = a.nmem;  // sentinel: sorts after every bin
+    int key = a.nmem << a.ksub;  // sentinel: sorts after every bin
     if (inside)
     {
-        int b[3] = {0, 0, 0};
+        // bin, then the cell inside the bin (z, y, x; x fastest), so that a bin's
+        // points are cell-sorted whatever the input order
+        int b[3] = {0, 0, 0}, l = 0;
 #pragma unroll
         for (int d = 0; d < NDIM; ++d) b[d] = (c[d] - a.dom_lo[d]) / a.bin;
-        key = bin_mem(NDIM, b[0], b[1], b[2], a.nb, a.nsb);
+#pragma unroll
+        for (int d = NDIM - 1; d >= 0; --d) l = l * a.bin + (c[d] - a.dom_lo[d] - b[d] * a.bin);
+        key = (bin_mem(NDIM, b[0], b[1], b[2], a.nb, a.nsb) << a.ksub) | (a.ksub ? l : 0);
     }
     a.keys[p] = key;
     a.cell[p] = make_int4(c[0], c[1], c[2], 0);
@@ -470,7 +476,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const int* __restrict
 // start[b] = first sorted position whose key is >= b, for b in [0, m] (keys in
 // [0, m)): one binary search per bin, so the work per thread stays bounded
 // however the points are spread over the bins.
-__global__ void k_bin_bounds(const int* __restrict__ keys_sorted, int n, int m, int* __restrict__ start)
+__global__ void k_bin_bounds(const int* __restrict__ keys_sorted, int n, int m, int ksub, int* __restrict__ start)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b > m) return;
@@ -478,7 +484,7 @@ __global__ void k_bin_bounds(const int* __restrict__ keys_sorted, int n, int m, 
     while (lo < hi)
     {
         const int mid = (lo + hi) >> 1;
-        if (keys_sorted[mid] < b)
+        if ((keys_sorted[mid] >> ksub) < b)
             lo = mid + 1;
         else
             hi = mid;
@@ -942,6 +948,7 @@ struct InterpArgs
     const int4* cells;
     const int* keys_sorted;
     const int* bin_start;
+    int ksub;          // keys_sorted >> ksub is the bin
     int n;
     double dx[3];
     int gcw[3];
@@ -990,7 +997,7 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a)
     constexpr bool FP64 = sizeof(T) == 8;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.n) return;
-    const int key = a.keys_sorted[k];
+    const int key = a.keys_sorted[k] >> a.ksub;
     const int4 c4 = a.cells[k];
     const int cc[3] = {c4.x, c4.y, c4.z};
     // owning patch: the one whose interior holds the cell (LDataManager.cpp:748-753)
@@ -2255,6 +2262,13 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     P->nbins = (int)nbins_l;
     for (int d = 0; d < 3; ++d) P->nsb[d] = (P->nb[d] + 1) / 2;
     P->nmem = ndim == 3 ? P->nsb[0] * P->nsb[1] * P->nsb[2] * 8 : P->nbins;
+    {
+        // keys carry the cell inside the bin when they fit in 31 bits
+        int lb = 0;
+        while ((1 << lb) < P->bin) ++lb;
+        const int sub = ndim * lb;
+        P->ksub = ((long long)(P->nmem + 1) << sub) < (1LL << 31) && (1 << lb) == P->bin ? sub : 0;
+    }
     P->npatches = L->npatches;
     P->max_points = C->max_points;
 
@@ -2501,6 +2515,7 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     }
     a.bin = P->bin;
     a.nmem = P->nmem;
+    a.ksub = P->ksub;
     a.keys = P->keys_d;
     a.cell = P->cell_d;
     const int G = (npoints + B - 1) / B;
@@ -2511,12 +2526,12 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     HIP_OK(hipGetLastError());
     // stable radix sort of keys [0, nmem] over the bits they use, into (keys_sorted_d, perm_d)
     {
-        const int rc = radix_sort_pairs(P, P->keys_d, nullptr, npoints, bits_for(P->nmem), P->keys_sorted_d,
+        const int rc = radix_sort_pairs(P, P->keys_d, nullptr, npoints, bits_for(P->nmem << P->ksub), P->keys_sorted_d,
                                         P->perm_d, P->keys2_d, P->idx_d, s);
         if (rc != IBAMR_OK) return rc;
     }
     hipLaunchKernelGGL(k_bin_bounds, dim3((m + 1 + B - 1) / B), dim3(B), 0, s, P->keys_sorted_d, npoints, m,
-                       P->bin_start_d);
+                       P->ksub, P->bin_start_d);
     HIP_OK(hipGetLastError());
     // idx_d is free once the sort is done: it now holds the inverse permutation
     hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d,
@@ -2813,6 +2828,7 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
         a.cells = P->cell_sorted_d;
         a.keys_sorted = P->keys_sorted_d;
         a.bin_start = P->bin_start_d;
+        a.ksub = P->ksub;
         a.n = P->npoints;
         for (int d = 0; d < 3; ++d)
         {

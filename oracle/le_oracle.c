@@ -541,7 +541,7 @@ void orc_spread_side(int kernel, int ndim, const double* dx, const double* x_low
 
 typedef struct
 {
-    int key[7];
+    int key[8];
     int point;
     int off[3];
 } entry_t;
@@ -550,7 +550,7 @@ static int cmp_entry(const void* a, const void* b)
 {
     const entry_t* x = (const entry_t*)a;
     const entry_t* y = (const entry_t*)b;
-    for (int k = 0; k < 7; ++k)
+    for (int k = 0; k < 8; ++k)
     {
         if (x->key[k] < y->key[k]) return -1;
         if (x->key[k] > y->key[k]) return 1;
@@ -559,6 +559,19 @@ static int cmp_entry(const void* a, const void* b)
 }
 
 static int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+/* The cell's position inside its bin, z-y-x order with x fastest: the library's
+ * binning sorts a bin's points by it, then by index. */
+static int cell_in_bin(int ndim, const int* c, const int* domain_lower, const int* bin)
+{
+    int l = 0;
+    for (int d = ndim - 1; d >= 0; --d)
+    {
+        const int r = c[d] - domain_lower[d] - floor_div(c[d] - domain_lower[d], bin[d]) * bin[d];
+        l = l * bin[d] + r;
+    }
+    return l;
+}
 
 int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* ilower, const int* iupper,
                             int idx_gcw, const int* periodic_shift, const int* domain_lower, const double* dx,
@@ -612,7 +625,8 @@ int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* i
                         en->key[4] = b[0];
                         en->key[5] = o[0];
                     }
-                    en->key[6] = p;
+                    en->key[6] = cell_in_bin(ndim, c, domain_lower, bin);
+                    en->key[7] = p;
                     en->point = p;
                     for (int d = 0; d < 3; ++d) en->off[d] = o[d];
                 }
@@ -651,16 +665,29 @@ static int cmp_zkey(const void* a, const void* b)
     return 0;
 }
 
-void orc_order_entries_z(int kernel, const int* cell, const int* indices, const double* Xshift, int nentries,
-                         const double* X, const double* x_lower, const double* dx, const int* domain_lower,
-                         const int* bin, int chunk, int* perm)
+typedef struct
+{
+    long bin;
+    int l, q;
+} rkey_t;
+
+static int cmp_rkey_bin(const void* a, const void* b)
+{
+    const rkey_t* x = (const rkey_t*)a;
+    const rkey_t* y = (const rkey_t*)b;
+    if (x->bin != y->bin) return x->bin < y->bin ? -1 : 1;
+    if (x->l != y->l) return x->l < y->l ? -1 : 1;
+    return x->q < y->q ? -1 : (x->q > y->q);
+}
+
+void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* indices, const double* Xshift,
+                         int nentries, const double* X, const double* x_lower, const double* dx,
+                         const int* domain_lower, const int* bin, int chunk, int* perm)
 {
     zkey_t* z = (zkey_t*)malloc(sizeof(zkey_t) * (size_t)(nentries > 0 ? nentries : 1));
-    /* rank of every point among the points of its bin in index order (the bin
-     * stores them in index order and the GPU cuts that whole list into chunks):
-     * one counting pass over points 0 .. max index */
-    int maxp = 0;
-    for (int l = 0; l < nentries; ++l) maxp = indices[l] + 1 > maxp ? indices[l] + 1 : maxp;
+    /* rank of every point among the points of its bin in the library's bin order
+     * (cell inside the bin, then index; the GPU cuts that whole list into chunks) */
+    const int maxp = npoints;
     int bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
     for (int q = 0; q < maxp; ++q)
         for (int d = 0; d < 3; ++d)
@@ -670,7 +697,7 @@ void orc_order_entries_z(int kernel, const int* cell, const int* indices, const 
             if (q == 0 || bq > bmax[d]) bmax[d] = bq;
         }
     const long nbx = bmax[0] - bmin[0] + 1, nby = bmax[1] - bmin[1] + 1, nbz = bmax[2] - bmin[2] + 1;
-    long* count = (long*)calloc((size_t)(maxp > 0 ? nbx * nby * nbz : 1), sizeof(long));
+    rkey_t* rk = (rkey_t*)malloc(sizeof(rkey_t) * (size_t)(maxp > 0 ? maxp : 1));
     long* rank_of = (long*)malloc(sizeof(long) * (size_t)(maxp > 0 ? maxp : 1));
     for (int q = 0; q < maxp; ++q)
     {
@@ -680,8 +707,17 @@ void orc_order_entries_z(int kernel, const int* cell, const int* indices, const 
             const long nbd = d == 0 ? nbx : (d == 1 ? nby : nbz);
             id = id * nbd + (floor_div(cell[3 * q + d] - domain_lower[d], bin[d]) - bmin[d]);
         }
-        rank_of[q] = count[id]++;
+        rk[q].bin = id;
+        rk[q].l = cell_in_bin(3, cell + 3 * q, domain_lower, bin);
+        rk[q].q = q;
     }
+    qsort(rk, (size_t)maxp, sizeof(rkey_t), cmp_rkey_bin);
+    for (int i = 0, run = 0; i < maxp; ++i)
+    {
+        run = (i > 0 && rk[i].bin == rk[i - 1].bin) ? run + 1 : 0;
+        rank_of[rk[i].q] = run;
+    }
+    free(rk);
     long group = -1;
     int prev_b[3] = {0, 0, 0}, prev_o[3] = {0, 0, 0};
     for (int l = 0; l < nentries; ++l)
@@ -712,13 +748,12 @@ void orc_order_entries_z(int kernel, const int* cell, const int* indices, const 
         z[l].key[0] = group;
         z[l].key[1] = rank / chunk;
         z[l].key[2] = orc_weights_1d_u(kernel, Xs, X[3 * p + 2], x_lower[2], dx[2], w);
-        z[l].key[3] = p;
+        z[l].key[3] = rank;  /* lane order inside a chunk = bin order */
         z[l].pos = l;
     }
     qsort(z, (size_t)nentries, sizeof(zkey_t), cmp_zkey);
     for (int l = 0; l < nentries; ++l) perm[l] = z[l].pos;
     free(rank_of);
-    free(count);
     free(z);
 }
 

@@ -131,11 +131,12 @@ int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* i
  * index along z for this array (x_lower is the array's lower corner, i.e. the
  * patch x_lower shifted by -dx/2 along the side axis), then by point.  Any
  * order gives the same sum to round-off; this one makes the FP64 GPU result
- * bitwise comparable.  `cell` holds the level cell of every point; perm
+ * bitwise comparable.  `cell` holds the level cell of every point (npoints of
+ * them: a bin's points are ranked by cell inside the bin, then index); perm
  * receives the new order as positions into indices[]. */
-void orc_order_entries_z(int kernel, const int* cell, const int* indices, const double* Xshift, int nentries,
-                         const double* X, const double* x_lower, const double* dx, const int* domain_lower,
-                         const int* bin, int chunk, int* perm);
+void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* indices, const double* Xshift,
+                         int nentries, const double* X, const double* x_lower, const double* dx,
+                         const int* domain_lower, const int* bin, int chunk, int* perm);
 
 /* Brute-force O(N*G) evaluation with the scalar delta functions: value of the
  * spread of point values V (depth 1) at one grid location, periodic images

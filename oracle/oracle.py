@@ -58,7 +58,7 @@ def lib():
                                       ctypes.POINTER(_dp), _ip, _dp, ctypes.c_int, _dp, _dp]
         L.orc_build_patch_entries.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip,
                                               _ip, _dp, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp]
-        L.orc_order_entries_z.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, _dp, _dp, _dp, _ip, _ip,
+        L.orc_order_entries_z.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, _dp, _dp, _ip, _ip,
                                           ctypes.c_int, _ip]
         L.orc_bruteforce_spread_at.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
                                                ctypes.c_int]
@@ -257,7 +257,7 @@ def order_entries_z(kernel, cell, indices, Xshift, X, x_lower, dx, domain_lower,
     Xshift = _dv(Xshift).reshape(-1, 3)
     X = _dv(X).reshape(-1, 3)
     perm = np.zeros(len(indices), dtype=np.int32)
-    lib().orc_order_entries_z(kernel_id(kernel), _i(cell), _i(indices), _d(Xshift), len(indices), _d(X),
+    lib().orc_order_entries_z(kernel_id(kernel), _i(cell), cell.shape[0], _i(indices), _d(Xshift), len(indices), _d(X),
                               _d(_dv(x_lower)), _d(_dv(dx)), _i(_iv(domain_lower)), _i(_iv(bin_size)), int(chunk),
                               _i(perm))
     return perm

@@ -266,8 +266,10 @@ def test_side_axis_is_shifted_cell_kernel(orc):
 
 def test_order_entries_z_matches_definition(orc):
     """The spread order of the oracle (group, rank-in-bin // 64, stencil z start,
-    point) against a direct O(n^2) evaluation of that definition, with bins
-    crowded past one 64-point chunk and periodic images."""
+    rank-in-bin) against a direct O(n^2) evaluation of that definition, with bins
+    crowded past one 64-point chunk and periodic images. A point's rank in its
+    bin counts the bin's points that come before it by (cell inside the bin in
+    z, y, x order with x fastest, point index)."""
     rng = np.random.default_rng(41)
     n, N, h, kernel = 700, 16, 1.0 / 16, "IB_6"
     X = rng.uniform(0.0, 1.0, (n, 3))
@@ -278,6 +280,8 @@ def test_order_entries_z_matches_definition(orc):
     xl = [0.0, 0.0, -0.5 * h]
     perm = orc.order_entries_z(kernel, cells, idx, sh, X, xl, [h] * 3, il, [8] * 3)
     b = cells // 8
+    r = cells - 8 * b
+    cib = (r[:, 2] * 8 + r[:, 1]) * 8 + r[:, 0]
     keys = []
     group, prev = -1, None
     for l, p in enumerate(idx):
@@ -286,8 +290,10 @@ def test_order_entries_z_matches_definition(orc):
         if g != prev:
             group += 1
             prev = g
-        rank = int(np.sum(np.all(b[:p] == b[p], axis=1)))
+        same = np.all(b == b[p], axis=1)
+        before = (cib < cib[p]) | ((cib == cib[p]) & (np.arange(n) < p))
+        rank = int(np.sum(same & before))
         z0 = orc.weights_1d(kernel, X[p, 2] + sh[l, 2], xl[2], h)[0]
-        keys.append((group, rank // 64, z0, p, l))
+        keys.append((group, rank // 64, z0, rank, l))
     want = [k[-1] for k in sorted(keys)]
     assert list(perm) == want
