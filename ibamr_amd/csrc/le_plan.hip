@@ -2524,7 +2524,7 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     else
         hipLaunchKernelGGL(k_bin_keys<2>, dim3(G), dim3(B), 0, s, a);
     HIP_OK(hipGetLastError());
-    // stable radix sort of keys [0, nmem] over the bits they use, into (keys_sorted_d, perm_d)
+    // stable radix sort of keys [0, nmem << ksub] over the bits they use, into (keys_sorted_d, perm_d)
     {
         const int rc = radix_sort_pairs(P, P->keys_d, nullptr, npoints, bits_for(P->nmem << P->ksub), P->keys_sorted_d,
                                         P->perm_d, P->keys2_d, P->idx_d, s);
