@@ -175,7 +175,7 @@ struct ibamr_plan
     int nsb[3] = {1, 1, 1};
     int nbins = 1;
     int nmem = 1;           // bin memory slots (>= nbins)
-    int ksub = 0;           // bits of the cell-in-bin part of the binning keys
+    int ksub = 0;           // bits of the plane-in-bin part of the binning keys
     int npatches = 0;
     std::vector<PatchDev> patches_h;
     int max_points = 0;
@@ -244,7 +244,7 @@ struct BinArgs
     int nsb[3];
     int bin;
     int nmem;
-    int ksub;   // bits of the cell-in-bin part of a key (0: bins only)
+    int ksub;   // bits of the plane-in-bin part of a key (0: bins only)
     int* keys;
     int4* cell;
 };
@@ -275,13 +275,13 @@ __global__ void k_bin_keys(BinArgs a)
     int key = a.nmem << a.ksub;  // sentinel: sorts after every bin
     if (inside)
     {
-        // bin, then the cell inside the bin (z, y, x; x fastest), so that a bin's
-        // points are cell-sorted whatever the input order
-        int b[3] = {0, 0, 0}, l = 0;
+        // bin, then the cell's plane inside the bin along the last axis (z in
+        // 3-D, y in 2-D), so that a bin's points are sorted by plane whatever the
+        // input order and a 64-point chunk of a crowded bin spans few z planes
+        int b[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < NDIM; ++d) b[d] = (c[d] - a.dom_lo[d]) / a.bin;
-#pragma unroll
-        for (int d = NDIM - 1; d >= 0; --d) l = l * a.bin + (c[d] - a.dom_lo[d] - b[d] * a.bin);
+        const int l = c[NDIM - 1] - a.dom_lo[NDIM - 1] - b[NDIM - 1] * a.bin;
         key = (bin_mem(NDIM, b[0], b[1], b[2], a.nb, a.nsb) << a.ksub) | (a.ksub ? l : 0);
     }
     a.keys[p] = key;
@@ -2263,10 +2263,10 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     for (int d = 0; d < 3; ++d) P->nsb[d] = (P->nb[d] + 1) / 2;
     P->nmem = ndim == 3 ? P->nsb[0] * P->nsb[1] * P->nsb[2] * 8 : P->nbins;
     {
-        // keys carry the cell inside the bin when they fit in 31 bits
+        // keys carry the cell's plane inside the bin when they fit in 31 bits
         int lb = 0;
         while ((1 << lb) < P->bin) ++lb;
-        const int sub = ndim * lb;
+        const int sub = lb;
         P->ksub = ((long long)(P->nmem + 1) << sub) < (1LL << 31) && (1 << lb) == P->bin ? sub : 0;
     }
     P->npatches = L->npatches;

@@ -560,17 +560,12 @@ static int cmp_entry(const void* a, const void* b)
 
 static int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
-/* The cell's position inside its bin, z-y-x order with x fastest: the library's
- * binning sorts a bin's points by it, then by index. */
-static int cell_in_bin(int ndim, const int* c, const int* domain_lower, const int* bin)
+/* The cell's plane inside its bin along the last axis (z in 3-D, y in 2-D):
+ * the library's binning sorts a bin's points by it, then by index. */
+static int plane_in_bin(int ndim, const int* c, const int* domain_lower, const int* bin)
 {
-    int l = 0;
-    for (int d = ndim - 1; d >= 0; --d)
-    {
-        const int r = c[d] - domain_lower[d] - floor_div(c[d] - domain_lower[d], bin[d]) * bin[d];
-        l = l * bin[d] + r;
-    }
-    return l;
+    const int d = ndim - 1;
+    return c[d] - domain_lower[d] - floor_div(c[d] - domain_lower[d], bin[d]) * bin[d];
 }
 
 int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* ilower, const int* iupper,
@@ -625,7 +620,7 @@ int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* i
                         en->key[4] = b[0];
                         en->key[5] = o[0];
                     }
-                    en->key[6] = cell_in_bin(ndim, c, domain_lower, bin);
+                    en->key[6] = plane_in_bin(ndim, c, domain_lower, bin);
                     en->key[7] = p;
                     en->point = p;
                     for (int d = 0; d < 3; ++d) en->off[d] = o[d];
@@ -686,7 +681,7 @@ void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* in
 {
     zkey_t* z = (zkey_t*)malloc(sizeof(zkey_t) * (size_t)(nentries > 0 ? nentries : 1));
     /* rank of every point among the points of its bin in the library's bin order
-     * (cell inside the bin, then index; the GPU cuts that whole list into chunks) */
+     * (cell's z plane inside the bin, then index; the GPU cuts that whole list into chunks) */
     const int maxp = npoints;
     int bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
     for (int q = 0; q < maxp; ++q)
@@ -708,7 +703,7 @@ void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* in
             id = id * nbd + (floor_div(cell[3 * q + d] - domain_lower[d], bin[d]) - bmin[d]);
         }
         rk[q].bin = id;
-        rk[q].l = cell_in_bin(3, cell + 3 * q, domain_lower, bin);
+        rk[q].l = plane_in_bin(3, cell + 3 * q, domain_lower, bin);
         rk[q].q = q;
     }
     qsort(rk, (size_t)maxp, sizeof(rkey_t), cmp_rkey_bin);

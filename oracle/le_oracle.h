@@ -132,7 +132,7 @@ int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* i
  * patch x_lower shifted by -dx/2 along the side axis), then by point.  Any
  * order gives the same sum to round-off; this one makes the FP64 GPU result
  * bitwise comparable.  `cell` holds the level cell of every point (npoints of
- * them: a bin's points are ranked by cell inside the bin, then index); perm
+ * them: a bin's points are ranked by the cell's z plane inside the bin, then index); perm
  * receives the new order as positions into indices[]. */
 void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* indices, const double* Xshift,
                          int nentries, const double* X, const double* x_lower, const double* dx,

@@ -268,8 +268,8 @@ def test_order_entries_z_matches_definition(orc):
     """The spread order of the oracle (group, rank-in-bin // 64, stencil z start,
     rank-in-bin) against a direct O(n^2) evaluation of that definition, with bins
     crowded past one 64-point chunk and periodic images. A point's rank in its
-    bin counts the bin's points that come before it by (cell inside the bin in
-    z, y, x order with x fastest, point index)."""
+    bin counts the bin's points that come before it by (the cell's z plane
+    inside the bin, point index)."""
     rng = np.random.default_rng(41)
     n, N, h, kernel = 700, 16, 1.0 / 16, "IB_6"
     X = rng.uniform(0.0, 1.0, (n, 3))
@@ -281,7 +281,7 @@ def test_order_entries_z_matches_definition(orc):
     perm = orc.order_entries_z(kernel, cells, idx, sh, X, xl, [h] * 3, il, [8] * 3)
     b = cells // 8
     r = cells - 8 * b
-    cib = (r[:, 2] * 8 + r[:, 1]) * 8 + r[:, 0]
+    cib = r[:, 2]
     keys = []
     group, prev = -1, None
     for l, p in enumerate(idx):
