@@ -401,6 +401,7 @@ def main_amr(args, torch, ia):
         "detail": {"spread_kernel_ms": r32["kt"]["spread_ms"], "interp_kernel_ms": r32["kt"]["interp_ms"],
                    "fp64_points_per_s": n / (r64["ms"] * 1e-3), "fp64_spread_kernel_ms": r64["kt"]["spread_ms"],
                    "fp64_interp_kernel_ms": r64["kt"]["interp_ms"], "fp32c_over_fp64": r64["ms"] / r32["ms"],
+                   "fp64_tile_autotune_ms": {str(h): t for h, t in r64["tune"].items()},
                    "fp32c_rel_linf_err_spread": err_f, "fp32c_rel_linf_err_interp": err_u},
         "roofline": None, "cpu_baseline": None,
     }
@@ -605,6 +606,7 @@ def main():
         err_u = float((r32["U"].double() - r64["U"]).abs().max() / r64["U"].abs().max())
         out["detail"].update({"fp64_points_per_s": n / (ms64 * 1e-3), "fp64_spread_ms": r64["spread_ms"],
                               "fp64_interp_ms": r64["interp_ms"], "fp32c_over_fp64": ms64 / ms32,
+                              "fp64_tile_autotune_ms": {str(h): t for h, t in r64["tune"].items()},
                               "fp32c_rel_linf_err_spread": err_f, "fp32c_rel_linf_err_interp": err_u})
         del r64
     if not args.no_cpu:
