@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=400000, help="points of the one-core CPU baseline")
     ap.add_argument("--cpu-cores", type=int, default=16,
                     help="threads of the all-cores CPU baseline (capped by the host's cores; 0: skip it)")
-    ap.add_argument("--cpu-sample-cores", type=int, default=2000000, help="points of the all-cores CPU baseline")
+    ap.add_argument("--cpu-sample-cores", type=int, default=8388608,
+                    help="points of the all-cores CPU baseline (at most the workload's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--point-order", choices=["regrid", "generated"], default="regrid",
@@ -282,9 +283,10 @@ def cpu_baseline_cores(X_np, F_np, N, kernel, sample, threads, seed=6):
         list(ex.map(work, slabs))
         t = time.perf_counter() - t0
     return {"value": X.shape[0] / t, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": "%d of the %d workload points (uniform random subset), full %d^3 side grid cut into %d z-slab "
+            "sample": "%d of the %d workload points (%s), full %d^3 side grid cut into %d z-slab "
                       "patches, one thread each, FP64 spread+interp, %.2f s"
-                      % (X.shape[0], X_np.shape[0], N, threads, t)}
+                      % (X.shape[0], X_np.shape[0],
+                         "all" if X.shape[0] == X_np.shape[0] else "uniform random subset", N, threads, t)}
 
 
 def cpu_baseline(X_np, F_np, N, kernel, sample, seed=5):
