@@ -476,6 +476,7 @@ def main_multi(args, torch, ia):
         X0, F64 = torch.cat([Xa, Xb]).contiguous(), torch.cat([Fa, Fb]).contiguous()
     res = {}
     dt = None
+    tunes = {}
     for prec, dt_ in ((ia.FP64, torch.float64), (ia.FP32C, torch.float32)):
         gw = [3] * 3
         S = ibstep.StructureSet(level, structs, prec, alpha=0.05 * h ** 3,
@@ -492,6 +493,8 @@ def main_multi(args, torch, ia):
         S.step(X, F, U, dt)   # warm-up before capture
         torch.cuda.synchronize()
         X.copy_(X0)
+        tunes[prec] = S.autotune(X, F)  # spread tile height per kernel group (untimed)
+        torch.cuda.synchronize()
         graph = S.capture(X, F, U, dt)
         torch.cuda.synchronize()
         X.copy_(X0)
@@ -514,8 +517,10 @@ def main_multi(args, torch, ia):
         "config": {"workload": "64 ellipsoidal shells (32 IB_4 + 32 BSPLINE_3, %d points) on a periodic 384^3 "
                                "side grid; step = bin + spread + u=u_bg+alpha*f + ghost fill + interp + X+=dtU, "
                                "one HIP-graph replay per step, %d steps, FP32 compensated" % (n, steps),
-                   "points": n, "grid": N, "structures": nshell, "point_order": args.point_order},
-        "detail": {"fp64_ms_per_step": res[ia.FP64]["ms"], "fp32c_over_fp64": res[ia.FP64]["ms"] / ms,
+                   "points": n, "grid": N, "structures": nshell, "point_order": args.point_order,
+                   "tile_autotune_ms": {k: {str(h): t for h, t in v.items()} for k, v in tunes[ia.FP32C].items()}},
+        "detail": {"fp64_ms_per_step": res[ia.FP64]["ms"],
+                   "fp64_tile_autotune_ms": {k: {str(h): t for h, t in v.items()} for k, v in tunes[ia.FP64].items()}, "fp32c_over_fp64": res[ia.FP64]["ms"] / ms,
                    "dt": res[ia.FP32C]["dt"], "fp32c_vs_fp64_position_drift_cells_after_%d_steps" % steps: drift,
                    "force_generation_ms": forces_ms,
                    "force_generation": "ring springs + beams per shell, a target point every 4th point, "

@@ -84,6 +84,17 @@ class StructureSet:
             p.interp(1, U[a:b], X[a:b], stream=s)
         ia.forward_euler_step(X, X, U, dt, stream=s)
 
+    def autotune(self, X, F, reps=2, stream=None):
+        """Picks each plan's spread tile height for its structures' current
+        positions (LevelPlan.autotune: trial spreads overwrite the force arrays,
+        which the next step overwrites anyway).  Returns {kernel: {height: ms}}.
+        Call it before capture: it rebuilds the plans' work lists."""
+        out = {}
+        for (k, _, _), (p, a, b) in zip(self.groups, self._slices()):
+            p.bin(X[a:b], stream=stream, check=False)
+            out[k] = p.autotune(0, F[a:b], X[a:b], reps=reps, stream=stream)
+        return out
+
     def capture(self, X, F, U, dt, steps_per_graph=1):
         """Captures steps_per_graph steps into one HIP graph (after one eager
         warm-up step has been run by the caller)."""
