@@ -93,6 +93,36 @@ def test_graph_replays_match_eager_bitwise():
         assert torch.equal(X, traj[k]), k
 
 
+def test_autotune_then_graph_matches_untuned_fp64():
+    """StructureSet.autotune times every built tile height per kernel group and
+    keeps the fastest; FP64 spreads are bitwise the same at every height, so a
+    captured step after it gives the trajectory of an untuned set, bit for bit."""
+    S0, X, F, U0, _ = _setup(ia.FP64)
+    S1, _, _, U1, _ = _setup(ia.FP64)
+    X0 = X.clone()
+    dt = 0.25 / N / 0.1
+    X1 = X.clone()
+    S1.step(X1, F, U1, dt)  # warm-up (bins) before autotune and capture
+    torch.cuda.synchronize()
+    X1.copy_(X0)
+    tunes = S1.autotune(X1, F, reps=1)
+    torch.cuda.synchronize()
+    assert sorted(tunes) == ["BSPLINE_3", "IB_4"]
+    for (k, _, _), p in zip(S1.groups, S1.plans):
+        assert sorted(tunes[k]) == [16, 32]
+        assert p.tile_height == min(tunes[k], key=tunes[k].get)
+    g = S1.capture(X1, F, U1, dt)
+    torch.cuda.synchronize()
+    X1.copy_(X0)
+    torch.cuda.synchronize()
+    for k in range(3):
+        S0.step(X, F, U0, dt)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(U1, U0), k
+        assert torch.equal(X1, X), k
+
+
 def test_fp32c_trajectory_tracks_fp64():
     S64, X64, F64t, U64, _ = _setup(ia.FP64)
     S32, X32, F32t, U32, _ = _setup(ia.FP32C)
