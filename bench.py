@@ -68,13 +68,14 @@ def copy_probe(torch):
     print(json.dumps({"copy_probe_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
-def counters_from_profile(kernel):
+def counters_from_profile(kernel, tag):
     """VALU-busy and LDS bank-conflict fractions of a kernel from the committed
-    rocprofv3 counter summary of the 512^3 workload ("Derived" table), or None."""
-    path = os.path.join(ROOT, "profiles", "counters_spread3_interp3_512c_side_ib6_8M_fp32c.md")
+    rocprofv3 counter summary of this workload ("Derived" table: the spread
+    kernel's column, then the interpolation's), or None."""
+    path = os.path.join(ROOT, "profiles", "counters_%s.md" % tag)
     if not os.path.exists(path):
         return None
-    col = 0 if kernel == "k_spread3" else 1
+    col = 0 if kernel.startswith("k_spread") else 1
     out = {}
     with open(path) as fh:
         for line in fh:
@@ -566,7 +567,7 @@ def main():
     B_dom = B_spread if dom == "spread" else B_interp
     t_dom = r32["spread_kernel_ms"] if dom == "spread" else r32["interp_kernel_ms"]
     achieved = B_dom / (t_dom * 1e-3) / 1e9
-    kname = "k_spread3" if dom == "spread" else "k_interp3"
+    kname = ("k_spread3_fxz" if r32["plan"].tile_height >= 32 else "k_spread3_fx") if dom == "spread" else "k_interp3"
     tag = "%s_%dc_side_%s_%dpts_fp32c" % (kname, N, kernel.lower(), n)
     tr = traffic_from_profile(tag)
 
@@ -587,8 +588,8 @@ def main():
                                "r=0.3), FP32 compensated spread+interp" % (N, kernel, n),
                    "grid": N, "points": n, "kernel": kernel, "centering": "side", "precision": "fp32c",
                    "point_order": args.point_order,
-                   "tile": [r32["info"]["tile"], r32["info"]["tile"], r32["plan"].tile_height],
-                   "tile_autotune_ms": {str(h): t for h, t in r32["tune"].items()},
+                   "tile": list(r32["plan"].tile_shape),
+                   "tile_autotune_ms": {"x".join(map(str, h)): t for h, t in r32["tune"].items()},
                    "bins": "%d^3 cells, sorted (CSR: no capacity limit)" % r32["info"]["tile"],
                    "spread_tiles": r32["info"]["ntiles"]},
         "roofline": {"bound": "hbm", "kernel": kname,
@@ -596,7 +597,7 @@ def main():
                      "traffic": tr["bytes_per_dispatch"] if tr else None, "algorithmic_bytes": B_dom,
                      "traffic_source": ("profiles/hbm_traffic_%s.txt" % tag) if tr else None,
                      # what bounds it instead: instruction issue (rocprofv3 counters of this workload)
-                     "counters": counters_from_profile(kname) if N == 512 and kernel == "IB_6" else None},
+                     "counters": counters_from_profile(kname, tag)},
         "detail": {"spread_ms": r32["spread_ms"], "interp_ms": r32["interp_ms"], "bin_ms": r32["bin_ms"],
                    "spread_kernel_ms": r32["spread_kernel_ms"], "interp_kernel_ms": r32["interp_kernel_ms"],
                    "touched_faces": T, "bytes_step": B_spread + B_interp,
@@ -613,7 +614,7 @@ def main():
         err_u = float((r32["U"].double() - r64["U"]).abs().max() / r64["U"].abs().max())
         out["detail"].update({"fp64_points_per_s": n / (ms64 * 1e-3), "fp64_spread_ms": r64["spread_ms"],
                               "fp64_interp_ms": r64["interp_ms"], "fp32c_over_fp64": ms64 / ms32,
-                              "fp64_tile_autotune_ms": {str(h): t for h, t in r64["tune"].items()},
+                              "fp64_tile_autotune_ms": {"x".join(map(str, h)): t for h, t in r64["tune"].items()},
                               "fp32c_rel_linf_err_spread": err_f, "fp32c_rel_linf_err_interp": err_u})
         del r64
     if not args.no_cpu:

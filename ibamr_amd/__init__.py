@@ -43,6 +43,7 @@ C_API = [
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
     "ibamr_plan_info", "ibamr_plan_set_tile_height", "ibamr_plan_get_tile_height", "ibamr_plan_autotune",
+    "ibamr_plan_set_tile_shape", "ibamr_plan_get_tile_shape", "ibamr_plan_get_cell_indices",
     "ibamr_plan_redistribute", "ibamr_permute_rows",
     "ibamr_plan_set_physical_boundary", "ibamr_plan_accumulate_physical_boundary", "ibamr_plan_fill_physical_boundary", "ibamr_fe_quadrature_points",
     "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
@@ -136,6 +137,9 @@ def lib():
                                   ctypes.POINTER(ctypes.c_int)]
     L.ibamr_plan_set_tile_height.argtypes = [vp, ctypes.c_int]
     L.ibamr_plan_get_tile_height.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.ibamr_plan_set_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.ibamr_plan_get_cell_indices.argtypes = [vp, vp, vp]
+    L.ibamr_plan_get_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.ibamr_plan_autotune.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), vp]
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
@@ -302,7 +306,7 @@ class LevelPlan:
 
     @property
     def tile_height(self):
-        """z values per spread tile (3-D; the tile is 8 x 8 grid lines of one array)."""
+        """z extent of the spread tile shape in use (3-D)."""
         h = ctypes.c_int()
         _check(lib().ibamr_plan_get_tile_height(self._h, ctypes.byref(h)))
         return h.value
@@ -311,15 +315,26 @@ class LevelPlan:
     def tile_height(self, h):
         _check(lib().ibamr_plan_set_tile_height(self._h, int(h)))
 
+    @property
+    def tile_shape(self):
+        """(x, y, z) extents of the spread tile shape in use."""
+        sh = (ctypes.c_int * 3)()
+        _check(lib().ibamr_plan_get_tile_shape(self._h, sh))
+        return tuple(sh)
+
+    @tile_shape.setter
+    def tile_shape(self, shape):
+        _check(lib().ibamr_plan_set_tile_shape(self._h, (ctypes.c_int * 3)(*[int(v) for v in shape])))
+
     def autotune(self, slot, F, X, reps=3, stream=None):
         """Time the spread of F at X (binned) into the arrays bound to `slot`
-        (overwritten) with every built tile height and keep the fastest.
-        Returns {height: ms per spread call} (-1: height not admissible)."""
-        n = 8  # IBAMR_MAX_TILE_HEIGHTS
-        hs, ms, nc = (ctypes.c_int * n)(), (ctypes.c_float * n)(), ctypes.c_int()
-        _check(lib().ibamr_plan_autotune(self._h, slot, _ptr(F), _ptr(X), reps, hs, ms, ctypes.byref(nc),
+        (overwritten) with every built tile shape and keep the fastest.
+        Returns {(x, y, z): ms per spread call}."""
+        n = 8  # IBAMR_MAX_TILE_SHAPES
+        sh, ms, nc = (ctypes.c_int * (3 * n))(), (ctypes.c_float * n)(), ctypes.c_int()
+        _check(lib().ibamr_plan_autotune(self._h, slot, _ptr(F), _ptr(X), reps, sh, ms, ctypes.byref(nc),
                                          _stream_ptr(stream)))
-        return {hs[i]: ms[i] for i in range(nc.value)}
+        return {(sh[3 * i], sh[3 * i + 1], sh[3 * i + 2]): ms[i] for i in range(nc.value)}
 
     def bind(self, slot, arrays):
         """arrays: flat list [patch0 axis0, patch0 axis1, ..., patch1 axis0, ...] of device tensors."""
@@ -334,6 +349,14 @@ class LevelPlan:
 
     def check(self, stream=None):
         _check(lib().ibamr_plan_check(self._h, _stream_ptr(stream)))
+
+    def cell_indices(self, n, stream=None):
+        """getCellIndex of each of the n points of the last binning, as the
+        binning computed it: device int32 [n][ndim] (INT_MIN: does not fit)."""
+        import torch
+        out = torch.empty((n, self.level.ndim), dtype=torch.int32, device="cuda")
+        _check(lib().ibamr_plan_get_cell_indices(self._h, _ptr(out), _stream_ptr(stream)))
+        return out
 
     def spread(self, slot, F, X, mode=ACCUMULATE, stream=None):
         _check(lib().ibamr_plan_spread(self._h, slot, _ptr(F), _ptr(X), mode, _stream_ptr(stream)))

@@ -21,7 +21,7 @@ struct BinArgs
     int nmem;
     int ksub;   // bits of the plane-in-bin part of a key (0: bins only)
     int* keys;
-    int4* cell;
+    int4* cell;  // getCellIndex of every point, in the caller's order
 };
 
 // IndexUtilities::getCellIndex (IndexUtilities-inl.h:52-72) on the level domain
@@ -32,20 +32,17 @@ __global__ void k_bin_keys(BinArgs a)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     int c[3] = {0, 0, 0};
+    int raw[3] = {0, 0, 0};
     bool inside = true;
 #pragma unroll
     for (int d = 0; d < NDIM; ++d)
     {
-        const double X = a.X[NDIM * p + d];
-        const double dXl = X - a.xlo[d], dXu = X - a.xhi[d];
-        double v;
-        if (fabs(dXl) <= fabs(dXu))
-            v = (double)a.ref_lo[d] + floor(dXl / a.dx[d]);
-        else
-            v = ((double)a.ref_hi[d] + floor(dXu / a.dx[d])) + 1.0;
+        const double v = cell_index_d(a.X[NDIM * p + d], a.xlo[d], a.xhi[d], a.dx[d], a.ref_lo[d], a.ref_hi[d]);
         // NaN or far outside: not selected
         inside = inside && (v >= (double)a.sel_lo[d]) && (v <= (double)a.sel_hi[d]);
         c[d] = inside ? (int)v : a.dom_lo[d];
+        // the index itself where an int holds it (INT_MIN for NaN or beyond 2^30)
+        raw[d] = (v >= -1073741824.0 && v <= 1073741824.0) ? (int)v : INT_MIN;
     }
     int key = a.nmem << a.ksub;  // sentinel: sorts after every bin
     if (inside)
@@ -60,7 +57,7 @@ __global__ void k_bin_keys(BinArgs a)
         key = (bin_mem(NDIM, b[0], b[1], b[2], a.nb, a.nsb) << a.ksub) | (a.ksub ? l : 0);
     }
     a.keys[p] = key;
-    a.cell[p] = make_int4(c[0], c[1], c[2], 0);
+    a.cell[p] = make_int4(raw[0], raw[1], raw[2], 0);
 }
 
 // Exclusive scan of count[0..m) into start[0..m] (start[m] = total), three passes.
@@ -248,6 +245,37 @@ __global__ void k_bin_bounds(const int* __restrict__ keys_sorted, int n, int m, 
     start[b] = lo;
 }
 
+// fx[0] = the largest number of points in one bin (bins [0, m)), for the
+// fixed-point scale of the FP32C spread; order-independent (integer atomicMax)
+__global__ __launch_bounds__(256) void k_max_bin_count(const int* __restrict__ start, int m, unsigned* __restrict__ fx)
+{
+    __shared__ int s_m[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int c = 0;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < m; b += gridDim.x * blockDim.x)
+        c = max(c, start[b + 1] - start[b]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off));
+    if (lane == 0) s_m[wv] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        const int v = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+        if (v > 0) atomicMax(fx, (unsigned)v);
+    }
+}
+
+// out[ndim * p + d] = coordinate d of point p's cell (caller's order)
+__global__ void k_cells_out(const int4* __restrict__ cell, int n, int ndim, int* __restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int4 c = cell[p];
+    out[(size_t)ndim * p] = c.x;
+    out[(size_t)ndim * p + 1] = c.y;
+    if (ndim == 3) out[(size_t)ndim * p + 2] = c.z;
+}
+
 // cells in bin order, and the inverse permutation (slot of every point)
 __global__ void k_gather_cells(const int* __restrict__ perm, const int4* __restrict__ cell, int n,
                                int4* __restrict__ out, int* __restrict__ inv)
@@ -311,12 +339,12 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     hipStream_t s = static_cast<hipStream_t>(stream);
     P->npoints = npoints;
     P->xs_valid = false;
-    P->ws_valid = false;
     const int B = 256;
     const int m = P->nmem + 1;  // bins plus the out-of-domain sentinel
+    if (P->fx_d)  // FP32C 3-D: the largest bin count, set below (fill kernels, not memsets: graph-safe)
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(P->fx_d), 1, 0);
     if (npoints == 0)
     {
-        // a fill kernel, not a memset: safe to capture in a HIP graph
         hipLaunchKernelGGL(k_fill_int, dim3((m + 1 + B - 1) / B), dim3(B), 0, s, P->bin_start_d, m + 1, 0);
         HIP_OK(hipGetLastError());
         return IBAMR_OK;
@@ -358,6 +386,12 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     hipLaunchKernelGGL(k_bin_bounds, dim3((m + 1 + B - 1) / B), dim3(B), 0, s, P->keys_sorted_d, npoints, m,
                        P->ksub, P->bin_start_d);
     HIP_OK(hipGetLastError());
+    if (P->fx_d)
+    {
+        hipLaunchKernelGGL(k_max_bin_count, dim3(std::min((P->nmem + B - 1) / B, 1024)), dim3(B), 0, s,
+                           P->bin_start_d, P->nmem, P->fx_d);
+        HIP_OK(hipGetLastError());
+    }
     // idx_d is free once the sort is done: it now holds the inverse permutation
     hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d,
                        P->idx_d);
@@ -376,6 +410,18 @@ int ibamr_plan_check(ibamr_plan* P, void* stream)
     if (first_sentinel != P->npoints)
         return fail(IBAMR_ERR_OUT_OF_DOMAIN, std::to_string(P->npoints - first_sentinel) +
                                                  " point(s) outside the level domain (or NaN positions)");
+    return IBAMR_OK;
+}
+
+int ibamr_plan_get_cell_indices(const ibamr_plan* P, int* cells, void* stream)
+{
+    if (!P || !cells) return fail(IBAMR_ERR_INVALID, "null argument");
+    if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
+    if (P->npoints == 0) return IBAMR_OK;
+    const int B = 256;
+    hipLaunchKernelGGL(k_cells_out, dim3((P->npoints + B - 1) / B), dim3(B), 0, static_cast<hipStream_t>(stream),
+                       P->cell_d, P->npoints, P->ndim, cells);
+    HIP_OK(hipGetLastError());
     return IBAMR_OK;
 }
 

@@ -154,7 +154,8 @@ int ibamr_plan_destroy(ibamr_plan* P)
     for (auto& v : P->tev)
         for (hipEvent_t e : v) (void)hipEventDestroy(e);
     dfree(P->patches_d);
-    dfree(P->tiles_d);
+    for (TileList& t : P->tlists) dfree(t.d);
+    dfree(P->fx_d);
     dfree(P->bin_patch_start_d);
     dfree(P->fill_segs_d);
     dfree(P->bin_patches_d);
@@ -167,8 +168,6 @@ int ibamr_plan_destroy(ibamr_plan* P)
     dfree(P->cell_sorted_d);
     dfree(P->Xs_d);
     dfree(P->Fs_d);
-    dfree(P->wt_d);
-    dfree(P->st_d);
     dfree(P->bin_start_d);
     dfree(P->keys2_d);
     dfree(P->pcell_off_d);
@@ -250,17 +249,6 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
         return fail(IBAMR_ERR_INVALID, "unsupported tile edge");
     }
     P->bin = P->tile;
-    P->tsz[0] = P->tile;
-    P->tsz[1] = P->tile;
-    P->tsz[2] = ndim == 3 ? (C->precision == IBAMR_FP64 ? S3_ZC_FP64 : S3_ZC_FP32) : 1;
-    // per-axis (bin, image) lists of a tile hold at most 16 entries: the image
-    // ranges partition the reachable cells, plus one split per domain side
-    for (int d = 0; d < ndim; ++d)
-        if ((P->tsz[d] + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16)
-        {
-            delete P;
-            return fail(IBAMR_ERR_INVALID, "drift_cells too large for the tile size");
-        }
     for (int d = 0; d < 3; ++d)
     {
         const bool on = d < ndim;
@@ -374,7 +362,9 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     PLAN_HIP(hipMalloc(&P->patches_d, sizeof(PatchDev) * P->patches_h.size()));
     PLAN_HIP(hipMemcpy(P->patches_d, P->patches_h.data(), sizeof(PatchDev) * P->patches_h.size(), hipMemcpyHostToDevice));
     {
-        const int rc = build_tiles(P);
+        // the spread's work lists of every admissible tile shape (a shape change
+        // later only selects another list)
+        const int rc = build_tile_lists(P);
         if (rc != IBAMR_OK) return cleanup_fail(rc);
     }
     PLAN_HIP(hipMalloc(&P->bin_patch_start_d, sizeof(int) * bps.size()));
@@ -395,10 +385,9 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     PLAN_HIP(hipMalloc(&P->Fs_d, (C->precision == IBAMR_FP64 ? 8 : 4) * (size_t)P->fdepth * np));
     if (ndim == 3 && C->precision != IBAMR_FP64)
     {
-        P->nvar = C->centering == IBAMR_SIDE ? 2 : 1;
-        const size_t rows = (size_t)3 * P->nvar * np;
-        PLAN_HIP(hipMalloc(&P->wt_d, sizeof(float) * kernel_width(k) * rows));
-        PLAN_HIP(hipMalloc(&P->st_d, sizeof(int) * rows));
+        // fixed-point bound of the FP32C spread: largest bin count, max |F| per component
+        PLAN_HIP(hipMalloc(&P->fx_d, sizeof(unsigned) * (1 + (size_t)P->fdepth)));
+        PLAN_HIP(hipMemset(P->fx_d, 0, sizeof(unsigned) * (1 + (size_t)P->fdepth)));
     }
     PLAN_HIP(hipMalloc(&P->bin_start_d, sizeof(int) * (P->nmem + 2)));
     {
@@ -471,7 +460,7 @@ int ibamr_plan_create_patch(ibamr_plan** out, int ndim, const double* dx, const 
 int ibamr_plan_info(const ibamr_plan* P, int* ntiles, int* tile, int* nbins)
 {
     if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
-    if (ntiles) *ntiles = P->ntiles;
+    if (ntiles) *ntiles = P->tlists.empty() ? 0 : P->tlists[P->tcur].n;
     if (tile) *tile = P->tile;
     if (nbins) *nbins = P->nbins;
     return IBAMR_OK;

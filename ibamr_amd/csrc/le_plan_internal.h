@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <climits>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -55,6 +56,16 @@ struct TileDesc
     int t[3];   // tile coordinates inside the array's ghost box
 };
 
+// The spread's work list for one tile shape: every tile of every array of every
+// patch, in XCD-aware launch order (built at plan creation, kept until the plan
+// is destroyed, so a HIP graph captured with any shape stays valid).
+struct TileList
+{
+    TileDesc* d = nullptr;
+    int n = 0;
+    int shape[3] = {1, 1, 1};
+};
+
 
 // radix sort and scan shapes (le_bin.hip; plan buffers are sized from them)
 constexpr int RS_BITS = 8;
@@ -65,9 +76,19 @@ constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
 static_assert(RS_THREADS == RS_DIGITS, "one thread per digit");
 constexpr int SCAN_B = 1024;
 
-// default spread tile heights (z values per tile) of the FP64 and FP32C 3-D forms
-constexpr int S3_ZC_FP64 = 32;
-constexpr int S3_ZC_FP32 = 16;
+
+// IndexUtilities::getCellIndex (IndexUtilities-inl.h:52-72; the level form
+// :226-243 passes the level's domain box and x range) along one axis, as a
+// double: the lower side's formula when X is at least as close to x_lower as to
+// x_upper, else the upper side's.  The result is an integer value (or NaN for a
+// NaN position); callers check its range before converting it.
+__device__ __forceinline__ double cell_index_d(double X, double x_lower, double x_upper, double dx, int ilower,
+                                               int iupper)
+{
+    const double dXl = X - x_lower, dXu = X - x_upper;
+    if (fabs(dXl) <= fabs(dXu)) return (double)ilower + floor(dXl / dx);
+    return ((double)iupper + floor(dXu / dx)) + 1.0;
+}
 
 __host__ __device__ inline int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
@@ -98,8 +119,7 @@ struct ibamr_plan
     int data_gcw[3] = {0, 0, 0};
     int idx_gcw = 0;
     int reach = 0;
-    int tile = 8;
-    int tsz[3] = {8, 8, 8};   // spread tile extent per axis
+    int tile = 8;           // bin edge (cells)
     int bin = 8;
     int drift = 0;
     double dx[3] = {1, 1, 1};
@@ -120,8 +140,13 @@ struct ibamr_plan
 
     // device buffers
     PatchDev* patches_d = nullptr;
-    TileDesc* tiles_d = nullptr;
-    int ntiles = 0;
+    std::vector<TileList> tlists;      // spread work lists, one per admissible tile shape (first: default)
+    int tcur = 0;                      // the shape in use
+    // FP32C 3-D spread (fixed point): fx_d[0] = largest number of points in one
+    // bin (set by the binning), fx_d[1 + c] = max |F_c| as float bits (set by
+    // each spread); fx_factor = bins whose points can reach one grid value
+    unsigned* fx_d = nullptr;
+    double fx_factor = 1.0;
     int* bin_patch_start_d = nullptr;  // [nmem+2]
     int* bin_patches_d = nullptr;
     int4* fill_segs_d = nullptr;       // level ghost fill: (patch, array, first element, count) segments
@@ -149,13 +174,7 @@ struct ibamr_plan
     int4* cell_sorted_d = nullptr;
     double* Xs_d = nullptr;            // positions in bin order
     bool xs_valid = false;             // Xs holds the positions of the last spread/interp (same binning)
-    bool ws_valid = false;             // wt_d/st_d hold the stencils of Xs (FP32C 3-D)
     void* Fs_d = nullptr;              // forces in bin order
-    // FP32C 3-D: per point in bin order, axis d and variant v, the first stencil
-    // index (st_d[(d*nvar+v)*n + k]) and the W weights (wt_d[((d*nvar+v)*n + k)*W + j])
-    float* wt_d = nullptr;
-    int* st_d = nullptr;
-    int nvar = 1;                      // variants per axis: 2 for side data (x_lower, x_lower - dx/2)
     int* bin_start_d = nullptr;        // [nmem+2]
 
     // optional timing of the main spread / interpolation kernels: event pairs
@@ -257,8 +276,9 @@ inline int kernel_width(int k)
 int radix_sort_pairs(const ibamr_plan* P, const int* kin, const int* vin, int n, int nbits, int* kout, int* vout,
                      int* kt, int* vt, hipStream_t s);
 int bits_for(int maxkey);
-// le_spread.hip: the spread's work list for the plan's tile shape
-int build_tiles(ibamr_plan* P);
+// le_spread.hip: the spread's work lists for every admissible tile shape of the
+// plan's form, and the fixed-point bound factor
+int build_tile_lists(ibamr_plan* P);
 // le_interp.hip: per-plan interpolation set-up (dynamic LDS limits)
 int interp_prepare(int ndim, int kernel, int precision);
 // le_plan.hip: timing of the main spread / interpolation launches

@@ -82,10 +82,8 @@ __global__ void k_redist_prepare(RedistArgs a)
             }
             a.X[NDIM * (size_t)p + d] = X;
             if (a.offset) a.offset[NDIM * (size_t)p + d] = lost ? 0 : (int)round((Xr - X) / a.dx[d]);
-            // IndexUtilities::getCellIndex (IndexUtilities-inl.h:52-72) on the level domain
-            const double dXl = X - a.xlo[d], dXu = X - a.xhi[d];
-            const double v = fabs(dXl) <= fabs(dXu) ? (double)a.dom_lo[d] + floor(dXl / a.dx[d])
-                                                    : ((double)a.dom_hi[d] + floor(dXu / a.dx[d])) + 1.0;
+            // IndexUtilities::getCellIndex on the level domain
+            const double v = cell_index_d(X, a.xlo[d], a.xhi[d], a.dx[d], a.dom_lo[d], a.dom_hi[d]);
             const bool inside = v >= (double)a.dom_lo[d] && v <= (double)a.dom_hi[d];
             ok = ok && !lost && !escaped && inside;
             c[d] = ok ? (int)v : a.dom_lo[d];
@@ -203,7 +201,6 @@ int ibamr_plan_redistribute(ibamr_plan* P, double* X, const int* lag_index, int 
     const int B = 256;
     P->npoints = -1;  // the binning referred to the old order
     P->xs_valid = false;
-    P->ws_valid = false;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, P->status_d, 8, 0);
     RedistArgs a{};
     a.X = X;

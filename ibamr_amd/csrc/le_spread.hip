@@ -24,79 +24,6 @@ __global__ void k_scatter_points(const int* __restrict__ inv, int n, int ndim, c
         for (int c = 0; c < fdepth; ++c) Fs[(size_t)fdepth * k + c] = F[(size_t)fdepth * p + c];
 }
 
-// FP32C 3-D spread set-up: for each point (bin order), axis d and variant v
-// (v = 0: the level's x_lower; v = 1: x_lower - dx/2, the normal axis of side
-// data, LEInteractor.cpp:3689) the first stencil index in the level's index
-// space and the W weights, in the level's frame.  The spread's tiles then stage
-// a candidate with loads instead of recomputing its weights for every tile it
-// reaches; a patch or periodic image shifts the start by whole cells.  With
-// perm, the kernel also performs the bin-order gather of X and F (side data:
-// records of 3 positions and 3 force components), so Xs is not read back.
-template <int KER>
-__global__ void k_point_weights(int n, double* __restrict__ Xs, int nvar, double3 xlo, double3 dx, int3 lo,
-                                float* __restrict__ wt, int* __restrict__ st, const int* __restrict__ perm,
-                                const double* __restrict__ Xin, const float* __restrict__ Fin,
-                                float* __restrict__ Fs)
-{
-    constexpr int W = KernelTraits<KER>::W;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double xl[3] = {xlo.x, xlo.y, xlo.z}, h[3] = {dx.x, dx.y, dx.z};
-    const int il[3] = {lo.x, lo.y, lo.z};
-    double x[3];
-    if (perm)
-    {
-        const size_t p = (size_t)perm[k];
-        float f[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-        {
-            x[d] = Xin[3 * p + d];
-            f[d] = Fin[3 * p + d];
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-        {
-            Xs[3 * (size_t)k + d] = x[d];
-            Fs[3 * (size_t)k + d] = f[d];
-        }
-    }
-    else
-#pragma unroll
-        for (int d = 0; d < 3; ++d) x[d] = Xs[3 * (size_t)k + d];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-        const double X = x[d];
-        for (int v = 0; v < nvar; ++v)
-        {
-            float w[W];
-            const int s0 = weights_1d<KER>(X, v ? xl[d] - 0.5 * h[d] : xl[d], h[d], w);
-            const size_t row = (size_t)(d * nvar + v) * n + k;
-            st[row] = il[d] + s0;
-#pragma unroll
-            for (int j = 0; j < W; ++j) wt[row * W + j] = w[j];
-        }
-    }
-}
-
-// FP32C 3-D: stencil starts and weights of the bin-ordered positions, used by
-// the spread and the interpolation of the same positions
-// (with X and F: the gather into bin order of side data's X and F as well)
-template <int KER>
-int launch_point_weights(ibamr_plan* P, hipStream_t s, const double* X = nullptr, const float* F = nullptr)
-{
-    if (!P->wt_d || !P->st_d) return fail(IBAMR_ERR_INVALID, "internal: no weight buffers");
-    const int B = 256;
-    hipLaunchKernelGGL((k_point_weights<KER>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->npoints, P->Xs_d,
-                       P->nvar, make_double3(P->dom_xlo[0], P->dom_xlo[1], P->dom_xlo[2]),
-                       make_double3(P->dx[0], P->dx[1], P->dx[2]),
-                       make_int3(P->dom_lo[0], P->dom_lo[1], P->dom_lo[2]), P->wt_d, P->st_d,
-                       X ? P->perm_d : nullptr, X, F, static_cast<float*>(P->Fs_d));
-    HIP_OK(hipGetLastError());
-    return IBAMR_OK;
-}
-
 // ----------------------------------------------------------------------------- spread
 
 template <typename T>
@@ -123,11 +50,10 @@ struct SpreadArgs
     int reach;
     int side;
     int mode;
-    // FP32C 3-D: the points' stencil starts and weights in the level's frame
-    const float* wt;
-    const int* st;
-    int nvar;
-    int npts;
+    // FP32C 3-D (fixed point): the bound buffer (largest bin count, max |F| per
+    // component) and the bins whose points can reach one grid value
+    const unsigned* fx;
+    double fx_factor;
 };
 
 template <int NDIM, int TILE>
@@ -259,14 +185,19 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
     const bool overwrite = a.mode == IBAMR_SPREAD_OVERWRITE;
     if (total == 0 && !overwrite) return;
 
-    // Load (accumulate) or clear (overwrite) the tile; LDS layout is TILE-strided.
+    // Load (accumulate: the raw LEInteractor::spread adds each contribution to
+    // the existing value, f.m4:1463) or clear (overwrite; interior mode, which
+    // spreads into zeroed data and adds the old values afterwards as
+    // LDataManager::spread does, LDataManager.cpp:594, 661-663) the tile; LDS
+    // layout is TILE-strided.
+    const bool from_old = a.mode == IBAMR_SPREAD_ACCUMULATE;
     for (int e = lane; e < TN; e += WAVE)
     {
         T v = T(0);
         const int i0 = e % TILE;
         const int i1 = (e / TILE) % TILE;
         const int i2 = NDIM == 3 ? e / (TILE * TILE) : 0;
-        if (!overwrite && i0 < fn[0] && i1 < fn[1] && i2 < fn[2])
+        if (from_old && i0 < fn[0] && i1 < fn[1] && i2 < fn[2])
         {
             const size_t g = (size_t)(flo[0] + i0 - alo[0]) +
                              (size_t)an[0] * ((size_t)(flo[1] + i1 - alo[1]) +
@@ -392,6 +323,7 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
                                           (NDIM == 3 ? (size_t)an[1] * (size_t)(flo[2] + i2 - alo[2]) : 0));
         T v = s_acc[li];
         if (!FP64) v = v - s_cmp[li];
+        if (!overwrite && !from_old) v = arr[comp_off + g] + v;
         arr[comp_off + g] = v;
     }
 }
@@ -401,10 +333,37 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
 
 namespace
 {
+// fmax[c] = max over the points of |F_c| as float bits (non-negative floats
+// order like their bit patterns; a NaN sorts above every finite value), for
+// the fixed-point scale of the FP32C spread.  Order-independent: atomicMax of
+// integers.
+__global__ __launch_bounds__(256) void k_fmax(const float* __restrict__ F, int n, int fdepth,
+                                              unsigned* __restrict__ fmax)
+{
+    __shared__ unsigned s_m[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c = 0; c < fdepth; ++c)
+    {
+        unsigned m = 0u;
+        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+            m = max(m, __float_as_uint(fabsf(F[(size_t)fdepth * k + c])));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+        if (lane == 0) s_m[wv] = m;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            const unsigned b = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+            if (b) atomicMax(fmax + c, b);
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T>
 void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double* X, int mode, SpreadArgs<T>& a)
 {
-    a.tiles = P->tiles_d;
+    a.tiles = P->tlists[P->tcur].d;
     a.patches = P->patches_d;
     a.arrays = reinterpret_cast<T* const*>(P->arrays_d[slot]);
     a.X = X;
@@ -429,71 +388,158 @@ void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double
     a.reach = P->reach;
     a.side = P->centering == IBAMR_SIDE;
     a.mode = mode;
-    a.wt = P->wt_d;
-    a.st = P->st_d;
-    a.nvar = P->nvar;
-    a.npts = P->npoints;
+    a.fx = P->fx_d;
+    a.fx_factor = P->fx_factor;
 }
-} // namespace
 
-// The spread's work list for the plan's tile shape, uploaded to tiles_d
-// (replacing any previous list).
-int build_tiles(ibamr_plan* P)
+// Spread tile shapes built for each form; the first is the default.
+//   2-D: a 16 x 16 tile in LDS (k_spread_tiles).
+//   3-D FP64: 8 x 8 grid lines times a column of z values in registers (k_spread3).
+//   3-D FP32C: a box of 64-bit fixed-point values in LDS (k_spread3_fx); the
+//     kernels of the path (IB_4, IB_6, BSPLINE_3) have several shapes, the
+//     others the default only.
+using Shape = std::array<int, 3>;
+const std::vector<Shape>& tile_shapes(int ndim, int precision, int kernel)
 {
-    // work list: every tile of every array of every patch.  XCD-aware launch
-    // order: workgroup i runs on XCD i % 8.  Tiles are grouped by (patch, z row,
-    // y row), each group holding that row of every array, and the groups are
-    // dealt to the XCDs in turn; each XCD runs its groups in (patch, z, y) order.
-    // A group's tiles read the same point data through one L2 (all arrays, x
-    // neighbours), every XCD gets a share of every region (point clouds are
-    // rarely uniform, so contiguous eighths of the domain would leave some XCDs
-    // idle), and all XCDs sweep the domain together, so the data they re-read is
-    // recent in the shared last-level cache.
-    std::vector<TileDesc> tiles;
-    {
-        const int ncomp_tiles = P->centering == IBAMR_SIDE ? P->ndim : P->depth;
-        std::vector<std::vector<TileDesc>> q(8);
-        size_t g = 0;
-        for (int i = 0; i < P->npatches; ++i)
-        {
-            const PatchDev& pd = P->patches_h[i];
-            std::vector<std::array<int, 3>> nt(ncomp_tiles, std::array<int, 3>{1, 1, 1});
-            int mt[3] = {1, 1, 1};
-            for (int c = 0; c < ncomp_tiles; ++c)
-            {
-                const int axis = P->centering == IBAMR_SIDE ? c : -1;
-                for (int d = 0; d < P->ndim; ++d)
-                {
-                    const int n = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
-                    nt[c][d] = (n + P->tsz[d] - 1) / P->tsz[d];
-                    mt[d] = std::max(mt[d], nt[c][d]);
-                }
-            }
-            for (int tz = 0; tz < mt[2]; ++tz)
-                for (int ty = 0; ty < mt[1]; ++ty, ++g)
-                    for (int c = 0; c < ncomp_tiles; ++c)
-                        if (ty < nt[c][1] && tz < nt[c][2])
-                            for (int tx = 0; tx < nt[c][0]; ++tx) q[g % 8].push_back(TileDesc{i, c, {tx, ty, tz}});
-        }
-        size_t per = 0;
-        for (const auto& v : q) per = std::max(per, v.size());
-        tiles.assign(8 * per, TileDesc{-1, 0, {0, 0, 0}});
-        for (int x = 0; x < 8; ++x)
-            for (size_t k = 0; k < q[x].size(); ++k) tiles[8 * k + x] = q[x][k];
+    static const std::vector<Shape> s2 = {{16, 16, 1}};
+    static const std::vector<Shape> s64 = {{8, 8, 32}, {8, 8, 16}};
+    static const std::vector<Shape> sfx = {{16, 16, 64}, {16, 16, 16}, {16, 16, 32}, {32, 8, 64}, {32, 16, 8}};
+    static const std::vector<Shape> sfx1 = {{16, 16, 64}, {16, 16, 16}};
+    if (ndim == 2) return s2;
+    if (precision == IBAMR_FP64) return s64;
+    return (kernel == K_IB4 || kernel == K_IB6 || kernel == K_BS3) ? sfx : sfx1;
+}
+
+// FP32C 3-D: z extents up to 16 are LDS boxes (k_spread3_fx); 32 and more are
+// column segments swept through an LDS window of FXZ_PLANES planes (k_spread3_fxz)
+constexpr int FXZ_PLANES = 16;
+bool is_column(const int* sh) { return sh[2] >= 32; }
+
+// The FP32C kernel of one tile shape (the shapes of tile_shapes)
+template <int KER>
+int launch_fx(const ibamr_plan* P, const SpreadArgs<float>& a, hipStream_t s)
+{
+    const int* sh = P->tlists[P->tcur].shape;
+    const dim3 G(P->tlists[P->tcur].n), B(FX_THREADS);
+#define IBAMR_FX_BOX(X, Y, Z)                                                            \
+    if (sh[0] == X && sh[1] == Y && sh[2] == Z)                                          \
+    {                                                                                    \
+        hipLaunchKernelGGL((k_spread3_fx<KER, X, Y, Z>), G, B, 0, s, a);                 \
+        return IBAMR_OK;                                                                 \
     }
-    dfree(P->tiles_d);
-    P->tiles_d = nullptr;
-    P->ntiles = (int)tiles.size();
-    HIP_OK(hipMalloc(&P->tiles_d, sizeof(TileDesc) * std::max<size_t>(1, tiles.size())));
-    if (!tiles.empty()) HIP_OK(hipMemcpy(P->tiles_d, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
+#define IBAMR_FX_COLUMN(X, Y)                                                            \
+    if (sh[0] == X && sh[1] == Y && is_column(sh))                                       \
+    {                                                                                    \
+        hipLaunchKernelGGL((k_spread3_fxz<KER, X, Y, FXZ_PLANES>), G, B, 0, s, a, sh[2]); \
+        return IBAMR_OK;                                                                 \
+    }
+    IBAMR_FX_COLUMN(16, 16)
+    IBAMR_FX_BOX(16, 16, 16)
+    if constexpr (KER == K_IB4 || KER == K_IB6 || KER == K_BS3)
+    {
+        IBAMR_FX_COLUMN(32, 8)
+        IBAMR_FX_BOX(32, 16, 8)
+    }
+#undef IBAMR_FX_BOX
+#undef IBAMR_FX_COLUMN
+    return fail(IBAMR_ERR_INVALID, "internal: tile shape without a kernel");
+}
+
+// A shape is admissible when each axis of a tile reaches at most 16 (bin,
+// image) entries (the image ranges partition the reachable cells, plus one
+// split per domain side), and for a column segment when one bin layer's points
+// reach no more planes than the window holds.
+bool shape_admissible(const ibamr_plan* P, const Shape& sh)
+{
+    for (int d = 0; d < P->ndim; ++d)
+        if ((sh[d] + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16) return false;
+    if (P->ndim == 3 && P->precision != IBAMR_FP64 && is_column(sh.data()) && P->bin + 2 * P->reach > FXZ_PLANES)
+        return false;
+    return true;
+}
+
+// Work list of one tile shape: every tile of every array of every patch.
+// XCD-aware launch order: workgroup i runs on XCD i % 8.  Tiles are grouped by
+// (patch, z row, y row), each group holding that row of every array, and the
+// groups are dealt to the XCDs in turn; each XCD runs its groups in (patch, z,
+// y) order.  A group's tiles read the same point data through one L2 (all
+// arrays, x neighbours), every XCD gets a share of every region (point clouds
+// are rarely uniform, so contiguous eighths of the domain would leave some XCDs
+// idle), and all XCDs sweep the domain together, so the data they re-read is
+// recent in the shared last-level cache.
+int build_tiles(ibamr_plan* P, const Shape& sh, TileList& out)
+{
+    std::vector<TileDesc> tiles;
+    const int ncomp_tiles = P->centering == IBAMR_SIDE ? P->ndim : P->depth;
+    std::vector<std::vector<TileDesc>> q(8);
+    size_t g = 0;
+    for (int i = 0; i < P->npatches; ++i)
+    {
+        const PatchDev& pd = P->patches_h[i];
+        std::vector<std::array<int, 3>> nt(ncomp_tiles, std::array<int, 3>{1, 1, 1});
+        int mt[3] = {1, 1, 1};
+        for (int c = 0; c < ncomp_tiles; ++c)
+        {
+            const int axis = P->centering == IBAMR_SIDE ? c : -1;
+            for (int d = 0; d < P->ndim; ++d)
+            {
+                const int n = pd.ihi[d] - pd.ilo[d] + 1 + 2 * P->data_gcw[d] + (d == axis ? 1 : 0);
+                nt[c][d] = (n + sh[d] - 1) / sh[d];
+                mt[d] = std::max(mt[d], nt[c][d]);
+            }
+        }
+        for (int tz = 0; tz < mt[2]; ++tz)
+            for (int ty = 0; ty < mt[1]; ++ty, ++g)
+                for (int c = 0; c < ncomp_tiles; ++c)
+                    if (ty < nt[c][1] && tz < nt[c][2])
+                        for (int tx = 0; tx < nt[c][0]; ++tx) q[g % 8].push_back(TileDesc{i, c, {tx, ty, tz}});
+    }
+    size_t per = 0;
+    for (const auto& v : q) per = std::max(per, v.size());
+    tiles.assign(8 * per, TileDesc{-1, 0, {0, 0, 0}});
+    for (int x = 0; x < 8; ++x)
+        for (size_t k = 0; k < q[x].size(); ++k) tiles[8 * k + x] = q[x][k];
+    out.n = (int)tiles.size();
+    for (int d = 0; d < 3; ++d) out.shape[d] = sh[d];
+    HIP_OK(hipMalloc(&out.d, sizeof(TileDesc) * std::max<size_t>(1, tiles.size())));
+    if (!tiles.empty()) HIP_OK(hipMemcpy(out.d, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
     return IBAMR_OK;
 }
 
-// Spread tile heights instantiated for each form (3-D); the first is the default.
-const std::vector<int>& tile_heights(int precision)
+int find_shape(const ibamr_plan* P, const int* shape)
 {
-    static const std::vector<int> fp64 = {32, 16}, fp32 = {16, 8, 24};
-    return precision == IBAMR_FP64 ? fp64 : fp32;
+    for (size_t i = 0; i < P->tlists.size(); ++i)
+        if (P->tlists[i].shape[0] == shape[0] && P->tlists[i].shape[1] == shape[1] &&
+            P->tlists[i].shape[2] == shape[2])
+            return (int)i;
+    return -1;
+}
+} // namespace
+
+int build_tile_lists(ibamr_plan* P)
+{
+    for (const Shape& sh : tile_shapes(P->ndim, P->precision, P->kernel))
+    {
+        if (!shape_admissible(P, sh)) continue;
+        P->tlists.emplace_back();
+        const int rc = build_tiles(P, sh, P->tlists.back());
+        if (rc != IBAMR_OK) return rc;
+    }
+    if (P->tlists.empty()) return fail(IBAMR_ERR_INVALID, "drift_cells too large for the tile size");
+    P->tcur = 0;
+    // FP32C fixed-point bound: the cells whose points can reach one grid value
+    // span 2 reach + 1 cells per axis, so at most nblk bins, and with periodic
+    // images as many more copies as that window can wrap around the period
+    double f = 1.0;
+    for (int d = 0; d < P->ndim; ++d)
+    {
+        const int win = 2 * P->reach + 1;
+        const int nblk = (win - 1 + P->bin - 1) / P->bin + 1;
+        const int wraps = P->pshift[d] > 0 ? 1 + (win - 1 + P->pshift[d] - 1) / P->pshift[d] : 1;
+        f *= (double)nblk * wraps;
+    }
+    P->fx_factor = f;
+    return IBAMR_OK;
 }
 } // namespace ibamr
 
@@ -507,44 +553,43 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
     if (mode != IBAMR_SPREAD_ACCUMULATE && mode != IBAMR_SPREAD_OVERWRITE && mode != IBAMR_SPREAD_ACCUMULATE_INTERIOR)
         return fail(IBAMR_ERR_INVALID, "bad mode");
     if (P->npoints > 0 && (!F || !X)) return fail(IBAMR_ERR_INVALID, "null F or X");
-    if (P->ntiles == 0) return IBAMR_OK;
+    if (P->tlists.empty() || P->tlists[P->tcur].n == 0) return IBAMR_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     return dispatch(P->ndim, P->kernel, P->precision, [&](auto nd, auto ker, auto prec) -> int {
         using T = decltype(prec);
         constexpr int NDIM = decltype(nd)::value;
         constexpr int KER = decltype(ker)::value;
+        constexpr bool FX = NDIM == 3 && sizeof(T) == 4;
         SpreadArgs<T> a{};
         fill_spread_args<T>(P, slot, F, X, mode, a);
+        const int B = 256;
         if (P->npoints > 0)
         {
-            const int B = 256;
-            // FP32C 3-D: stencils set up here (not for kernels whose side depends on the unshifted X)
-            constexpr bool WTS = NDIM == 3 && sizeof(T) == 4 && !side_by_unshifted_x<KER>();
-            if (WTS && P->fdepth == NDIM)
-            {
-                // gather and stencils in one pass
-                const int rc = launch_point_weights<KER>(P, s, X, reinterpret_cast<const float*>(F));
-                if (rc != IBAMR_OK) return rc;
-            }
+            // the points in bin order
+            if (P->fdepth == NDIM)
+                hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
+                                   P->npoints, X, P->Xs_d, static_cast<const T*>(F), static_cast<T*>(P->Fs_d));
             else
-            {
-                if (P->fdepth == NDIM)
-                    hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
-                                       P->perm_d, P->npoints, X, P->Xs_d, static_cast<const T*>(F),
-                                       static_cast<T*>(P->Fs_d));
-                else
-                    hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
-                                       P->idx_d, P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
-                                       static_cast<T*>(P->Fs_d));
-                HIP_OK(hipGetLastError());
-                if (WTS)
-                {
-                    const int rc = launch_point_weights<KER>(P, s);
-                    if (rc != IBAMR_OK) return rc;
-                }
-            }
+                hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
+                                   P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
+                                   static_cast<T*>(P->Fs_d));
+            HIP_OK(hipGetLastError());
             P->xs_valid = true;
-            P->ws_valid = WTS;
+        }
+        if constexpr (FX)
+        {
+            // fixed-point scale: max |F| per component (a fill kernel, not a
+            // memset: the launches may be captured in a HIP graph)
+            if (!P->fx_d) return fail(IBAMR_ERR_INVALID, "internal: no fixed-point bound buffer");
+            hipLaunchKernelGGL(k_fill_int, dim3((P->fdepth + B - 1) / B), dim3(B), 0, s,
+                               reinterpret_cast<int*>(P->fx_d + 1), P->fdepth, 0);
+            if (P->npoints > 0)
+            {
+                const int G = std::min((P->npoints + B - 1) / B, 1024);
+                hipLaunchKernelGGL(k_fmax, dim3(G), dim3(B), 0, s, static_cast<const float*>(P->Fs_d), P->npoints,
+                                   P->fdepth, P->fx_d + 1);
+            }
+            HIP_OK(hipGetLastError());
         }
         a.X = P->Xs_d;
         a.F = static_cast<const T*>(P->Fs_d);
@@ -553,91 +598,89 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
         hipEvent_t t_end;
         const int trc = time_begin(P, 0, s, &t_end);
         if (trc != IBAMR_OK) return trc;
-        if (NDIM == 3)
+        const TileList& tl = P->tlists[P->tcur];
+        if constexpr (FX)
         {
-            // FP32C packs chunks across bins; FP64 cuts them per bin (the oracle's fixed order)
-            constexpr bool MERGE = sizeof(T) == 4;
-            switch (P->tsz[2])
+            const int rc = launch_fx<KER>(P, a, s);
+            if (rc != IBAMR_OK) return rc;
+        }
+        else if constexpr (NDIM == 3)
+        {
+            switch (tl.shape[2])
             {
-            case 8: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 8>), dim3(P->ntiles), dim3(64), 0, s, a); break;
-            case 16: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 16>), dim3(P->ntiles), dim3(64), 0, s, a); break;
-            case 24: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 24>), dim3(P->ntiles), dim3(64), 0, s, a); break;
-            case 32: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 32>), dim3(P->ntiles), dim3(64), 0, s, a); break;
+            case 16: hipLaunchKernelGGL((k_spread3<KER, 16>), dim3(tl.n), dim3(64), 0, s, a); break;
+            case 32: hipLaunchKernelGGL((k_spread3<KER, 32>), dim3(tl.n), dim3(64), 0, s, a); break;
             default: return fail(IBAMR_ERR_INVALID, "internal: tile height without a kernel");
             }
         }
         else
-            hipLaunchKernelGGL((k_spread_tiles<2, KER, T, 16>), dim3(P->ntiles), dim3(WAVE), 0, s, a);
+            hipLaunchKernelGGL((k_spread_tiles<2, KER, T, 16>), dim3(tl.n), dim3(WAVE), 0, s, a);
         HIP_OK(hipGetLastError());
         if (t_end) HIP_OK(hipEventRecord(t_end, s));
         return IBAMR_OK;
     });
 }
 
+int ibamr_plan_set_tile_shape(ibamr_plan* P, const int* shape)
+{
+    if (!P || !shape) return fail(IBAMR_ERR_INVALID, "null argument");
+    const int i = find_shape(P, shape);
+    if (i < 0)
+        return fail(IBAMR_ERR_INVALID, "tile shape " + std::to_string(shape[0]) + "x" + std::to_string(shape[1]) + "x" +
+                                           std::to_string(shape[2]) + " not built for this plan");
+    P->tcur = i;
+    return IBAMR_OK;
+}
+
+int ibamr_plan_get_tile_shape(const ibamr_plan* P, int* shape)
+{
+    if (!P || !shape) return fail(IBAMR_ERR_INVALID, "null argument");
+    for (int d = 0; d < 3; ++d) shape[d] = P->tlists.empty() ? 0 : P->tlists[P->tcur].shape[d];
+    return IBAMR_OK;
+}
+
 int ibamr_plan_set_tile_height(ibamr_plan* P, int height)
 {
     if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
-    if (P->ndim != 3) return height == P->tsz[2] ? IBAMR_OK : fail(IBAMR_ERR_INVALID, "2-D plans have one tile shape");
-    const std::vector<int>& hs = tile_heights(P->precision);
-    if (std::find(hs.begin(), hs.end(), height) == hs.end())
-        return fail(IBAMR_ERR_INVALID, "tile height " + std::to_string(height) + " not built for this precision");
-    if ((height + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16)
-        return fail(IBAMR_ERR_INVALID, "drift_cells too large for this tile height");
-    if (height == P->tsz[2]) return IBAMR_OK;
-    const int old = P->tsz[2];
-    P->tsz[2] = height;
-    const int rc = build_tiles(P);
-    if (rc != IBAMR_OK)
-    {
-        const std::string m = g_err;
-        P->tsz[2] = old;
-        (void)build_tiles(P);
-        g_err = m;
-        return rc;
-    }
-    return IBAMR_OK;
+    for (size_t i = 0; i < P->tlists.size(); ++i)
+        if (P->tlists[i].shape[2] == height)
+        {
+            P->tcur = (int)i;
+            return IBAMR_OK;
+        }
+    return fail(IBAMR_ERR_INVALID, "tile height " + std::to_string(height) + " not built for this plan");
 }
 
 int ibamr_plan_get_tile_height(const ibamr_plan* P, int* height)
 {
     if (!P || !height) return fail(IBAMR_ERR_INVALID, "null argument");
-    *height = P->tsz[2];
+    *height = P->tlists.empty() ? 0 : P->tlists[P->tcur].shape[2];
     return IBAMR_OK;
 }
 
-int ibamr_plan_autotune(ibamr_plan* P, int slot, const void* F, const double* X, int reps, int* heights, float* ms,
+int ibamr_plan_autotune(ibamr_plan* P, int slot, const void* F, const double* X, int reps, int* shapes, float* ms,
                         int* ncand, void* stream)
 {
-    if (!P || !heights || !ms || !ncand) return fail(IBAMR_ERR_INVALID, "null argument");
+    if (!P || !shapes || !ms || !ncand) return fail(IBAMR_ERR_INVALID, "null argument");
     if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
     *ncand = 0;
-    if (P->ndim != 3)
-    {
-        heights[0] = P->tsz[2];
-        ms[0] = 0.0f;
-        *ncand = 1;
-        return IBAMR_OK;
-    }
+    if ((int)P->tlists.size() > IBAMR_MAX_TILE_SHAPES) return fail(IBAMR_ERR_INVALID, "internal: too many tile shapes");
     reps = std::max(1, reps);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const std::vector<int> hs = tile_heights(P->precision);
-    if ((int)hs.size() > IBAMR_MAX_TILE_HEIGHTS) return fail(IBAMR_ERR_INVALID, "internal: too many tile heights");
     const bool timing = P->timing;
     P->timing = false;  // the trials are not the caller's launches
+    const int start = P->tcur;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = IBAMR_OK, best = P->tsz[2];
+    int best = start;
     float best_ms = std::numeric_limits<float>::infinity();
     auto run = [&]() -> int {
         HIP_OK(hipEventCreate(&e0));
         HIP_OK(hipEventCreate(&e1));
-        for (size_t i = 0; i < hs.size(); ++i)
+        for (size_t i = 0; i < P->tlists.size(); ++i)
         {
-            heights[i] = hs[i];
-            ms[i] = -1.0f;  // not admissible (drift too large for this height)
-            if ((hs[i] + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16) continue;
-            int r = ibamr_plan_set_tile_height(P, hs[i]);
-            if (r != IBAMR_OK) return r;
-            r = ibamr_plan_spread(P, slot, F, X, IBAMR_SPREAD_OVERWRITE, stream);
+            for (int d = 0; d < 3; ++d) shapes[3 * i + d] = P->tlists[i].shape[d];
+            P->tcur = (int)i;
+            int r = ibamr_plan_spread(P, slot, F, X, IBAMR_SPREAD_OVERWRITE, stream);
             if (r != IBAMR_OK) return r;
             HIP_OK(hipEventRecord(e0, s));
             for (int k = 0; k < reps; ++k)
@@ -653,18 +696,23 @@ int ibamr_plan_autotune(ibamr_plan* P, int slot, const void* F, const double* X,
             if (ms[i] < best_ms)
             {
                 best_ms = ms[i];
-                best = hs[i];
+                best = (int)i;
             }
         }
         return IBAMR_OK;
     };
-    rc = run();
+    const int rc = run();
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     P->timing = timing;
-    if (rc != IBAMR_OK) return rc;
-    *ncand = (int)hs.size();
-    return ibamr_plan_set_tile_height(P, best);
+    if (rc != IBAMR_OK)
+    {
+        P->tcur = start;
+        return rc;
+    }
+    *ncand = (int)P->tlists.size();
+    P->tcur = best;
+    return IBAMR_OK;
 }
 
 } // extern "C"

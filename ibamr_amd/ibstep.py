@@ -9,6 +9,11 @@ Reference behaviour mirrored:
     structures sharing a kernel is one level plan; the first plan overwrites f,
     the others accumulate.
   * IBMethod::forwardEulerStep: X_new = X + dt U (src/IB/IBMethod.cpp:713-739).
+  * Positions that leave the periodic box are moved back into it every step,
+    as LDataManager::beginDataRedistribution does at a regrid (X += L while
+    X < x_lower, then X -= L while X >= x_upper, LDataManager.cpp:1403-1407):
+    the points are binned every step here, and a point outside the domain would
+    otherwise drop out of the spread and keep a stale velocity.
   * With a force generator (ibamr_amd.structure.Forces), F is set to the force
     at X and the previous U at the start of every step, as
     IBMethod::computeLagrangianForce zeroes F and calls the force generator
@@ -20,8 +25,8 @@ by the periodic ghost fill, so the interpolated velocity depends on the spread.
 
 Graph capture (torch.cuda.CUDAGraph) records only kernels: the force
 generation, point binning (ibamr_plan_bin_async), spread, the stand-in
-combination (a torch elementwise kernel), the periodic ghost fill, interp and
-the Euler update.  No memset and
+combination (a torch elementwise kernel), the periodic ghost fill, interp, the
+Euler update and the periodic wrap (torch elementwise kernels).  No memset and
 no host synchronization happens inside a step.
 """
 import torch
@@ -83,12 +88,28 @@ class StructureSet:
         for p, a, b in self._slices():
             p.interp(1, U[a:b], X[a:b], stream=s)
         ia.forward_euler_step(X, X, U, dt, stream=s)
+        self._wrap(X, s)
+
+    def _wrap(self, X, s):
+        """Moves positions back into the periodic box (one period suffices: a
+        step moves a point by less than one period)."""
+        L = self.level
+        with torch.cuda.stream(s):
+            for d in range(L.ndim):
+                if not L.periodic_shift[d]:
+                    continue
+                lo, hi = L.domain_x_lower[d], L.domain_x_upper[d]
+                x = X[:, d]
+                x.copy_(torch.where(x < lo, x + (hi - lo), x))
+                x.copy_(torch.where(x >= hi, x - (hi - lo), x))
 
     def autotune(self, X, F, reps=2, stream=None):
-        """Picks each plan's spread tile height for its structures' current
+        """Picks each plan's spread tile shape for its structures' current
         positions (LevelPlan.autotune: trial spreads overwrite the force arrays,
-        which the next step overwrites anyway).  Returns {kernel: {height: ms}}.
-        Call it before capture: it rebuilds the plans' work lists."""
+        which the next step overwrites anyway).  Returns {kernel: {shape: ms}}.
+        A graph captured before keeps replaying the shapes it captured (every
+        shape's work list lives as long as its plan); capture again to use the
+        chosen ones."""
         out = {}
         for (k, _, _), (p, a, b) in zip(self.groups, self._slices()):
             p.bin(X[a:b], stream=stream, check=False)

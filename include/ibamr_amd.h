@@ -17,8 +17,8 @@
  *   - Lagrangian data: LData local form, AoS: X[n][ndim] (always double),
  *     F/U[n][depth] (side data: depth = ndim).
  *   - FP64 plans use double grids and F/U; FP32C plans use float grids and
- *     F/U with compensated accumulation; positions and stencil selection stay
- *     FP64 in both.
+ *     F/U (spread sums in 64-bit fixed point, compensated interpolation
+ *     sums); positions and stencil selection stay FP64 in both.
  */
 #ifndef IBAMR_AMD_H
 #define IBAMR_AMD_H
@@ -48,7 +48,8 @@ enum
 enum
 {
     IBAMR_FP64 = 0,  /* plain FP64, results identical to the reference arithmetic */
-    IBAMR_FP32C = 1  /* FP32 grid/force data, FP32 weights, Kahan-compensated sums */
+    IBAMR_FP32C = 1  /* FP32 grid/force data, FP32 weights; spread sums in 64-bit fixed point,
+                        interpolation sums compensated; positions and stencil selection FP64 */
 };
 enum
 {
@@ -144,12 +145,29 @@ int ibamr_plan_bin(ibamr_plan* plan, const double* X, int npoints, void* stream)
 int ibamr_plan_bin_async(ibamr_plan* plan, const double* X, int npoints, void* stream);
 int ibamr_plan_check(ibamr_plan* plan, void* stream);
 
+/* The cell index of every binned point, as the binning computed it
+ * (IndexUtilities::getCellIndex, IndexUtilities-inl.h:52-72; level form
+ * :226-243 with the level's domain box and x range, or the patch box and x
+ * range for ibamr_plan_create_patch plans): cells (device, [n][ndim] int, the
+ * caller's point order) receives it for each point of the last binning, inside
+ * the domain or not; INT_MIN where the index does not fit in an int (NaN or
+ * far away positions).  Kernel only (graph-safe). */
+int ibamr_plan_get_cell_indices(const ibamr_plan* plan, int* cells, void* stream);
+
 /* LDataManager::spread for one level (LDataManager.cpp:623-658): for every
  * patch, LEInteractor::spread over the Lagrangian index ghost box, with
- * periodic images, into the arrays bound to `slot`.  Deterministic: every
- * grid value sums its contributions in one fixed order (bin, image, point),
- * with no atomics, so results are bitwise reproducible.  F: [n][ndim] (side)
- * or [n][depth] (cell); X: current positions. */
+ * periodic images, into the arrays bound to `slot`.  Deterministic, with no
+ * floating-point atomics, so results are bitwise reproducible: FP64 sums every
+ * grid value's contributions in one fixed order (bin, image, point); FP32C
+ * rounds every contribution to a 64-bit fixed-point integer (one global scale
+ * per force component, set from the largest bin count and max |F|) and adds
+ * the integers, whose sum does not depend on the order.  An FP32C spread of a
+ * force holding a non-finite value writes NaN into every value the points can
+ * reach.  F: [n][ndim] (side) or [n][depth] (cell); X: current positions.
+ * Modes: IBAMR_SPREAD_ACCUMULATE adds each contribution to the existing value
+ * (the raw LEInteractor::spread); IBAMR_SPREAD_ACCUMULATE_INTERIOR sums the
+ * contributions from zero and then adds the existing value (LDataManager::
+ * spread's zero, spread, add, LDataManager.cpp:594, 661-663). */
 int ibamr_plan_spread(ibamr_plan* plan, int slot, const void* F, const double* X, int mode, void* stream);
 
 /* LDataManager::interp for one level (LDataManager.cpp:748-802): each point
@@ -325,24 +343,36 @@ int ibamr_plan_kernel_times(ibamr_plan* plan, double* spread_ms, int* nspread, d
 /* Plan facts for reports: number of spread tiles, tile edge, bins. */
 int ibamr_plan_info(const ibamr_plan* plan, int* ntiles, int* tile, int* nbins);
 
-/* Spread tile shape (3-D plans).  A spread tile is 8 x 8 grid lines of one
- * array times `height` z values held in registers; the heights built for a
- * form are 32, 16 (FP64) and 16, 8, 24 (FP32C), the first being the default.
- * FP64 results do not depend on the height (the summation order per value is
- * fixed by the bins); FP32C results do only by rounding.  No counterpart in the
- * reference, whose CPU loops have no tiles: these tune this implementation's
- * work decomposition to the caller's point distribution and hardware.
- *   set: choose a height (rebuilds the work list; not graph-safe).
- *   autotune: for each height, one untimed and `reps` timed spreads of F at X
+/* Spread tile shape.  The spread's work is a list of tiles (boxes of grid
+ * values of one array of one patch), one workgroup each; a plan builds the
+ * work list of every shape built for its form when it is created and keeps
+ * them until it is destroyed, so changing the shape only selects another list
+ * (a HIP graph captured earlier keeps launching the shape it captured, which
+ * stays valid).  Shapes (x, y, z extents; the first is the default):
+ *   2-D: 16 x 16.
+ *   3-D FP64: 8 x 8 grid lines times 32 or 16 z values held in registers.
+ *   3-D FP32C: an LDS box of 64-bit fixed-point values, 16x16x16, 32x16x8,
+ *     16x16x8 and 16x8x16 for IB_4, IB_6 and BSPLINE_3 (16x16x16 for the other
+ *     kernels).
+ * Results do not depend on the shape: FP64 sums each value in an order fixed
+ * by the bins, FP32C sums fixed-point integers with one global scale.  No
+ * counterpart in the reference, whose CPU loops have no tiles: these tune this
+ * implementation's work decomposition to the caller's points and hardware.
+ *   set_tile_shape / get_tile_shape: shape[3].
+ *   set_tile_height / get_tile_height: the first shape with that z extent / the
+ *     z extent of the shape in use.
+ *   autotune: for each shape, one untimed and `reps` timed spreads of F at X
  *     into the arrays bound to `slot` (IBAMR_SPREAD_OVERWRITE: their values are
- *     replaced), then keeps the fastest.  heights[] and ms[] (capacity
- *     IBAMR_MAX_TILE_HEIGHTS) receive each candidate and its mean time per
- *     spread call, *ncand their number; synchronises the stream.  Points must be
- *     binned.  2-D plans report their one shape. */
-#define IBAMR_MAX_TILE_HEIGHTS 8
+ *     replaced), then keeps the fastest.  shapes[3*i .. 3*i+2] and ms[i]
+ *     (capacity IBAMR_MAX_TILE_SHAPES) receive each candidate and its mean time
+ *     per spread call, *ncand their number; synchronises the stream.  Points
+ *     must be binned. */
+#define IBAMR_MAX_TILE_SHAPES 8
+int ibamr_plan_set_tile_shape(ibamr_plan* plan, const int* shape);
+int ibamr_plan_get_tile_shape(const ibamr_plan* plan, int* shape);
 int ibamr_plan_set_tile_height(ibamr_plan* plan, int height);
 int ibamr_plan_get_tile_height(const ibamr_plan* plan, int* height);
-int ibamr_plan_autotune(ibamr_plan* plan, int slot, const void* F, const double* X, int reps, int* heights,
+int ibamr_plan_autotune(ibamr_plan* plan, int slot, const void* F, const double* X, int reps, int* shapes,
                         float* ms, int* ncand, void* stream);
 
 const char* ibamr_last_error(void);

@@ -94,9 +94,11 @@ def test_graph_replays_match_eager_bitwise():
 
 
 def test_autotune_then_graph_matches_untuned_fp64():
-    """StructureSet.autotune times every built tile height per kernel group and
-    keeps the fastest; FP64 spreads are bitwise the same at every height, so a
-    captured step after it gives the trajectory of an untuned set, bit for bit."""
+    """StructureSet.autotune times every built tile shape per kernel group and
+    keeps the fastest; FP64 spreads are bitwise the same with every shape, so a
+    captured step after it gives the trajectory of an untuned set, bit for bit.
+    One group is moved off its default shape before capture, so the captured
+    step runs a non-default work list."""
     S0, X, F, U0, _ = _setup(ia.FP64)
     S1, _, _, U1, _ = _setup(ia.FP64)
     X0 = X.clone()
@@ -109,8 +111,10 @@ def test_autotune_then_graph_matches_untuned_fp64():
     torch.cuda.synchronize()
     assert sorted(tunes) == ["BSPLINE_3", "IB_4"]
     for (k, _, _), p in zip(S1.groups, S1.plans):
-        assert sorted(tunes[k]) == [16, 32]
-        assert p.tile_height == min(tunes[k], key=tunes[k].get)
+        assert sorted(tunes[k]) == [(8, 8, 16), (8, 8, 32)]
+        assert p.tile_shape == min(tunes[k], key=tunes[k].get)
+    p0 = S1.plans[0]
+    p0.tile_shape = (8, 8, 16) if p0.tile_shape == (8, 8, 32) else (8, 8, 32)
     g = S1.capture(X1, F, U1, dt)
     torch.cuda.synchronize()
     X1.copy_(X0)
@@ -194,3 +198,36 @@ def test_step_with_force_generation_graph_and_oracle(orc):
         torch.cuda.synchronize()
         assert torch.equal(X, traj[k]), k
     S.forces.close()
+
+
+def test_points_crossing_the_periodic_boundary_stay_in_the_spread():
+    """Points that move across x = 1 are wrapped back into the box inside the
+    step (as the reference does at a regrid), so they stay binned: the next
+    step's spread still holds every point's force (sum of f h^3 over the
+    periodic faces equals sum F, partition of unity)."""
+    level = ia.Level.uniform_periodic(N, 3)
+    h = 1.0 / N
+    n = 500
+    g = torch.Generator(device="cuda")
+    g.manual_seed(17)
+    X = torch.rand((n, 3), generator=g, device="cuda", dtype=torch.float64)
+    X[:, 0] = 1.0 - 0.3 * h * torch.rand(n, generator=g, device="cuda", dtype=torch.float64)
+    gw = ia.get_minimum_ghost_width("IB_4")
+    ubg = [torch.full(ia.array_shape(level, 0, [gw] * 3, ia.SIDE, axis=a), 0.5 if a == 0 else 0.0,
+                      dtype=torch.float64, device="cuda") for a in range(3)]
+    S = ibstep.StructureSet(level, [("IB_4", n)], ia.FP64, alpha=0.0, u_bg=ubg)
+    F = torch.randn((n, 3), generator=g, device="cuda", dtype=torch.float64)
+    U = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    print("ibstep periodic crossing", flush=True)
+    S.step(X, F, U, dt=h)
+    torch.cuda.synchronize()
+    assert torch.allclose(U[:, 0], torch.full_like(U[:, 0], 0.5), rtol=0, atol=1e-12)
+    assert bool(((X >= 0.0) & (X < 1.0)).all())
+    assert int((X[:, 0] < 0.5).sum()) > n // 2  # most points crossed and were wrapped
+    S.step(X, F, U, dt=0.0)
+    torch.cuda.synchronize()
+    for a in range(3):
+        sl = tuple(slice(gw, gw + N) for _ in range(3))
+        tot = float(S.f[a][sl].sum()) * h ** 3
+        ref = float(F[:, a].sum())
+        assert abs(tot - ref) <= 1e-10 * max(1.0, abs(ref)), (a, tot, ref)
