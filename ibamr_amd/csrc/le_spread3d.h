@@ -556,6 +556,7 @@ __global__ __launch_bounds__(FX_THREADS) void k_spread3_fx(SpreadArgs<float> a)
 
     for (int cb = 0; cb < ncomb; cb += S3_MAXCOMB)
     {
+        __syncthreads();  // every wave is done with the previous group's prefix
         // ---- this group of combinations: ranges and their prefix
         {
             const int c = cb + lane;
@@ -624,14 +625,16 @@ __global__ __launch_bounds__(FX_THREADS) void k_spread3_fx(SpreadArgs<float> a)
             __syncthreads();
             // ---- a full queue: every thread applies one point
             const int qn = s_qn;
+            __syncthreads();  // every wave has the count before a later pass adds to it
             if (qn >= NT)
             {
+                // each thread takes entry tid and moves entry NT + tid down (the
+                // only reader of slot tid is thread tid itself)
                 const int2 e = s_q[tid];
-                apply(e.x, e.y);
-                __syncthreads();
                 if (tid < qn - NT) s_q[tid] = s_q[NT + tid];
                 if (tid == 0) s_qn = qn - NT;
-                __syncthreads();
+                __syncthreads();  // the shifted queue and count before the next pass appends
+                apply(e.x, e.y);
             }
         }
     }
@@ -869,6 +872,7 @@ __global__ __launch_bounds__(FX_THREADS) void k_spread3_fxz(SpreadArgs<float> a,
             advance(vlo - a.reach, vlo - a.reach);
         for (int cb = 0; cb < nxy; cb += S3_MAXCOMB)
         {
+            __syncthreads();  // every wave is done with the previous group's prefix
             // ---- this group of (x, y) combinations of the layer: ranges and their prefix
             {
                 const int c = cb + lane;
@@ -937,14 +941,16 @@ __global__ __launch_bounds__(FX_THREADS) void k_spread3_fxz(SpreadArgs<float> a,
                 __syncthreads();
                 // ---- a full queue: every thread applies one point
                 const int qn = s_qn;
+                __syncthreads();  // every wave has the count before a later pass adds to it
                 if (qn >= NT)
                 {
+                    // each thread takes entry tid and moves entry NT + tid down (the
+                    // only reader of slot tid is thread tid itself)
                     const int2 e = s_q[tid];
-                    apply(e.x, e.y);
-                    __syncthreads();
                     if (tid < qn - NT) s_q[tid] = s_q[NT + tid];
                     if (tid == 0) s_qn = qn - NT;
-                    __syncthreads();
+                    __syncthreads();  // the shifted queue and count before the next pass appends
+                    apply(e.x, e.y);
                 }
             }
         }
