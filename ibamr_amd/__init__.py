@@ -40,7 +40,7 @@ ACCUMULATE, OVERWRITE, ACCUMULATE_INTERIOR = 0, 1, 2
 C_API = [
     "ibamr_is_known_kernel", "ibamr_get_stencil_size", "ibamr_get_minimum_ghost_width",
     "ibamr_plan_create", "ibamr_plan_create_patch", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
-    "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_interp",
+    "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_spread_ds", "ibamr_plan_interp",
     "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
     "ibamr_plan_info", "ibamr_plan_set_tile_height", "ibamr_plan_get_tile_height", "ibamr_plan_autotune",
     "ibamr_plan_set_tile_shape", "ibamr_plan_get_tile_shape", "ibamr_plan_get_cell_indices",
@@ -119,6 +119,7 @@ def lib():
     L.ibamr_plan_bin_async.argtypes = [vp, vp, ctypes.c_int, vp]
     L.ibamr_plan_check.argtypes = [vp, vp]
     L.ibamr_plan_spread.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
+    L.ibamr_plan_spread_ds.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
     L.ibamr_plan_interp.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     L.ibamr_plan_fill_periodic_ghosts.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_plan_fill_ghosts.argtypes = [vp, ctypes.c_int, vp]
@@ -358,8 +359,16 @@ class LevelPlan:
         _check(lib().ibamr_plan_get_cell_indices(self._h, _ptr(out), _stream_ptr(stream)))
         return out
 
-    def spread(self, slot, F, X, mode=ACCUMULATE, stream=None):
-        _check(lib().ibamr_plan_spread(self._h, slot, _ptr(F), _ptr(X), mode, _stream_ptr(stream)))
+    def spread(self, slot, F, X, mode=ACCUMULATE, stream=None, ds=None):
+        """LDataManager::spread into the arrays bound to slot; with ds (device
+        [n] of F's type) the points carry F*ds (the ds_data overload)."""
+        if ds is None:
+            _check(lib().ibamr_plan_spread(self._h, slot, _ptr(F), _ptr(X), mode, _stream_ptr(stream)))
+        else:
+            if ds.dtype != F.dtype or ds.numel() != F.shape[0]:
+                raise ValueError("ds must hold one value of F's type per point")
+            _check(lib().ibamr_plan_spread_ds(self._h, slot, _ptr(F), _ptr(X), _ptr(ds.contiguous()), mode,
+                                              _stream_ptr(stream)))
 
     def interp(self, slot, U, X, stream=None):
         _check(lib().ibamr_plan_interp(self._h, slot, _ptr(U), _ptr(X), _stream_ptr(stream)))

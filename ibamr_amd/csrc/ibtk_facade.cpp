@@ -655,6 +655,23 @@ void LDataManager::spread(std::vector<std::shared_ptr<SideData<double>>>& f_data
               "LDataManager::spread");
 }
 
+void LDataManager::spread(std::vector<std::shared_ptr<SideData<double>>>& f_data, const LData<double>& F_data,
+                          const LData<double>& X_data, const LData<double>& ds_data)
+{
+    if (ds_data.getLocalNodeCount() != F_data.getLocalNodeCount() || ds_data.getDepth() != 1)
+        throw Error("LDataManager::spread: ds_data must hold one value per node of F_data");
+    if (d_impl->npoints != X_data.getLocalNodeCount()) endDataRedistribution(X_data);
+    std::vector<void*> a = side_arrays(f_data, d_impl->ndim, d_impl->level.patches.size());
+    if (a != d_impl->bound[0])
+    {
+        abi_check(ibamr_plan_bind_data(d_impl->spread_plan, 0, a.data()), "LDataManager::spread");
+        d_impl->bound[0] = a;
+    }
+    abi_check(ibamr_plan_spread_ds(d_impl->spread_plan, 0, F_data.getArray(), X_data.getArray(), ds_data.getArray(),
+                                   IBAMR_SPREAD_ACCUMULATE_INTERIOR, nullptr),
+              "LDataManager::spread");
+}
+
 void LDataManager::interp(const std::vector<std::shared_ptr<SideData<double>>>& u_data, LData<double>& U_data,
                           const LData<double>& X_data)
 {

@@ -206,7 +206,8 @@ __global__ void k_copy_int(const int* __restrict__ a, int n, int* __restrict__ b
 // faster than the scattered-write form for these 24 + 12-byte records.
 template <int D, typename T>
 __global__ void k_gather_points(const int* __restrict__ perm, int n, const double* __restrict__ X,
-                                double* __restrict__ Xs, const T* __restrict__ F, T* __restrict__ Fs)
+                                double* __restrict__ Xs, const T* __restrict__ F, T* __restrict__ Fs,
+                                const T* __restrict__ ds = nullptr)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -221,6 +222,12 @@ __global__ void k_gather_points(const int* __restrict__ perm, int n, const doubl
         T f[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) f[d] = F[D * p + d];
+        if (ds)  // F_ds[k][d] = F[k][d] * ds[k]  (LDataManager.cpp:439-445)
+        {
+            const T w = ds[p];
+#pragma unroll
+            for (int d = 0; d < D; ++d) f[d] = f[d] * w;
+        }
 #pragma unroll
         for (int d = 0; d < D; ++d) Fs[D * (size_t)i + d] = f[d];
     }

@@ -12,16 +12,25 @@ namespace
 // Driven by the points: read X and F in the caller's order (coalesced) and
 // write each point's record to its slot (a scattered write of a few bytes
 // costs less than a scattered read of whole cache lines).
+// ds (optional): F_ds[k][c] = F[k][c] * ds[k] (LDataManager.cpp:439-445).
 template <typename T>
 __global__ void k_scatter_points(const int* __restrict__ inv, int n, int ndim, const double* __restrict__ X,
-                                 double* __restrict__ Xs, int fdepth, const T* __restrict__ F, T* __restrict__ Fs)
+                                 double* __restrict__ Xs, int fdepth, const T* __restrict__ F, T* __restrict__ Fs,
+                                 const T* __restrict__ ds)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const size_t k = (size_t)inv[p];
     for (int d = 0; d < ndim; ++d) Xs[(size_t)ndim * k + d] = X[(size_t)ndim * p + d];
     if (F)
-        for (int c = 0; c < fdepth; ++c) Fs[(size_t)fdepth * k + c] = F[(size_t)fdepth * p + c];
+    {
+        const T w = ds ? ds[p] : T(1);
+        for (int c = 0; c < fdepth; ++c)
+        {
+            const T f = F[(size_t)fdepth * p + c];
+            Fs[(size_t)fdepth * k + c] = ds ? f * w : f;
+        }
+    }
 }
 
 // ----------------------------------------------------------------------------- spread
@@ -547,6 +556,12 @@ extern "C" {
 
 int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, int mode, void* stream)
 {
+    return ibamr_plan_spread_ds(P, slot, F, X, nullptr, mode, stream);
+}
+
+int ibamr_plan_spread_ds(ibamr_plan* P, int slot, const void* F, const double* X, const void* ds, int mode,
+                         void* stream)
+{
     if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
     if (slot < 0 || slot >= MAX_SLOTS || !P->arrays_d[slot]) return fail(IBAMR_ERR_INVALID, "slot not bound");
     if (P->npoints < 0) return fail(IBAMR_ERR_NOT_BINNED, "points not binned");
@@ -568,11 +583,12 @@ int ibamr_plan_spread(ibamr_plan* P, int slot, const void* F, const double* X, i
             // the points in bin order
             if (P->fdepth == NDIM)
                 hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
-                                   P->npoints, X, P->Xs_d, static_cast<const T*>(F), static_cast<T*>(P->Fs_d));
+                                   P->npoints, X, P->Xs_d, static_cast<const T*>(F), static_cast<T*>(P->Fs_d),
+                                   static_cast<const T*>(ds));
             else
                 hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
                                    P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
-                                   static_cast<T*>(P->Fs_d));
+                                   static_cast<T*>(P->Fs_d), static_cast<const T*>(ds));
             HIP_OK(hipGetLastError());
             P->xs_valid = true;
         }

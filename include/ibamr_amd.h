@@ -170,6 +170,15 @@ int ibamr_plan_get_cell_indices(const ibamr_plan* plan, int* cells, void* stream
  * spread's zero, spread, add, LDataManager.cpp:594, 661-663). */
 int ibamr_plan_spread(ibamr_plan* plan, int slot, const void* F, const double* X, int mode, void* stream);
 
+/* LDataManager::spread with a per-point weight ds (LDataManager.cpp:394-464):
+ * spreads F_ds[k][c] = F[k][c] * ds[k], the product formed in the plan's
+ * force precision (double for FP64, as the reference forms it; float for
+ * FP32C) while the points are put in bin order, so it costs no pass of its
+ * own.  ds: [n] of the force's type, the caller's point order; NULL spreads F
+ * itself (ibamr_plan_spread). */
+int ibamr_plan_spread_ds(ibamr_plan* plan, int slot, const void* F, const double* X, const void* ds, int mode,
+                         void* stream);
+
 /* LDataManager::interp for one level (LDataManager.cpp:748-802): each point
  * is interpolated from the patch whose interior holds its cell; ghost values
  * must be filled beforehand (see ibamr_plan_fill_periodic_ghosts).  U is

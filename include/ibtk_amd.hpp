@@ -236,6 +236,11 @@ public:
     // interiors, patch ghost values unchanged.  f_data[p] is patch p's data.
     void spread(std::vector<std::shared_ptr<SideData<double>>>& f_data, const LData<double>& F_data,
                 const LData<double>& X_data);
+    // LDataManager::spread with ds_data (LDataManager.cpp:394-464): spreads
+    // F*ds, F_ds[k][d] = F[k][d] * ds[k], formed while the points are put in
+    // bin order (no separate pass over the points).
+    void spread(std::vector<std::shared_ptr<SideData<double>>>& f_data, const LData<double>& F_data,
+                const LData<double>& X_data, const LData<double>& ds_data);
     // LDataManager::interp (LDataManager.cpp:699-813): U <- J u for every point
     // in a patch interior; u's ghosts must be filled (fillPeriodicGhosts does it
     // for a fully periodic single-patch level).
