@@ -18,6 +18,7 @@ KERNELS = {"PIECEWISE_LINEAR": 0, "IB_4": 1, "IB_6": 2, "BSPLINE_3": 3, "BSPLINE
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
 
 
 def build():
@@ -27,7 +28,9 @@ def build():
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # IBAMR_ORACLE_LIB: another build of the same sources (the bench's
+        # host-tuned CPU baseline, `make native`)
+        path = os.environ.get("IBAMR_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
@@ -59,7 +62,8 @@ def lib():
         L.orc_build_patch_entries.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip,
                                               _ip, _dp, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.orc_order_entries_z.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, _dp, _dp, _ip, _ip,
-                                          ctypes.c_int, _ip]
+                                          ctypes.c_int, _lp, _ip]
+        L.orc_bin_ranks.argtypes = [_ip, ctypes.c_int, _ip, _ip, _lp]
         L.orc_bruteforce_spread_at.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
                                                ctypes.c_int]
         L.orc_bruteforce_spread_at.restype = ctypes.c_double
@@ -249,22 +253,35 @@ def bruteforce_spread_at(kernel, xg, dx, period, X, V):
                                           _d(X), _d(_dv(V)), X.shape[0])
 
 
-def order_entries_z(kernel, cell, indices, Xshift, X, x_lower, dx, domain_lower, bin_size, chunk=64):
+def bin_ranks(cell, domain_lower, bin_size):
+    """orc_bin_ranks: each point's position among its bin's points (cell's z
+    plane inside the bin, then index); shared by all patches of a level."""
+    cell = _iv(cell).reshape(-1, 3)
+    out = np.zeros(max(1, cell.shape[0]), dtype=np.int64)
+    lib().orc_bin_ranks(_i(cell), cell.shape[0], _i(_iv(domain_lower)), _i(_iv(bin_size)),
+                        out.ctypes.data_as(_lp))
+    return out
+
+
+def order_entries_z(kernel, cell, indices, Xshift, X, x_lower, dx, domain_lower, bin_size, chunk=64, ranks=None):
     """Permutation of a canonical 3-D entry list into the GPU spread order for one
-    array (bin/image group, 64-point chunk of the bin, stencil z-start, point)."""
+    array (bin/image group, 64-point chunk of the bin, stencil z-start, point).
+    ranks: bin_ranks(cell, domain_lower, bin_size), computed here when None."""
     cell = _iv(cell).reshape(-1, 3)
     indices = _iv(indices)
     Xshift = _dv(Xshift).reshape(-1, 3)
     X = _dv(X).reshape(-1, 3)
     perm = np.zeros(len(indices), dtype=np.int32)
+    if ranks is None:
+        ranks = bin_ranks(cell, domain_lower, bin_size)
     lib().orc_order_entries_z(kernel_id(kernel), _i(cell), cell.shape[0], _i(indices), _d(Xshift), len(indices), _d(X),
                               _d(_dv(x_lower)), _d(_dv(dx)), _i(_iv(domain_lower)), _i(_iv(bin_size)), int(chunk),
-                              _i(perm))
+                              np.ascontiguousarray(ranks, dtype=np.int64).ctypes.data_as(_lp), _i(perm))
     return perm
 
 
 def spread_patch(kernel, centering_side, dx, x_lower, ilower, iupper, gcw, arrays, X, F, cells, idx_gcw,
-                 periodic_shift, domain_lower, depth=1, bin_size=None):
+                 periodic_shift, domain_lower, depth=1, bin_size=None, ranks=None):
     """LEInteractor::spread of one patch over its Lagrangian ghost box (entries
     with periodic images, LIndexSetData model) into `arrays` (side: one float64
     array per axis; cell: one (depth, ...) array), in the summation order the
@@ -279,6 +296,8 @@ def spread_patch(kernel, centering_side, dx, x_lower, ilower, iupper, gcw, array
         return arrays
     X = _dv(X).reshape(-1, ndim)
     F = _dv(F).reshape(X.shape[0], -1)
+    if ndim == 3 and ranks is None:
+        ranks = bin_ranks(cells, domain_lower, bin_size)
     if centering_side:
         for a in range(ndim):
             xl = list(x_lower)
@@ -286,7 +305,7 @@ def spread_patch(kernel, centering_side, dx, x_lower, ilower, iupper, gcw, array
             iu = list(iupper)
             iu[a] += 1
             if ndim == 3:
-                perm = order_entries_z(kernel, cells, idx, sh, X, xl, dx, domain_lower, bin_size)
+                perm = order_entries_z(kernel, cells, idx, sh, X, xl, dx, domain_lower, bin_size, ranks=ranks)
                 ii, ss = idx[perm], sh[perm]
             else:
                 ii, ss = idx, sh
@@ -294,7 +313,7 @@ def spread_patch(kernel, centering_side, dx, x_lower, ilower, iupper, gcw, array
             spread(kernel, dx, xl, ilower, iu, gcw, q, X, np.ascontiguousarray(F[:, a]), indices=ii, Xshift=ss)
     else:
         if ndim == 3:
-            perm = order_entries_z(kernel, cells, idx, sh, X, x_lower, dx, domain_lower, bin_size)
+            perm = order_entries_z(kernel, cells, idx, sh, X, x_lower, dx, domain_lower, bin_size, ranks=ranks)
             idx, sh = idx[perm], sh[perm]
         spread(kernel, dx, x_lower, ilower, iupper, gcw, arrays, X, F, indices=idx, Xshift=sh, depth=depth)
     return arrays
