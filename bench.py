@@ -44,6 +44,8 @@ def parse():
                     help="points of the all-cores CPU baseline (at most the workload's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="default workload only: skip the AMR-level and multi-structure summaries")
     ap.add_argument("--point-order", choices=["regrid", "generated"], default="regrid",
                     help="regrid: the points in LDataManager's order after a redistribution (the reference's "
                          "steady state); generated: the generator's order, random in space")
@@ -119,9 +121,16 @@ def make_points(torch, n, N, seed):
     return X, F
 
 
-def touched_faces(torch, X, N, kernel):
+def spread_kernel_name(shape):
+    """The FP32C 3-D spread kernel a tile shape runs: z extents of 32 and more
+    are column segments swept through an LDS window, the rest LDS boxes."""
+    return "k_spread3_fxz" if shape[2] >= 32 else "k_spread3_fx"
+
+
+def touched_faces(torch, X, N, kernel, periodic=True):
     """T: distinct (axis, face) grid values in the union of all point stencils
-    (periodic DOFs), counted exactly on the GPU."""
+    (periodic DOFs, or the faces of the N^3 index space and its border when not
+    periodic), counted exactly on the GPU."""
     W = {"IB_4": 4, "IB_6": 6, "BSPLINE_3": 3}[kernel]
     h = 1.0 / N
     total = 0
@@ -135,14 +144,17 @@ def touched_faces(torch, X, N, kernel):
             else:
                 s = torch.floor(t + 0.5) - (W // 2)   # NINT for t >= 0
             starts.append(s.to(torch.int64))
-        mask = torch.zeros(N * N * N, dtype=torch.bool, device="cuda")
+        # not periodic: indices shifted by W into an (N + 2W)^3 box, no wrap
+        M, off = (N, 0) if periodic else (N + 2 * W, W)
+        wrap = (lambda v: torch.remainder(v, N)) if periodic else (lambda v: v + off)
+        mask = torch.zeros(M * M * M, dtype=torch.bool, device="cuda")
         for k2 in range(W):
-            i2 = torch.remainder(starts[2] + k2, N)
+            i2 = wrap(starts[2] + k2)
             for k1 in range(W):
-                i1 = torch.remainder(starts[1] + k1, N)
-                base = (i2 * N + i1) * N
+                i1 = wrap(starts[1] + k1)
+                base = (i2 * M + i1) * M
                 for k0 in range(W):
-                    mask[base + torch.remainder(starts[0] + k0, N)] = True
+                    mask[base + wrap(starts[0] + k0)] = True
         total += int(mask.sum().item())
         del mask
     return total
@@ -244,6 +256,37 @@ def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
     return {"plan": plan, "tune": tune, "f": f, "u": u, "U": U, "wall_s": wall, "spread_ms": sum(sp) / len(sp),
             "interp_ms": sum(it) / len(it), "bin_ms": bin_ms, "info": plan.info(),
             "spread_kernel_ms": kt["spread_ms"], "interp_kernel_ms": kt["interp_ms"]}
+
+
+def host_oracle():
+    """Builds the oracle for this host (oracle/Makefile `native`: -O3
+    -march=native -ffp-contract=off) into a temporary directory and points the
+    oracle loader at it; returns a description of the build (flags, CPU
+    model).  A build that fails leaves the plain -O2 library in use and says
+    so."""
+    import subprocess
+    import tempfile
+
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    odir = os.path.join(ROOT, "oracle")
+    out = os.path.join(tempfile.mkdtemp(prefix="ibamr_oracle_"), "liboracle.so")
+    try:
+        subprocess.run(["make", "-s", "-C", odir, "native", "NATIVE_OUT=" + out], check=True, capture_output=True,
+                       timeout=300)
+        flags = subprocess.run(["make", "-s", "-C", odir, "native-flags"], check=True, capture_output=True,
+                               text=True, timeout=60).stdout.strip()
+        os.environ["IBAMR_ORACLE_LIB"] = out
+    except (subprocess.SubprocessError, OSError) as e:
+        flags = "-O2 -ffp-contract=off (host build failed: %s)" % type(e).__name__
+    return {"cpu_model": model, "build": "gcc " + flags}
 
 
 def cpu_baseline_cores(X_np, F_np, N, kernel, sample, threads, seed=6):
@@ -358,15 +401,18 @@ def run_amr_form(torch, ia, precision, level, X, F, kernel, steps, warmup, seed=
     kt = plan.kernel_times()
     plan.set_timing(False)
     return {"f": f, "U": U, "ms": e0.elapsed_time(e1) / steps, "kt": kt, "info": plan.info(), "tune": tune,
-            "height": plan.tile_height}
+            "shape": plan.tile_shape}
 
 
-def main_amr(args, torch, ia):
+def bench_amr(args, torch, ia):
+    """The AMR fine level (coarse 128^3, ratio 4, 240 patches around 4 shells,
+    2^21 points, BSPLINE_3): FP32C and FP64 spread + interp, one launch each
+    per level.  Returns the bench record."""
     import numpy as np
     from ibamr_amd import synthetic
 
     kernel = "BSPLINE_3"
-    level, Xn = synthetic.amr_level(coarse=128, ratio=4, npoints=1 << 21, nshell=4)
+    level, Xn = synthetic.amr_level(coarse=128, ratio=4, npoints=1 << 21, nshell=4, npatches=240)
     n = Xn.shape[0]
     cells = sum(int(np.prod([p.iupper[d] - p.ilower[d] + 1 for d in range(3)])) for p in level.patches)
     X = torch.tensor(Xn, device="cuda")
@@ -389,6 +435,14 @@ def main_amr(args, torch, ia):
     err_f = max(float((a.double() - b).abs().max()) for a, b in zip(r32["f"], r64["f"])) / \
         max(float(b.abs().max()) for b in r64["f"])
     err_u = float((r32["U"].double() - r64["U"]).abs().max() / r64["U"].abs().max())
+    # dominant kernel against the HBM roofline: N (8d + 4d) + 4 T bytes per launch,
+    # T = distinct grid values in the union of the stencils (fine level index space)
+    T = touched_faces(torch, X, 128 * 4, kernel, periodic=False)
+    B = n * (8 * 3 + 4 * 3) + 4 * T
+    dom = "spread" if r32["kt"]["spread_ms"] >= r32["kt"]["interp_ms"] else "interp"
+    t_dom = r32["kt"][dom + "_ms"]
+    kname = {"spread": spread_kernel_name(r32["shape"]), "interp": "k_interp3"}[dom]
+    achieved = B / (t_dom * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": n / (r32["ms"] * 1e-3), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r32["ms"], "higher_is_better": True, "scaling": "weak",
@@ -398,17 +452,24 @@ def main_amr(args, torch, ia):
                                % (len(level.patches), cells, n),
                    "patches": len(level.patches), "fine_cells": cells, "points": n, "kernel": kernel,
                    "point_order": args.point_order, "spread_tiles": r32["info"]["ntiles"],
-                   "tile": [r32["info"]["tile"], r32["info"]["tile"], r32["height"]],
-                   "tile_autotune_ms": {str(h): t for h, t in r32["tune"].items()},
+                   "tile": list(r32["shape"]),
+                   "tile_autotune_ms": {"x".join(map(str, h)): t for h, t in r32["tune"].items()},
                    "bins": "%d^3 cells, sorted (CSR: no capacity limit)" % r32["info"]["tile"]},
         "detail": {"spread_kernel_ms": r32["kt"]["spread_ms"], "interp_kernel_ms": r32["kt"]["interp_ms"],
                    "fp64_points_per_s": n / (r64["ms"] * 1e-3), "fp64_spread_kernel_ms": r64["kt"]["spread_ms"],
                    "fp64_interp_kernel_ms": r64["kt"]["interp_ms"], "fp32c_over_fp64": r64["ms"] / r32["ms"],
-                   "fp64_tile_autotune_ms": {str(h): t for h, t in r64["tune"].items()},
-                   "fp32c_rel_linf_err_spread": err_f, "fp32c_rel_linf_err_interp": err_u},
-        "roofline": None, "cpu_baseline": None,
+                   "fp64_tile_autotune_ms": {"x".join(map(str, h)): t for h, t in r64["tune"].items()},
+                   "fp32c_rel_linf_err_spread": err_f, "fp32c_rel_linf_err_interp": err_u,
+                   "touched_faces": T},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": B},
+        "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def main_amr(args, torch, ia):
+    print(json.dumps(bench_amr(args, torch, ia)), flush=True)
 
 
 def time_forces(torch, counts, X0, reps=20):
@@ -447,11 +508,12 @@ def time_forces(torch, counts, X0, reps=20):
     return e0.elapsed_time(e1) / reps
 
 
-def main_multi(args, torch, ia):
+def bench_multi(args, torch, ia):
     """64 ellipsoidal shells on a periodic 384^3 grid, shells 0-31 IB_4 and 32-63
     BSPLINE_3, point counts log-uniform in [2000, 200000]; 100 steps of
     spread -> u = u_bg + alpha f -> ghost fill -> interp -> X += dt U, each step
-    one replay of a captured HIP graph.  FP32C and FP64 trajectories compared."""
+    one replay of a captured HIP graph.  FP32C and FP64 trajectories compared.
+    Returns the bench record."""
     import math
 
     import numpy as np
@@ -511,6 +573,13 @@ def main_multi(args, torch, ia):
     drift = float((res[ia.FP32C]["X"] - res[ia.FP64]["X"]).abs().max()) / h
     ms = res[ia.FP32C]["ms"]
     forces_ms = time_forces(torch, counts, X0_nodes)
+    # the step against the HBM roofline: per kernel group a spread and an
+    # interpolation, each N (8d + 4d) + 4 T bytes (T at the initial positions)
+    n4 = sum(c for k, c in structs if k == "IB_4")
+    B = 0
+    for kern, Xg in (("IB_4", X0[:n4]), ("BSPLINE_3", X0[n4:])):
+        B += 2 * (Xg.shape[0] * (8 * 3 + 4 * 3) + 4 * touched_faces(torch, Xg, N, kern))
+    achieved = B / (ms * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "points/s", "n_gpus": 1, "steps": steps, "warmup": 1,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
@@ -519,16 +588,44 @@ def main_multi(args, torch, ia):
                                "side grid; step = bin + spread + u=u_bg+alpha*f + ghost fill + interp + X+=dtU, "
                                "one HIP-graph replay per step, %d steps, FP32 compensated" % (n, steps),
                    "points": n, "grid": N, "structures": nshell, "point_order": args.point_order,
-                   "tile_autotune_ms": {k: {str(h): t for h, t in v.items()} for k, v in tunes[ia.FP32C].items()}},
-        "detail": {"fp64_ms_per_step": res[ia.FP64]["ms"],
-                   "fp64_tile_autotune_ms": {k: {str(h): t for h, t in v.items()} for k, v in tunes[ia.FP64].items()}, "fp32c_over_fp64": res[ia.FP64]["ms"] / ms,
+                   "tile_autotune_ms": {k: {"x".join(map(str, h)): t for h, t in v.items()}
+                                        for k, v in tunes[ia.FP32C].items()}},
+        "detail": {"fp64_ms_per_step": res[ia.FP64]["ms"], "fp64_points_per_s": n / (res[ia.FP64]["ms"] * 1e-3),
+                   "fp64_tile_autotune_ms": {k: {"x".join(map(str, h)): t for h, t in v.items()}
+                                             for k, v in tunes[ia.FP64].items()},
+                   "fp32c_over_fp64": res[ia.FP64]["ms"] / ms,
                    "dt": res[ia.FP32C]["dt"], "fp32c_vs_fp64_position_drift_cells_after_%d_steps" % steps: drift,
                    "force_generation_ms": forces_ms,
                    "force_generation": "ring springs + beams per shell, a target point every 4th point, "
                                        "timed alone (not in the step above)"},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "kernel": "step (bin, spread, grid update, ghost fill, interp, Euler update; "
+                                              "both kernel groups)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": B},
+        "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def main_multi(args, torch, ia):
+    print(json.dumps(bench_multi(args, torch, ia)), flush=True)
+
+
+def summary(rec):
+    """The figures of a workload's record that the default run reports beside
+    its own: throughput, FP32C/FP64, error or drift, roofline fraction."""
+    d = rec["detail"]
+    out = {"workload": rec["config"]["workload"], "points_per_s": rec["value"], "ms_per_step": rec["ms_per_step"],
+           "fp64_points_per_s": d.get("fp64_points_per_s"), "fp32c_over_fp64": d.get("fp32c_over_fp64"),
+           "roofline_kernel": rec["roofline"]["kernel"], "roofline_frac": rec["roofline"]["frac"],
+           "roofline_achieved_GBs": rec["roofline"]["achieved"]}
+    for k, v in d.items():
+        if k.startswith("fp32c_rel_linf_err") or k.startswith("fp32c_vs_fp64_position_drift") or k.endswith(
+                "kernel_ms"):
+            out[k] = v
+    if "tile" in rec["config"]:
+        out["tile"] = rec["config"]["tile"]
+    return out
 
 
 def main():
@@ -567,7 +664,7 @@ def main():
     B_dom = B_spread if dom == "spread" else B_interp
     t_dom = r32["spread_kernel_ms"] if dom == "spread" else r32["interp_kernel_ms"]
     achieved = B_dom / (t_dom * 1e-3) / 1e9
-    kname = ("k_spread3_fxz" if r32["plan"].tile_height >= 32 else "k_spread3_fx") if dom == "spread" else "k_interp3"
+    kname = spread_kernel_name(r32["plan"].tile_shape) if dom == "spread" else "k_interp3"
     tag = "%s_%dc_side_%s_%dpts_fp32c" % (kname, N, kernel.lower(), n)
     tr = traffic_from_profile(tag)
 
@@ -619,18 +716,28 @@ def main():
         del r64
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
+        host = host_oracle()
         Xh, Fh = X.cpu().numpy(), F64.cpu().numpy()
-        one = cpu_baseline(Xh, Fh, N, kernel, args.cpu_sample)
+        one = dict(cpu_baseline(Xh, Fh, N, kernel, args.cpu_sample), **host)
         cores = min(args.cpu_cores, os.cpu_count() or 1)
         if cores > 1:
             # the reference's CPU path on the host's cores (one patch per core), the baseline reported;
             # the one-core figure goes in detail
-            out["cpu_baseline"] = cpu_baseline_cores(Xh, Fh, N, kernel, args.cpu_sample_cores, cores)
+            out["cpu_baseline"] = dict(cpu_baseline_cores(Xh, Fh, N, kernel, args.cpu_sample_cores, cores), **host)
             out["detail"]["cpu_one_core"] = one
         else:
             out["cpu_baseline"] = one
     else:
         out["cpu_baseline"] = None
+    if not args.no_others:
+        # the AMR-level and multi-structure workloads, summarised beside the headline one
+        del r32
+        torch.cuda.empty_cache()
+        print("bench: AMR level", file=sys.stderr, flush=True)
+        out["detail"]["amr_level"] = summary(bench_amr(args, torch, ia))
+        torch.cuda.empty_cache()
+        print("bench: multi-structure step", file=sys.stderr, flush=True)
+        out["detail"]["multi_structure"] = summary(bench_multi(args, torch, ia))
     print(json.dumps(out), flush=True)
 
 

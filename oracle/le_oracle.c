@@ -675,11 +675,8 @@ static int cmp_rkey_bin(const void* a, const void* b)
     return x->q < y->q ? -1 : (x->q > y->q);
 }
 
-void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* indices, const double* Xshift,
-                         int nentries, const double* X, const double* x_lower, const double* dx,
-                         const int* domain_lower, const int* bin, int chunk, int* perm)
+void orc_bin_ranks(const int* cell, int npoints, const int* domain_lower, const int* bin, long* rank_of)
 {
-    zkey_t* z = (zkey_t*)malloc(sizeof(zkey_t) * (size_t)(nentries > 0 ? nentries : 1));
     /* rank of every point among the points of its bin in the library's bin order
      * (cell's z plane inside the bin, then index; the GPU cuts that whole list into chunks) */
     const int maxp = npoints;
@@ -693,7 +690,6 @@ void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* in
         }
     const long nbx = bmax[0] - bmin[0] + 1, nby = bmax[1] - bmin[1] + 1, nbz = bmax[2] - bmin[2] + 1;
     rkey_t* rk = (rkey_t*)malloc(sizeof(rkey_t) * (size_t)(maxp > 0 ? maxp : 1));
-    long* rank_of = (long*)malloc(sizeof(long) * (size_t)(maxp > 0 ? maxp : 1));
     for (int q = 0; q < maxp; ++q)
     {
         long id = 0;
@@ -713,6 +709,20 @@ void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* in
         rank_of[rk[i].q] = run;
     }
     free(rk);
+}
+
+void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* indices, const double* Xshift,
+                         int nentries, const double* X, const double* x_lower, const double* dx,
+                         const int* domain_lower, const int* bin, int chunk, const long* ranks, int* perm)
+{
+    zkey_t* z = (zkey_t*)malloc(sizeof(zkey_t) * (size_t)(nentries > 0 ? nentries : 1));
+    long* own = NULL;
+    if (!ranks)
+    {
+        own = (long*)malloc(sizeof(long) * (size_t)(npoints > 0 ? npoints : 1));
+        orc_bin_ranks(cell, npoints, domain_lower, bin, own);
+        ranks = own;
+    }
     long group = -1;
     int prev_b[3] = {0, 0, 0}, prev_o[3] = {0, 0, 0};
     for (int l = 0; l < nentries; ++l)
@@ -737,7 +747,7 @@ void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* in
                 prev_o[d] = o[d];
             }
         }
-        const long rank = rank_of[p];
+        const long rank = ranks[p];
         double w[8];
         const double Xs = X[3 * p + 2] + Xshift[3 * l + 2];
         z[l].key[0] = group;
@@ -748,7 +758,7 @@ void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* in
     }
     qsort(z, (size_t)nentries, sizeof(zkey_t), cmp_zkey);
     for (int l = 0; l < nentries; ++l) perm[l] = z[l].pos;
-    free(rank_of);
+    free(own);
     free(z);
 }
 

@@ -136,7 +136,13 @@ int orc_build_patch_entries(int ndim, const int* cell, int npoints, const int* i
  * receives the new order as positions into indices[]. */
 void orc_order_entries_z(int kernel, const int* cell, int npoints, const int* indices, const double* Xshift,
                          int nentries, const double* X, const double* x_lower, const double* dx,
-                         const int* domain_lower, const int* bin, int chunk, int* perm);
+                         const int* domain_lower, const int* bin, int chunk, const long* ranks, int* perm);
+
+/* The ranks orc_order_entries_z uses (ranks == NULL computes them per call):
+ * rank_of[q] = position of point q among the points of its bin, ordered by the
+ * cell's z plane inside the bin, then point index.  They depend only on the
+ * cells and the bins, so a level's patches can share one computation. */
+void orc_bin_ranks(const int* cell, int npoints, const int* domain_lower, const int* bin, long* rank_of);
 
 /* Brute-force O(N*G) evaluation with the scalar delta functions: value of the
  * spread of point values V (depth 1) at one grid location, periodic images

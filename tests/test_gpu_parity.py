@@ -10,7 +10,9 @@ FP64 plans must match the oracle bitwise:
 FP32C plans: relative L-infinity error against the FP64 oracle <= 1e-6 (the
 bar of the precision form).
 """
+import os
 import zlib
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -67,14 +69,28 @@ def _oracle_spread(orc, level, plan, arrays_np, X, F, kernel):
     cells = orc.get_cell_index(X, level.domain_x_lower, level.domain_x_upper, level.dx, level.domain_lower,
                                level.domain_upper)
     out = [a.copy() for a in arrays_np]
-    for pi, P in enumerate(level.patches):
-        if plan.centering == ia.SIDE:
-            q = out[pi * ndim:(pi + 1) * ndim]
-        else:
-            q = out[pi]
+    # each point's rank in its bin: one computation for all the level's patches
+    ranks = orc.bin_ranks(cells, level.domain_lower, [8] * 3) if ndim == 3 else None
+
+    def one(pi):
+        P = level.patches[pi]
+        q = out[pi * ndim:(pi + 1) * ndim] if plan.centering == ia.SIDE else out[pi]
         orc.spread_patch(kernel, plan.centering == ia.SIDE, level.dx, P.x_lower, P.ilower, P.iupper, plan.data_gcw,
-                         q, X, F, cells, igcw, level.periodic_shift, level.domain_lower, depth=plan.depth)
+                         q, X, F, cells, igcw, level.periodic_shift, level.domain_lower, depth=plan.depth,
+                         ranks=ranks)
+
+    _over_patches(one, len(level.patches))  # every patch writes only its own arrays
     return out
+
+
+def _over_patches(fn, npatches):
+    """fn(p) for every patch on host threads (the oracle's C calls release the GIL)."""
+    if npatches < 4:
+        for p in range(npatches):
+            fn(p)
+        return
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(fn, range(npatches)))
 
 
 def _oracle_interp(orc, level, plan, arrays_np, X, kernel):
@@ -83,11 +99,13 @@ def _oracle_interp(orc, level, plan, arrays_np, X, kernel):
                                level.domain_upper)
     fdepth = ndim if plan.centering == ia.SIDE else plan.depth
     U = np.full((X.shape[0], fdepth), np.nan)
-    for pi, P in enumerate(level.patches):
+
+    def one(pi):
+        P = level.patches[pi]
         idx, sh = orc.build_patch_entries(cells, P.ilower, P.iupper, 0, level.periodic_shift, level.domain_lower,
                                           level.dx, [_tile(ndim)] * ndim, interior_only=True)
         if len(idx) == 0:
-            continue
+            return
         if plan.centering == ia.SIDE:
             q = arrays_np[pi * ndim:(pi + 1) * ndim]
             V = orc.interp_side(kernel, level.dx, P.x_lower, P.ilower, P.iupper, plan.data_gcw, q, X, indices=idx,
@@ -95,7 +113,9 @@ def _oracle_interp(orc, level, plan, arrays_np, X, kernel):
         else:
             V = orc.interp(kernel, level.dx, P.x_lower, P.ilower, P.iupper, plan.data_gcw, arrays_np[pi], X,
                            indices=idx, Xshift=sh, depth=plan.depth)
-        U[idx] = V[idx]
+        U[idx] = V[idx]  # the patches' interior points are disjoint
+
+    _over_patches(one, len(level.patches))
     return U
 
 
