@@ -122,9 +122,8 @@ def make_points(torch, n, N, seed):
 
 
 def spread_kernel_name(shape):
-    """The FP32C 3-D spread kernel a tile shape runs: z extents of 32 and more
-    are column segments swept through an LDS window, the rest LDS boxes."""
-    return "k_spread3_fxz" if shape[2] >= 32 else "k_spread3_fx"
+    """The 3-D spread kernel (every tile shape runs the register-column kernel)."""
+    return "k_spread3"
 
 
 def touched_faces(torch, X, N, kernel, periodic=True):

@@ -245,26 +245,6 @@ __global__ void k_bin_bounds(const int* __restrict__ keys_sorted, int n, int m, 
     start[b] = lo;
 }
 
-// fx[0] = the largest number of points in one bin (bins [0, m)), for the
-// fixed-point scale of the FP32C spread; order-independent (integer atomicMax)
-__global__ __launch_bounds__(256) void k_max_bin_count(const int* __restrict__ start, int m, unsigned* __restrict__ fx)
-{
-    __shared__ int s_m[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int c = 0;
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < m; b += gridDim.x * blockDim.x)
-        c = max(c, start[b + 1] - start[b]);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off));
-    if (lane == 0) s_m[wv] = c;
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        const int v = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
-        if (v > 0) atomicMax(fx, (unsigned)v);
-    }
-}
-
 // out[ndim * p + d] = coordinate d of point p's cell (caller's order)
 __global__ void k_cells_out(const int4* __restrict__ cell, int n, int ndim, int* __restrict__ out)
 {
@@ -339,10 +319,9 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     hipStream_t s = static_cast<hipStream_t>(stream);
     P->npoints = npoints;
     P->xs_valid = false;
+    P->ws_valid = false;
     const int B = 256;
     const int m = P->nmem + 1;  // bins plus the out-of-domain sentinel
-    if (P->fx_d)  // FP32C 3-D: the largest bin count, set below (fill kernels, not memsets: graph-safe)
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(P->fx_d), 1, 0);
     if (npoints == 0)
     {
         hipLaunchKernelGGL(k_fill_int, dim3((m + 1 + B - 1) / B), dim3(B), 0, s, P->bin_start_d, m + 1, 0);
@@ -386,12 +365,6 @@ int ibamr_plan_bin_async(ibamr_plan* P, const double* X, int npoints, void* stre
     hipLaunchKernelGGL(k_bin_bounds, dim3((m + 1 + B - 1) / B), dim3(B), 0, s, P->keys_sorted_d, npoints, m,
                        P->ksub, P->bin_start_d);
     HIP_OK(hipGetLastError());
-    if (P->fx_d)
-    {
-        hipLaunchKernelGGL(k_max_bin_count, dim3(std::min((P->nmem + B - 1) / B, 1024)), dim3(B), 0, s,
-                           P->bin_start_d, P->nmem, P->fx_d);
-        HIP_OK(hipGetLastError());
-    }
     // idx_d is free once the sort is done: it now holds the inverse permutation
     hipLaunchKernelGGL(k_gather_cells, dim3(G), dim3(B), 0, s, P->perm_d, P->cell_d, npoints, P->cell_sorted_d,
                        P->idx_d);

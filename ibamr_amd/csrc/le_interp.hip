@@ -31,6 +31,13 @@ struct InterpArgs
     int fdepth;
     int depth;
     int side;
+    // FP32C 3-D: the points' stencil starts and weights in the level's frame
+    // (null: evaluate them from X and the level's x_lower / lower index)
+    const float* wt;
+    const int* st;
+    int nvar;
+    double lev_xlo[3];
+    int lev_lo[3];
 };
 
 // Compensated (FP32) or plain (FP64, same order as the reference) sum.
@@ -179,9 +186,13 @@ int interp_prepare(int ndim, int kernel, int precision)
     return dispatch(3, kernel, precision, [&](auto, auto ker, auto prec) -> int {
         using T = decltype(prec);
         constexpr int KER = decltype(ker)::value;
-        HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_interp3<KER, T>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   interp3_lds_elems<KER, T>() * (int)sizeof(T)));
+        // the variant reading the set-up pass's stencils exists for FP32C only
+        const void* fs[2] = {reinterpret_cast<const void*>(&k_interp3<KER, T, false>), nullptr};
+        if constexpr (sizeof(T) == 4) fs[1] = reinterpret_cast<const void*>(&k_interp3<KER, T, true>);
+        for (const void* f : fs)
+            if (f)
+                HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           interp3_lds_elems<KER, T>() * (int)sizeof(T)));
         return IBAMR_OK;
     });
 }
@@ -218,9 +229,18 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
                                P->npoints, X, P->Xs_d, static_cast<const T*>(nullptr), static_cast<T*>(nullptr));
             HIP_OK(hipGetLastError());
             P->xs_valid = true;
+            P->ws_valid = false;
         }
         // FP32C: the spread set-up's stencils of these positions when it computed
         // them, else the kernel evaluates the same ones (level frame)
+        a.wt = P->ws_valid ? P->wt_d : nullptr;
+        a.st = P->ws_valid ? P->st_d : nullptr;
+        a.nvar = P->nvar;
+        for (int d = 0; d < 3; ++d)
+        {
+            a.lev_xlo[d] = P->dom_xlo[d];
+            a.lev_lo[d] = P->dom_lo[d];
+        }
         a.X = P->Xs_d;
         a.U = static_cast<T*>(U);
         a.cells = P->cell_sorted_d;
@@ -252,7 +272,18 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
             const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
             const int nrow = P->nsb[1] * P->nsb[2];
             const int per = (nrow + 7) / 8 * P->nsb[0];  // superbins per XCD (see k_interp3)
-            hipLaunchKernelGGL((k_interp3<KER, T>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems, nsb,
+            if constexpr (sizeof(T) == 4)
+            {
+                if (a.st)  // the spread set-up's stencils of these positions
+                {
+                    hipLaunchKernelGGL((k_interp3<KER, T, true>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a,
+                                       elems, nsb, P->nsb[0]);
+                    HIP_OK(hipGetLastError());
+                    if (t_end) HIP_OK(hipEventRecord(t_end, s));
+                    return IBAMR_OK;
+                }
+            }
+            hipLaunchKernelGGL((k_interp3<KER, T, false>), dim3(8 * per), dim3(I3_THREADS), bytes, s, a, elems, nsb,
                                P->nsb[0]);
         }
         else

@@ -54,13 +54,19 @@ __device__ __forceinline__ int interp_owner(const InterpArgs<T>& a, int k, const
     return -1;
 }
 
-template <int KER, typename T>
+template <int KER, typename T, bool LD>
 __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds_elems, int nsb, int nsx)
 {
     using KT = KernelTraits<KER>;
     constexpr int W = KT::W;
     constexpr bool FP64 = sizeof(T) == 8;
     constexpr int UNIT = sizeof(T) / 4;  // dwords per value
+    // FP32C IB kernels place stencils in the level's frame, as the spread set-up
+    // does, and read its starts and weights when it computed them for these
+    // positions (their weights, with square roots and divisions, cost more than
+    // the loads).  BSPLINE and FP64 evaluate them here in the patch frame.
+    constexpr bool PRE = !FP64 && KT::IB;
+    constexpr bool pre_ld = PRE && LD;
     extern __shared__ __align__(16) unsigned char s_raw[];
     T* s_u = reinterpret_cast<T*>(s_raw);
     unsigned int* s_dw = reinterpret_cast<unsigned int*>(s_raw);
@@ -116,11 +122,26 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
 #pragma unroll
             for (int d = 0; d < 3; ++d)
             {
+                if (!PRE)
                 {
                     T w[W];
                     const double X = a.X[3 * (size_t)k + d];
                     bc[d] = P.ilo[d] + weights_1d<KER>(X, P.xlo[d], a.dx[d], w);
                     bs[d] = a.side ? P.ilo[d] + weights_1d<KER>(X, P.xlo[d] - 0.5 * a.dx[d], a.dx[d], w) : bc[d];
+                }
+                else if (pre_ld)
+                {
+                    // the starts in the level's frame (k_point_weights)
+                    bc[d] = a.st[(size_t)(d * a.nvar) * a.n + k];
+                    bs[d] = a.side ? a.st[(size_t)(d * a.nvar + 1) * a.n + k] : bc[d];
+                }
+                else
+                {
+                    T w[W];
+                    const double X = a.X[3 * (size_t)k + d];
+                    bc[d] = a.lev_lo[d] + weights_1d<KER>(X, a.lev_xlo[d], a.dx[d], w);
+                    bs[d] = a.side ? a.lev_lo[d] + weights_1d<KER>(X, a.lev_xlo[d] - 0.5 * a.dx[d], a.dx[d], w)
+                                   : bc[d];
                 }
                 lmin[0][d] = min(lmin[0][d], bc[d]);
                 lmax[0][d] = max(lmax[0][d], bc[d]);
@@ -268,8 +289,24 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
                 {
-                    const double xl = Pq.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);
-                    bse[d] = Pq.ilo[d] + weights_1d<KER>(a.X[3 * (size_t)k + d], xl, a.dx[d], w[d]);
+                    if (!PRE)
+                    {
+                        const double xl = Pq.xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);
+                        bse[d] = Pq.ilo[d] + weights_1d<KER>(a.X[3 * (size_t)k + d], xl, a.dx[d], w[d]);
+                    }
+                    else if (pre_ld)
+                    {
+                        // precomputed in the level's frame, as the spread uses them
+                        const size_t row = (size_t)(d * a.nvar + (a.side && d == axis ? 1 : 0)) * a.n + k;
+                        bse[d] = a.st[row];
+#pragma unroll
+                        for (int j = 0; j < W; ++j) w[d][j] = (T)a.wt[row * W + j];
+                    }
+                    else
+                    {
+                        const double xl = a.lev_xlo[d] - (d == axis ? 0.5 * a.dx[d] : 0.0);
+                        bse[d] = a.lev_lo[d] + weights_1d<KER>(a.X[3 * (size_t)k + d], xl, a.dx[d], w[d]);
+                    }
                 }
                 Sum<T> acc;
                 if (staged)

@@ -155,7 +155,8 @@ int ibamr_plan_destroy(ibamr_plan* P)
         for (hipEvent_t e : v) (void)hipEventDestroy(e);
     dfree(P->patches_d);
     for (TileList& t : P->tlists) dfree(t.d);
-    dfree(P->fx_d);
+    dfree(P->wt_d);
+    dfree(P->st_d);
     dfree(P->bin_patch_start_d);
     dfree(P->fill_segs_d);
     dfree(P->bin_patches_d);
@@ -385,9 +386,11 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     PLAN_HIP(hipMalloc(&P->Fs_d, (C->precision == IBAMR_FP64 ? 8 : 4) * (size_t)P->fdepth * np));
     if (ndim == 3 && C->precision != IBAMR_FP64)
     {
-        // fixed-point bound of the FP32C spread: largest bin count, max |F| per component
-        PLAN_HIP(hipMalloc(&P->fx_d, sizeof(unsigned) * (1 + (size_t)P->fdepth)));
-        PLAN_HIP(hipMemset(P->fx_d, 0, sizeof(unsigned) * (1 + (size_t)P->fdepth)));
+        // FP32C: stencil starts and weights of every point, per axis and variant
+        P->nvar = C->centering == IBAMR_SIDE ? 2 : 1;
+        const size_t rows = (size_t)3 * P->nvar * np;
+        PLAN_HIP(hipMalloc(&P->wt_d, sizeof(float) * kernel_width(k) * rows));
+        PLAN_HIP(hipMalloc(&P->st_d, sizeof(int) * rows));
     }
     PLAN_HIP(hipMalloc(&P->bin_start_d, sizeof(int) * (P->nmem + 2)));
     {

@@ -142,11 +142,14 @@ struct ibamr_plan
     PatchDev* patches_d = nullptr;
     std::vector<TileList> tlists;      // spread work lists, one per admissible tile shape (first: default)
     int tcur = 0;                      // the shape in use
-    // FP32C 3-D spread (fixed point): fx_d[0] = largest number of points in one
-    // bin (set by the binning), fx_d[1 + c] = max |F_c| as float bits (set by
-    // each spread); fx_factor = bins whose points can reach one grid value
-    unsigned* fx_d = nullptr;
-    double fx_factor = 1.0;
+    // FP32C 3-D: the stencils of the bin-ordered positions, set up once per
+    // spread (k_point_weights): for axis d and variant v (v = 1: x_lower - dx/2,
+    // the normal axis of side data) the first stencil index
+    // (st_d[(d*nvar+v)*n + k]) and the W weights (wt_d[((d*nvar+v)*n + k)*W + j])
+    float* wt_d = nullptr;
+    int* st_d = nullptr;
+    int nvar = 1;                      // variants per axis: 2 for side data
+    bool ws_valid = false;             // wt_d/st_d hold the stencils of Xs
     int* bin_patch_start_d = nullptr;  // [nmem+2]
     int* bin_patches_d = nullptr;
     int4* fill_segs_d = nullptr;       // level ghost fill: (patch, array, first element, count) segments

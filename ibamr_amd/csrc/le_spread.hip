@@ -33,6 +33,87 @@ __global__ void k_scatter_points(const int* __restrict__ inv, int n, int ndim, c
     }
 }
 
+// FP32C 3-D spread set-up: for each point (bin order), axis d and variant v
+// (v = 0: the level's x_lower; v = 1: x_lower - dx/2, the normal axis of side
+// data, LEInteractor.cpp:3689) the first stencil index in the level's index
+// space and the W weights, in the level's frame.  The spread's tiles then stage
+// a candidate with loads instead of recomputing its weights for every tile it
+// reaches; a patch or periodic image shifts the start by whole cells.  With
+// perm, the kernel also performs the bin-order gather of X and F (side data:
+// records of 3 positions and 3 force components, F times ds[p] when ds is
+// given), so Xs is not read back.
+template <int KER>
+__global__ void k_point_weights(int n, double* __restrict__ Xs, int nvar, double3 xlo, double3 dx, int3 lo,
+                                float* __restrict__ wt, int* __restrict__ st, const int* __restrict__ perm,
+                                const double* __restrict__ Xin, const float* __restrict__ Fin,
+                                float* __restrict__ Fs, const float* __restrict__ ds)
+{
+    constexpr int W = KernelTraits<KER>::W;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double xl[3] = {xlo.x, xlo.y, xlo.z}, h[3] = {dx.x, dx.y, dx.z};
+    const int il[3] = {lo.x, lo.y, lo.z};
+    double x[3];
+    if (perm)
+    {
+        const size_t p = (size_t)perm[k];
+        float f[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            x[d] = Xin[3 * p + d];
+            f[d] = Fin[3 * p + d];
+        }
+        if (ds)  // F_ds[k][d] = F[k][d] * ds[k]  (LDataManager.cpp:439-445)
+        {
+            const float w = ds[p];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) f[d] = f[d] * w;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            Xs[3 * (size_t)k + d] = x[d];
+            Fs[3 * (size_t)k + d] = f[d];
+        }
+    }
+    else
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[d] = Xs[3 * (size_t)k + d];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const double X = x[d];
+        for (int v = 0; v < nvar; ++v)
+        {
+            float w[W];
+            const int s0 = weights_1d<KER>(X, v ? xl[d] - 0.5 * h[d] : xl[d], h[d], w);
+            const size_t row = (size_t)(d * nvar + v) * n + k;
+            st[row] = il[d] + s0;
+#pragma unroll
+            for (int j = 0; j < W; ++j) wt[row * W + j] = w[j];
+        }
+    }
+}
+
+// FP32C 3-D: stencil starts and weights of the bin-ordered positions, used by
+// the spread and the interpolation of the same positions
+// (with X and F: the gather into bin order of side data's X and F as well)
+template <int KER>
+int launch_point_weights(ibamr_plan* P, hipStream_t s, const double* X = nullptr, const float* F = nullptr,
+                         const float* ds = nullptr)
+{
+    if (!P->wt_d || !P->st_d) return fail(IBAMR_ERR_INVALID, "internal: no weight buffers");
+    const int B = 256;
+    hipLaunchKernelGGL((k_point_weights<KER>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->npoints, P->Xs_d,
+                       P->nvar, make_double3(P->dom_xlo[0], P->dom_xlo[1], P->dom_xlo[2]),
+                       make_double3(P->dx[0], P->dx[1], P->dx[2]),
+                       make_int3(P->dom_lo[0], P->dom_lo[1], P->dom_lo[2]), P->wt_d, P->st_d,
+                       X ? P->perm_d : nullptr, X, F, static_cast<float*>(P->Fs_d), ds);
+    HIP_OK(hipGetLastError());
+    return IBAMR_OK;
+}
+
 // ----------------------------------------------------------------------------- spread
 
 template <typename T>
@@ -59,10 +140,11 @@ struct SpreadArgs
     int reach;
     int side;
     int mode;
-    // FP32C 3-D (fixed point): the bound buffer (largest bin count, max |F| per
-    // component) and the bins whose points can reach one grid value
-    const unsigned* fx;
-    double fx_factor;
+    // FP32C 3-D: the points' stencil starts and weights in the level's frame
+    const float* wt;
+    const int* st;
+    int nvar;
+    int npts;
 };
 
 template <int NDIM, int TILE>
@@ -342,33 +424,6 @@ __global__ __launch_bounds__(WAVE) void k_spread_tiles(SpreadArgs<T> a)
 
 namespace
 {
-// fmax[c] = max over the points of |F_c| as float bits (non-negative floats
-// order like their bit patterns; a NaN sorts above every finite value), for
-// the fixed-point scale of the FP32C spread.  Order-independent: atomicMax of
-// integers.
-__global__ __launch_bounds__(256) void k_fmax(const float* __restrict__ F, int n, int fdepth,
-                                              unsigned* __restrict__ fmax)
-{
-    __shared__ unsigned s_m[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int c = 0; c < fdepth; ++c)
-    {
-        unsigned m = 0u;
-        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-            m = max(m, __float_as_uint(fabsf(F[(size_t)fdepth * k + c])));
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
-        if (lane == 0) s_m[wv] = m;
-        __syncthreads();
-        if (threadIdx.x == 0)
-        {
-            const unsigned b = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
-            if (b) atomicMax(fmax + c, b);
-        }
-        __syncthreads();
-    }
-}
-
 template <typename T>
 void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double* X, int mode, SpreadArgs<T>& a)
 {
@@ -397,73 +452,34 @@ void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double
     a.reach = P->reach;
     a.side = P->centering == IBAMR_SIDE;
     a.mode = mode;
-    a.fx = P->fx_d;
-    a.fx_factor = P->fx_factor;
+    a.wt = P->wt_d;
+    a.st = P->st_d;
+    a.nvar = P->nvar;
+    a.npts = P->npoints;
 }
 
 // Spread tile shapes built for each form; the first is the default.
 //   2-D: a 16 x 16 tile in LDS (k_spread_tiles).
-//   3-D FP64: 8 x 8 grid lines times a column of z values in registers (k_spread3).
-//   3-D FP32C: a box of 64-bit fixed-point values in LDS (k_spread3_fx); the
-//     kernels of the path (IB_4, IB_6, BSPLINE_3) have several shapes, the
-//     others the default only.
+//   3-D: 8 x 8 grid lines times a column of z values in registers (k_spread3);
+//     FP32C keeps two register sets per z value (the FP64 total and the
+//     chunk's FP32 partial), so its columns are shorter.
 using Shape = std::array<int, 3>;
-const std::vector<Shape>& tile_shapes(int ndim, int precision, int kernel)
+const std::vector<Shape>& tile_shapes(int ndim, int precision)
 {
     static const std::vector<Shape> s2 = {{16, 16, 1}};
     static const std::vector<Shape> s64 = {{8, 8, 32}, {8, 8, 16}};
-    static const std::vector<Shape> sfx = {{16, 16, 64}, {16, 16, 16}, {16, 16, 32}, {32, 8, 64}, {32, 16, 8}};
-    static const std::vector<Shape> sfx1 = {{16, 16, 64}, {16, 16, 16}};
+    static const std::vector<Shape> s32 = {{8, 8, 16}, {8, 8, 8}, {8, 8, 24}};
     if (ndim == 2) return s2;
-    if (precision == IBAMR_FP64) return s64;
-    return (kernel == K_IB4 || kernel == K_IB6 || kernel == K_BS3) ? sfx : sfx1;
-}
-
-// FP32C 3-D: z extents up to 16 are LDS boxes (k_spread3_fx); 32 and more are
-// column segments swept through an LDS window of FXZ_PLANES planes (k_spread3_fxz)
-constexpr int FXZ_PLANES = 16;
-bool is_column(const int* sh) { return sh[2] >= 32; }
-
-// The FP32C kernel of one tile shape (the shapes of tile_shapes)
-template <int KER>
-int launch_fx(const ibamr_plan* P, const SpreadArgs<float>& a, hipStream_t s)
-{
-    const int* sh = P->tlists[P->tcur].shape;
-    const dim3 G(P->tlists[P->tcur].n), B(FX_THREADS);
-#define IBAMR_FX_BOX(X, Y, Z)                                                            \
-    if (sh[0] == X && sh[1] == Y && sh[2] == Z)                                          \
-    {                                                                                    \
-        hipLaunchKernelGGL((k_spread3_fx<KER, X, Y, Z>), G, B, 0, s, a);                 \
-        return IBAMR_OK;                                                                 \
-    }
-#define IBAMR_FX_COLUMN(X, Y)                                                            \
-    if (sh[0] == X && sh[1] == Y && is_column(sh))                                       \
-    {                                                                                    \
-        hipLaunchKernelGGL((k_spread3_fxz<KER, X, Y, FXZ_PLANES>), G, B, 0, s, a, sh[2]); \
-        return IBAMR_OK;                                                                 \
-    }
-    IBAMR_FX_COLUMN(16, 16)
-    IBAMR_FX_BOX(16, 16, 16)
-    if constexpr (KER == K_IB4 || KER == K_IB6 || KER == K_BS3)
-    {
-        IBAMR_FX_COLUMN(32, 8)
-        IBAMR_FX_BOX(32, 16, 8)
-    }
-#undef IBAMR_FX_BOX
-#undef IBAMR_FX_COLUMN
-    return fail(IBAMR_ERR_INVALID, "internal: tile shape without a kernel");
+    return precision == IBAMR_FP64 ? s64 : s32;
 }
 
 // A shape is admissible when each axis of a tile reaches at most 16 (bin,
 // image) entries (the image ranges partition the reachable cells, plus one
-// split per domain side), and for a column segment when one bin layer's points
-// reach no more planes than the window holds.
+// split per domain side).
 bool shape_admissible(const ibamr_plan* P, const Shape& sh)
 {
     for (int d = 0; d < P->ndim; ++d)
         if ((sh[d] + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16) return false;
-    if (P->ndim == 3 && P->precision != IBAMR_FP64 && is_column(sh.data()) && P->bin + 2 * P->reach > FXZ_PLANES)
-        return false;
     return true;
 }
 
@@ -527,7 +543,7 @@ int find_shape(const ibamr_plan* P, const int* shape)
 
 int build_tile_lists(ibamr_plan* P)
 {
-    for (const Shape& sh : tile_shapes(P->ndim, P->precision, P->kernel))
+    for (const Shape& sh : tile_shapes(P->ndim, P->precision))
     {
         if (!shape_admissible(P, sh)) continue;
         P->tlists.emplace_back();
@@ -536,18 +552,6 @@ int build_tile_lists(ibamr_plan* P)
     }
     if (P->tlists.empty()) return fail(IBAMR_ERR_INVALID, "drift_cells too large for the tile size");
     P->tcur = 0;
-    // FP32C fixed-point bound: the cells whose points can reach one grid value
-    // span 2 reach + 1 cells per axis, so at most nblk bins, and with periodic
-    // images as many more copies as that window can wrap around the period
-    double f = 1.0;
-    for (int d = 0; d < P->ndim; ++d)
-    {
-        const int win = 2 * P->reach + 1;
-        const int nblk = (win - 1 + P->bin - 1) / P->bin + 1;
-        const int wraps = P->pshift[d] > 0 ? 1 + (win - 1 + P->pshift[d] - 1) / P->pshift[d] : 1;
-        f *= (double)nblk * wraps;
-    }
-    P->fx_factor = f;
     return IBAMR_OK;
 }
 } // namespace ibamr
@@ -574,38 +578,43 @@ int ibamr_plan_spread_ds(ibamr_plan* P, int slot, const void* F, const double* X
         using T = decltype(prec);
         constexpr int NDIM = decltype(nd)::value;
         constexpr int KER = decltype(ker)::value;
-        constexpr bool FX = NDIM == 3 && sizeof(T) == 4;
         SpreadArgs<T> a{};
         fill_spread_args<T>(P, slot, F, X, mode, a);
-        const int B = 256;
         if (P->npoints > 0)
         {
-            // the points in bin order
-            if (P->fdepth == NDIM)
-                hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->perm_d,
-                                   P->npoints, X, P->Xs_d, static_cast<const T*>(F), static_cast<T*>(P->Fs_d),
-                                   static_cast<const T*>(ds));
-            else
-                hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
-                                   P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
-                                   static_cast<T*>(P->Fs_d), static_cast<const T*>(ds));
-            HIP_OK(hipGetLastError());
-            P->xs_valid = true;
-        }
-        if constexpr (FX)
-        {
-            // fixed-point scale: max |F| per component (a fill kernel, not a
-            // memset: the launches may be captured in a HIP graph)
-            if (!P->fx_d) return fail(IBAMR_ERR_INVALID, "internal: no fixed-point bound buffer");
-            hipLaunchKernelGGL(k_fill_int, dim3((P->fdepth + B - 1) / B), dim3(B), 0, s,
-                               reinterpret_cast<int*>(P->fx_d + 1), P->fdepth, 0);
-            if (P->npoints > 0)
+            const int B = 256;
+            // FP32C 3-D: stencils set up here (not for kernels whose side depends on the unshifted X)
+            constexpr bool WTS = NDIM == 3 && sizeof(T) == 4 && !side_by_unshifted_x<KER>();
+            if constexpr (WTS)
             {
-                const int G = std::min((P->npoints + B - 1) / B, 1024);
-                hipLaunchKernelGGL(k_fmax, dim3(G), dim3(B), 0, s, static_cast<const float*>(P->Fs_d), P->npoints,
-                                   P->fdepth, P->fx_d + 1);
+                if (P->fdepth == NDIM)
+                {
+                    // gather (with F*ds) and stencils in one pass
+                    const int rc = launch_point_weights<KER>(P, s, X, reinterpret_cast<const float*>(F),
+                                                             reinterpret_cast<const float*>(ds));
+                    if (rc != IBAMR_OK) return rc;
+                }
             }
-            HIP_OK(hipGetLastError());
+            if (!WTS || P->fdepth != NDIM)
+            {
+                // the points in bin order
+                if (P->fdepth == NDIM)
+                    hipLaunchKernelGGL((k_gather_points<NDIM, T>), dim3((P->npoints + B - 1) / B), dim3(B), 0, s,
+                                       P->perm_d, P->npoints, X, P->Xs_d, static_cast<const T*>(F),
+                                       static_cast<T*>(P->Fs_d), static_cast<const T*>(ds));
+                else
+                    hipLaunchKernelGGL(k_scatter_points<T>, dim3((P->npoints + B - 1) / B), dim3(B), 0, s, P->idx_d,
+                                       P->npoints, NDIM, X, P->Xs_d, P->fdepth, static_cast<const T*>(F),
+                                       static_cast<T*>(P->Fs_d), static_cast<const T*>(ds));
+                HIP_OK(hipGetLastError());
+                if constexpr (WTS)
+                {
+                    const int rc = launch_point_weights<KER>(P, s);
+                    if (rc != IBAMR_OK) return rc;
+                }
+            }
+            P->xs_valid = true;
+            P->ws_valid = WTS;
         }
         a.X = P->Xs_d;
         a.F = static_cast<const T*>(P->Fs_d);
@@ -615,17 +624,16 @@ int ibamr_plan_spread_ds(ibamr_plan* P, int slot, const void* F, const double* X
         const int trc = time_begin(P, 0, s, &t_end);
         if (trc != IBAMR_OK) return trc;
         const TileList& tl = P->tlists[P->tcur];
-        if constexpr (FX)
+        if constexpr (NDIM == 3)
         {
-            const int rc = launch_fx<KER>(P, a, s);
-            if (rc != IBAMR_OK) return rc;
-        }
-        else if constexpr (NDIM == 3)
-        {
+            // FP32C packs chunks across bins; FP64 cuts them per bin (the oracle's fixed order)
+            constexpr bool MERGE = sizeof(T) == 4;
             switch (tl.shape[2])
             {
-            case 16: hipLaunchKernelGGL((k_spread3<KER, 16>), dim3(tl.n), dim3(64), 0, s, a); break;
-            case 32: hipLaunchKernelGGL((k_spread3<KER, 32>), dim3(tl.n), dim3(64), 0, s, a); break;
+            case 8: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 8>), dim3(tl.n), dim3(64), 0, s, a); break;
+            case 16: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 16>), dim3(tl.n), dim3(64), 0, s, a); break;
+            case 24: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 24>), dim3(tl.n), dim3(64), 0, s, a); break;
+            case 32: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 32>), dim3(tl.n), dim3(64), 0, s, a); break;
             default: return fail(IBAMR_ERR_INVALID, "internal: tile height without a kernel");
             }
         }

@@ -17,8 +17,9 @@
  *   - Lagrangian data: LData local form, AoS: X[n][ndim] (always double),
  *     F/U[n][depth] (side data: depth = ndim).
  *   - FP64 plans use double grids and F/U; FP32C plans use float grids and
- *     F/U (spread sums in 64-bit fixed point, compensated interpolation
- *     sums); positions and stencil selection stay FP64 in both.
+ *     F/U (spread sums: FP32 partial sums of at most 64 points folded into
+ *     FP64 totals; compensated interpolation sums); positions and stencil
+ *     selection stay FP64 in both.
  */
 #ifndef IBAMR_AMD_H
 #define IBAMR_AMD_H
@@ -48,8 +49,9 @@ enum
 enum
 {
     IBAMR_FP64 = 0,  /* plain FP64, results identical to the reference arithmetic */
-    IBAMR_FP32C = 1  /* FP32 grid/force data, FP32 weights; spread sums in 64-bit fixed point,
-                        interpolation sums compensated; positions and stencil selection FP64 */
+    IBAMR_FP32C = 1  /* FP32 grid/force data, FP32 weights; spread partial sums in FP32 folded into
+                        FP64 totals, interpolation sums compensated; positions and stencil
+                        selection FP64 */
 };
 enum
 {
@@ -159,11 +161,9 @@ int ibamr_plan_get_cell_indices(const ibamr_plan* plan, int* cells, void* stream
  * periodic images, into the arrays bound to `slot`.  Deterministic, with no
  * floating-point atomics, so results are bitwise reproducible: FP64 sums every
  * grid value's contributions in one fixed order (bin, image, point); FP32C
- * rounds every contribution to a 64-bit fixed-point integer (one global scale
- * per force component, set from the largest bin count and max |F|) and adds
- * the integers, whose sum does not depend on the order.  An FP32C spread of a
- * force holding a non-finite value writes NaN into every value the points can
- * reach.  F: [n][ndim] (side) or [n][depth] (cell); X: current positions.
+ * sums the contributions of each chunk of at most 64 points in FP32 and folds
+ * the chunk sums into FP64 totals, in an order fixed by the bins, the points'
+ * order and the tile shape (the same inputs and shape give the same bits).  F: [n][ndim] (side) or [n][depth] (cell); X: current positions.
  * Modes: IBAMR_SPREAD_ACCUMULATE adds each contribution to the existing value
  * (the raw LEInteractor::spread); IBAMR_SPREAD_ACCUMULATE_INTERIOR sums the
  * contributions from zero and then adds the existing value (LDataManager::
@@ -360,11 +360,11 @@ int ibamr_plan_info(const ibamr_plan* plan, int* ntiles, int* tile, int* nbins);
  * stays valid).  Shapes (x, y, z extents; the first is the default):
  *   2-D: 16 x 16.
  *   3-D FP64: 8 x 8 grid lines times 32 or 16 z values held in registers.
- *   3-D FP32C: an LDS box of 64-bit fixed-point values, 16x16x16, 32x16x8,
- *     16x16x8 and 16x8x16 for IB_4, IB_6 and BSPLINE_3 (16x16x16 for the other
- *     kernels).
- * Results do not depend on the shape: FP64 sums each value in an order fixed
- * by the bins, FP32C sums fixed-point integers with one global scale.  No
+ *   3-D FP32C: 8 x 8 grid lines times 16, 8 or 24 z values held in registers.
+ * FP64 results do not depend on the shape (each value is summed in an order
+ * fixed by the bins); FP32C results are reproducible for a given shape and
+ * differ between shapes within the FP32C tolerance (a shape groups the points
+ * into other 64-point chunks).  No
  * counterpart in the reference, whose CPU loops have no tiles: these tune this
  * implementation's work decomposition to the caller's points and hardware.
  *   set_tile_shape / get_tile_shape: shape[3].
