@@ -48,7 +48,7 @@ C_API = [
     "ibamr_plan_set_physical_boundary", "ibamr_plan_accumulate_physical_boundary", "ibamr_plan_fill_physical_boundary", "ibamr_fe_quadrature_points",
     "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
-    "ibamr_s0_interp", "ibamr_s0_spread",
+    "ibamr_s0_interp", "ibamr_s0_spread", "ibamr_s0_interp_noalloc",
 ] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear",
                                                         "bspline_4", "bspline_5", "bspline_6")
      for op in ("interp", "spread") for d in ("2d", "3d")]

@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -59,15 +61,26 @@ __device__ __forceinline__ void s0_stencil(const S0Geom& g, const double* Xs, co
 }
 
 // last[s] = largest list position with index s
-__global__ void k_s0_last(const int* __restrict__ indices, int n, int* __restrict__ last)
+// (entries whose index lies outside [0, npts) take no part)
+__global__ void k_s0_last(const int* __restrict__ indices, int n, int npts, int* __restrict__ last)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < n) atomicMax(&last[indices[l]], l);
+    if (l >= n) return;
+    const int s = indices[l];
+    if (s >= 0 && s < npts) atomicMax(&last[s], l);
+}
+
+// last[0..n) = -1 (a kernel, not a memset: the allocation-free form may be
+// captured in a HIP graph)
+__global__ void k_s0_clear(int* __restrict__ last, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) last[i] = -1;
 }
 
 template <int NDIM, int KER>
 __global__ void k_s0_interp(S0Geom g, const double* __restrict__ u, const int* __restrict__ indices,
-                            const double* __restrict__ Xshift, int n, const double* __restrict__ X,
+                            const double* __restrict__ Xshift, int n, int npts, const double* __restrict__ X,
                             double* __restrict__ V, const int* __restrict__ last)
 {
     using KT = KernelTraits<KER>;
@@ -75,7 +88,8 @@ __global__ void k_s0_interp(S0Geom g, const double* __restrict__ u, const int* _
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
     const int s = indices[l];
-    if (last[s] != l) return;  // an later entry with the same index overwrites V(:, s)
+    if (s < 0 || s >= npts) return;
+    if (last[s] != l) return;  // a later entry with the same index overwrites V(:, s)
     double Xs[3];
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) Xs[d] = X[NDIM * (size_t)s + d] + Xshift[NDIM * (size_t)l + d];
@@ -277,9 +291,73 @@ int by_kernel(int ndim, int k, F&& f)
     return ndim == 3 ? go(std::integral_constant<int, 3>{}) : go(std::integral_constant<int, 2>{});
 }
 
+// The allocation-free interpolation's scratch: the last list entry of every
+// point, one file-scope array per GPU (held until the process ends).  Host
+// code uses the per-device address hipGetSymbolAddress returns, never the
+// array's name.
+__device__ __align__(256) int scratch[IBAMR_S0_NOALLOC_MAX_POINTS];
+
+std::mutex g_scratch_mu;
+std::map<int, int*> g_scratch_ptr;  // device -> address of its copy of scratch
+
+// The current device's copy of scratch: looked up once per device, with that
+// device current.
+int scratch_ptr(int** out)
+{
+    int dev = 0;
+    S0_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end())
+    {
+        int* ptr = nullptr;
+        S0_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&ptr), HIP_SYMBOL(scratch)));
+        if (!ptr) return set_error(IBAMR_ERR_DEVICE, "hipGetSymbolAddress returned a null scratch address");
+        it = g_scratch_ptr.emplace(dev, ptr).first;
+    }
+    *out = it->second;
+    return IBAMR_OK;
+}
 } // namespace
 
 extern "C" {
+
+int ibamr_s0_interp_noalloc(const char* kernel, int ndim, const double* dx, const double* x_lower, int depth,
+                            const int* ilower, const int* iupper, const int* nugc, const double* u,
+                            const int* indices, const double* Xshift, int nindices, const double* X, int npoints,
+                            double* V, void* stream)
+{
+    const int k = kid(kernel);
+    if (k < 0) return set_error(IBAMR_ERR_BAD_KERNEL, std::string("unknown kernel function ") + (kernel ? kernel : ""));
+    S0Geom g;
+    size_t ucount = 0;
+    int rc = make_geom(ndim, dx, x_lower, depth, ilower, iupper, nugc, g, ucount);
+    if (rc != IBAMR_OK) return rc;
+    if (nindices < 0 || npoints < 0) return set_error(IBAMR_ERR_INVALID, "negative count");
+    if (npoints > IBAMR_S0_NOALLOC_MAX_POINTS)
+        return set_error(IBAMR_ERR_CAPACITY, "npoints " + std::to_string(npoints) +
+                                                 " exceeds the allocation-free scratch of " +
+                                                 std::to_string(IBAMR_S0_NOALLOC_MAX_POINTS) + " points");
+    if (nindices == 0 || npoints == 0) return IBAMR_OK;
+    if (!u || !indices || !Xshift || !X || !V) return set_error(IBAMR_ERR_INVALID, "null device pointer");
+    int* last = nullptr;
+    rc = scratch_ptr(&last);
+    if (rc != IBAMR_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int B = 256, G = (nindices + B - 1) / B;
+    // the array keeps the previous call's contents: clear the part this call uses
+    hipLaunchKernelGGL(k_s0_clear, dim3((npoints + B - 1) / B), dim3(B), 0, s, last, npoints);
+    hipLaunchKernelGGL(k_s0_last, dim3(G), dim3(B), 0, s, indices, nindices, npoints, last);
+    S0_HIP(hipGetLastError());
+    rc = by_kernel(ndim, k, [&](auto nd, auto ker) -> int {
+        hipLaunchKernelGGL((k_s0_interp<decltype(nd)::value, decltype(ker)::value>), dim3(G), dim3(B), 0, s, g, u,
+                           indices, Xshift, nindices, npoints, X, V, last);
+        return IBAMR_OK;
+    });
+    if (rc != IBAMR_OK) return rc;
+    S0_HIP(hipGetLastError());
+    return IBAMR_OK;
+}
 
 int ibamr_s0_interp(const char* kernel, int ndim, const double* dx, const double* x_lower, int depth,
                     const int* ilower, const int* iupper, const int* nugc, const double* u, const int* indices,
@@ -313,12 +391,13 @@ int ibamr_s0_interp(const char* kernel, int ndim, const double* dx, const double
     S0_HIP(hipMemcpy(dV.p, V, ns * depth * sizeof(double), hipMemcpyHostToDevice));
     S0_HIP(hipMemset(dlast.p, 0xff, ns * sizeof(int)));
     const int B = 256, G = (nindices + B - 1) / B;
-    hipLaunchKernelGGL(k_s0_last, dim3(G), dim3(B), 0, 0, static_cast<int*>(di.p), nindices, static_cast<int*>(dlast.p));
+    hipLaunchKernelGGL(k_s0_last, dim3(G), dim3(B), 0, 0, static_cast<int*>(di.p), nindices, (int)ns,
+                       static_cast<int*>(dlast.p));
     S0_HIP(hipGetLastError());
     rc = by_kernel(ndim, k, [&](auto nd, auto ker) -> int {
         hipLaunchKernelGGL((k_s0_interp<decltype(nd)::value, decltype(ker)::value>), dim3(G), dim3(B), 0, 0, g,
                            static_cast<double*>(du.p), static_cast<int*>(di.p), static_cast<double*>(dsh.p),
-                           nindices, static_cast<double*>(dX.p), static_cast<double*>(dV.p),
+                           nindices, (int)ns, static_cast<double*>(dX.p), static_cast<double*>(dV.p),
                            static_cast<int*>(dlast.p));
         return IBAMR_OK;
     });

@@ -106,6 +106,25 @@ int ibamr_s0_spread(const char* kernel, int ndim, const double* dx, const double
                     const int* indices, const double* Xshift, int nindices, const double* X, const double* V,
                     const int* ilower, const int* iupper, const int* nugc, double* u);
 
+/* Allocation-free, device-pointer form of ibamr_s0_interp.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  dx, x_lower, ilower, iupper and nugc are host arrays as above;
+ * u, indices, Xshift, X and V are device pointers; X and V hold npoints points
+ * (entries whose index lies outside [0, npoints) are skipped).  The kernels'
+ * scratch -- the last list entry of every point, which decides which entry
+ * sets V(:, s) when an index repeats -- is a file-scope device array of
+ * IBAMR_S0_NOALLOC_MAX_POINTS ints that each GPU the library runs on holds
+ * until the process ends; a larger npoints fails with IBAMR_ERR_CAPACITY.
+ * Enqueues kernels on `stream` only: no allocation, copy or synchronisation.
+ * Every call on one GPU uses that GPU's array, so calls that overlap on one
+ * GPU (other streams, other threads) race. */
+#define IBAMR_S0_NOALLOC_MAX_POINTS (1 << 22)
+int ibamr_s0_interp_noalloc(const char* kernel, int ndim, const double* dx, const double* x_lower, int depth,
+                            const int* ilower, const int* iupper, const int* nugc, const double* u,
+                            const int* indices, const double* Xshift, int nindices, const double* X, int npoints,
+                            double* V, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
