@@ -20,7 +20,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libibamr_amd.so")
+# IBAMR_AMD_LIB: another build of the library (a different path), e.g. to compare builds
+LIB_PATH = os.environ.get("IBAMR_AMD_LIB") or os.path.join(_HERE, "lib", "libibamr_amd.so")
 
 IBAMR_OK = 0
 IBAMR_ERR_BAD_KERNEL = -1
