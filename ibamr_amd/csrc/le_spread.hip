@@ -460,15 +460,15 @@ void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double
 
 // Spread tile shapes built for each form; the first is the default.
 //   2-D: a 16 x 16 tile in LDS (k_spread_tiles).
-//   3-D: 8 x 8 grid lines times a column of z values in registers (k_spread3);
-//     FP32C keeps two register sets per z value (the FP64 total and the
-//     chunk's FP32 partial), so its columns are shorter.
+//   3-D: 8 x 8 or 16 x 4 grid lines (one per lane) times a column of z values
+//     in registers (k_spread3); FP32C keeps two register sets per z value (the
+//     FP64 total and the chunk's FP32 partial), so its columns are shorter.
 using Shape = std::array<int, 3>;
 const std::vector<Shape>& tile_shapes(int ndim, int precision)
 {
     static const std::vector<Shape> s2 = {{16, 16, 1}};
-    static const std::vector<Shape> s64 = {{8, 8, 32}, {8, 8, 16}};
-    static const std::vector<Shape> s32 = {{8, 8, 16}, {8, 8, 8}, {8, 8, 24}};
+    static const std::vector<Shape> s64 = {{8, 8, 32}, {8, 8, 16}, {16, 4, 32}};
+    static const std::vector<Shape> s32 = {{8, 8, 16}, {8, 8, 8}, {8, 8, 24}, {16, 4, 16}};
     if (ndim == 2) return s2;
     return precision == IBAMR_FP64 ? s64 : s32;
 }
@@ -628,14 +628,30 @@ int ibamr_plan_spread_ds(ibamr_plan* P, int slot, const void* F, const double* X
         {
             // FP32C packs chunks across bins; FP64 cuts them per bin (the oracle's fixed order)
             constexpr bool MERGE = sizeof(T) == 4;
-            switch (tl.shape[2])
+            const dim3 G(tl.n), B(64);
+#define IBAMR_S3(TX, ZC)                                                              \
+    if (tl.shape[0] == TX && tl.shape[2] == ZC)                                       \
+    {                                                                                 \
+        hipLaunchKernelGGL((k_spread3<KER, T, MERGE, ZC, TX>), G, B, 0, s, a);        \
+        HIP_OK(hipGetLastError());                                                    \
+        if (t_end) HIP_OK(hipEventRecord(t_end, s));                                  \
+        return IBAMR_OK;                                                              \
+    }
+            if constexpr (sizeof(T) == 8)
             {
-            case 8: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 8>), dim3(tl.n), dim3(64), 0, s, a); break;
-            case 16: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 16>), dim3(tl.n), dim3(64), 0, s, a); break;
-            case 24: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 24>), dim3(tl.n), dim3(64), 0, s, a); break;
-            case 32: hipLaunchKernelGGL((k_spread3<KER, T, MERGE, 32>), dim3(tl.n), dim3(64), 0, s, a); break;
-            default: return fail(IBAMR_ERR_INVALID, "internal: tile height without a kernel");
+                IBAMR_S3(8, 32)
+                IBAMR_S3(8, 16)
+                IBAMR_S3(16, 32)
             }
+            else
+            {
+                IBAMR_S3(8, 16)
+                IBAMR_S3(8, 8)
+                IBAMR_S3(8, 24)
+                IBAMR_S3(16, 16)
+            }
+#undef IBAMR_S3
+            return fail(IBAMR_ERR_INVALID, "internal: tile shape without a kernel");
         }
         else
             hipLaunchKernelGGL((k_spread_tiles<2, KER, T, 16>), dim3(tl.n), dim3(WAVE), 0, s, a);

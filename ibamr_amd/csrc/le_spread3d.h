@@ -2,8 +2,8 @@
 // which defines SpreadArgs, PatchDev and TileDesc).
 //
 // Spread (k_spread3): owner-computes, register-resident.  One wavefront owns a
-// column tile of one array: 8 x 8 (x, y) grid lines, one per lane, times
-// ZC consecutive z values held in registers.  Points that can reach the
+// column tile of one array: TX x TY (x, y) grid lines (8 x 8 or 16 x 4), one
+// per lane, times ZC consecutive z values held in registers.  Points that can reach the
 // tile are staged 64 at a time: each lane takes one point's stencil (FP64
 // computes the reference's weights; FP32C loads the starts and weights the
 // set-up pass computed once per point) and writes the x and y weights, aligned
@@ -25,10 +25,16 @@
 
 namespace
 {
-constexpr int S3_PAD = 9;    // staged weight row: the 8 tile lanes (odd stride: no bank conflicts)
+// staged weight rows: the tile's TX (x) or TY (y) lanes, padded to an odd
+// stride (no bank conflicts between the rows of different points)
+template <int N>
+constexpr int s3_pad()
+{
+    return N + 1;
+}
 constexpr int S3_MAXCOMB = 64;
 
-template <int KER, typename T, bool MERGE, int ZC>
+template <int KER, typename T, bool MERGE, int ZC, int TX>
 __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
 {
     using KT = KernelTraits<KER>;
@@ -37,8 +43,11 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     // FP32C stages the set-up pass's stencils, except for kernels whose
     // stencil side depends on the unshifted X (an image's differs)
     constexpr bool PRECOMP = !FP64 && !side_by_unshifted_x<KER>();
-    __shared__ T s_rx[64 * S3_PAD];
-    __shared__ T s_ry[64 * S3_PAD];
+    constexpr int TY = 64 / TX;  // the 64 lanes: TX x TY grid lines
+    constexpr int PX = s3_pad<TX>(), PY = s3_pad<TY>();
+    static_assert(TX * TY == 64, "one lane per grid line");
+    __shared__ T s_rx[64 * PX];
+    __shared__ T s_ry[64 * PY];
     __shared__ T s_wz[64 * 8];
     __shared__ T s_F[64];
     __shared__ int s_dx0[64];
@@ -52,12 +61,12 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
     __shared__ int s_cp[S3_MAXCOMB + 1];// prefix of lengths
 
     const int lane = threadIdx.x;
-    const int x = lane & 7, y = lane >> 3;
+    const int x = lane % TX, y = lane / TX;
     const TileDesc td = a.tiles[blockIdx.x];
     if (td.patch < 0) return;  // padding of the XCD-ordered work list
     const PatchDev P = a.patches[td.patch];
     const int axis = a.side ? td.comp : -1;
-    const int tsz[3] = {8, 8, ZC};
+    const int tsz[3] = {TX, TY, ZC};
     int alo[3], ahi[3], an[3], flo[3], fhi[3];
     double xl[3];
 #pragma unroll
@@ -328,23 +337,21 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                         }
                     }
                     const int dx0 = bse[0] - flo[0], dy0 = bse[1] - flo[1], zz = bse[2] - flo[2];
-                    if (dx0 > -W && dx0 < 8 && dy0 > -W && dy0 < 8 && zz > -W && zz < ZC)
+                    if (dx0 > -W && dx0 < TX && dy0 > -W && dy0 < TY && zz > -W && zz < ZC)
                     {
                         // rows hold the weight of each of the tile's 8 x (y) lanes, zero outside the stencil
-                        T* rx = s_rx + lane * S3_PAD;
-                        T* ry = s_ry + lane * S3_PAD;
+                        T* rx = s_rx + lane * PX;
+                        T* ry = s_ry + lane * PY;
 #pragma unroll
-                        for (int i = 0; i < 8; ++i)
-                        {
-                            rx[i] = T(0);
-                            ry[i] = T(0);
-                        }
+                        for (int i = 0; i < TX; ++i) rx[i] = T(0);
+#pragma unroll
+                        for (int i = 0; i < TY; ++i) ry[i] = T(0);
                         const T fx = FP64 ? T(1) : (KT::IB ? Fv : Fv * inv_vol);
 #pragma unroll
                         for (int j = 0; j < W; ++j)
                         {
-                            if ((unsigned)(dx0 + j) < 8u) rx[dx0 + j] = FP64 ? w[0][j] : w[0][j] * fx;
-                            if ((unsigned)(dy0 + j) < 8u) ry[dy0 + j] = w[1][j];
+                            if ((unsigned)(dx0 + j) < (unsigned)TX) rx[dx0 + j] = FP64 ? w[0][j] : w[0][j] * fx;
+                            if ((unsigned)(dy0 + j) < (unsigned)TY) ry[dy0 + j] = w[1][j];
                             // FP64 IB folds 1/vol into w2 exactly as the reference (f.m4:1435)
                             s_wz[lane * 8 + j] = (FP64 && KT::IB) ? w[2][j] / vol
                                                                   : ((!FP64 && KT::IB) ? w[2][j] * inv_vol : w[2][j]);
@@ -384,10 +391,10 @@ __global__ __launch_bounds__(64) void k_spread3(SpreadArgs<T> a)
                         const int j2 = m ? __builtin_ctzll(m) : j;
                         const bool two = m != 0;
                         if (two) m &= m - 1;
-                        const T wx = s_rx[j * S3_PAD + x];
-                        const T wy = s_ry[j * S3_PAD + y];
-                        const T wx2 = s_rx[j2 * S3_PAD + x];
-                        const T wy2 = s_ry[j2 * S3_PAD + y];
+                        const T wx = s_rx[j * PX + x];
+                        const T wy = s_ry[j * PY + y];
+                        const T wx2 = s_rx[j2 * PX + x];
+                        const T wy2 = s_ry[j2 * PY + y];
                         T wz[W], wz2[W];
 #pragma unroll
                         for (int i = 0; i < W; ++i)

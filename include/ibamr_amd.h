@@ -359,8 +359,9 @@ int ibamr_plan_info(const ibamr_plan* plan, int* ntiles, int* tile, int* nbins);
  * (a HIP graph captured earlier keeps launching the shape it captured, which
  * stays valid).  Shapes (x, y, z extents; the first is the default):
  *   2-D: 16 x 16.
- *   3-D FP64: 8 x 8 grid lines times 32 or 16 z values held in registers.
- *   3-D FP32C: 8 x 8 grid lines times 16, 8 or 24 z values held in registers.
+ *   3-D FP64: 8 x 8 grid lines times 32 or 16 z values held in registers, or
+ *     16 x 4 grid lines times 32.
+ *   3-D FP32C: 8 x 8 grid lines times 16, 8 or 24 z values, or 16 x 4 times 16.
  * FP64 results do not depend on the shape (each value is summed in an order
  * fixed by the bins); FP32C results are reproducible for a given shape and
  * differ between shapes within the FP32C tolerance (a shape groups the points

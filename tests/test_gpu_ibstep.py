@@ -111,10 +111,10 @@ def test_autotune_then_graph_matches_untuned_fp64():
     torch.cuda.synchronize()
     assert sorted(tunes) == ["BSPLINE_3", "IB_4"]
     for (k, _, _), p in zip(S1.groups, S1.plans):
-        assert sorted(tunes[k]) == [(8, 8, 16), (8, 8, 32)]
+        assert sorted(tunes[k]) == [(8, 8, 16), (8, 8, 32), (16, 4, 32)]
         assert p.tile_shape == min(tunes[k], key=tunes[k].get)
     p0 = S1.plans[0]
-    p0.tile_shape = (8, 8, 16) if p0.tile_shape == (8, 8, 32) else (8, 8, 32)
+    p0.tile_shape = (16, 4, 32) if p0.tile_shape != (16, 4, 32) else (8, 8, 32)
     g = S1.capture(X1, F, U1, dt)
     torch.cuda.synchronize()
     X1.copy_(X0)

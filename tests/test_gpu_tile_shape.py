@@ -45,19 +45,28 @@ def _spread(plan, X, F, shapes, dt):
     return [a.cpu().numpy() for a in f]
 
 
+FP64_SHAPES = [(8, 8, 32), (8, 8, 16), (16, 4, 32)]
+
+
 @pytest.mark.parametrize("kernel,centering", [("IB_4", ia.SIDE), ("IB_6", ia.SIDE), ("BSPLINE_3", ia.CELL)])
-def test_fp64_bitwise_with_every_shape(kernel, centering):
+def test_fp64_bitwise_with_every_shape(orc, kernel, centering):
+    """Every FP64 tile shape (footprint and height) against the oracle, bit for bit."""
+    from test_gpu_parity import _oracle_spread
     plan, X, F, shapes, dt = _setup(ia.FP64, kernel, centering)
-    assert plan.tile_shape == (8, 8, 32)
-    ref = _spread(plan, X, F, shapes, dt)
-    plan.tile_shape = (8, 8, 16)
-    assert plan.tile_height == 16
-    got = _spread(plan, X, F, shapes, dt)
-    for a, b in zip(ref, got):
-        assert np.array_equal(a, b)
+    assert plan.tile_shape == FP64_SHAPES[0]
+    level = ia.Level.uniform_periodic(40, 3)
+    Xh, Fh = X.cpu().numpy(), F.cpu().numpy()
+    want = _oracle_spread(orc, level, plan, [np.zeros(s) for s in shapes], Xh, Fh, kernel)
+    for sh in FP64_SHAPES:
+        plan.tile_shape = sh
+        got = _spread(plan, X, F, shapes, dt)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b), sh
+    plan.tile_height = 16
+    assert plan.tile_shape == (8, 8, 16)
 
 
-FP32C_SHAPES = [(8, 8, 16), (8, 8, 8), (8, 8, 24)]
+FP32C_SHAPES = [(8, 8, 16), (8, 8, 8), (8, 8, 24), (16, 4, 16)]
 
 
 def _rel(a, b):
