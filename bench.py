@@ -70,11 +70,11 @@ def copy_probe(torch):
     print(json.dumps({"copy_probe_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
-def counters_from_profile(kernel, tag):
+def counters_from_profile(kernel, workload_tag):
     """VALU-busy and LDS bank-conflict fractions of a kernel from the committed
     rocprofv3 counter summary of this workload ("Derived" table: the spread
     kernel's column, then the interpolation's), or None."""
-    path = os.path.join(ROOT, "profiles", "counters_%s.md" % tag)
+    path = os.path.join(ROOT, "profiles", "counters_%s.md" % workload_tag)
     if not os.path.exists(path):
         return None
     col = 0 if kernel.startswith("k_spread") else 1
@@ -693,7 +693,7 @@ def main():
                      "traffic": tr["bytes_per_dispatch"] if tr else None, "algorithmic_bytes": B_dom,
                      "traffic_source": ("profiles/hbm_traffic_%s.txt" % tag) if tr else None,
                      # what bounds it instead: instruction issue (rocprofv3 counters of this workload)
-                     "counters": counters_from_profile(kname, tag)},
+                     "counters": counters_from_profile(kname, "%dc_side_%s_%dpts_fp32c" % (N, kernel.lower(), n))},
         "detail": {"spread_ms": r32["spread_ms"], "interp_ms": r32["interp_ms"], "bin_ms": r32["bin_ms"],
                    "spread_kernel_ms": r32["spread_kernel_ms"], "interp_kernel_ms": r32["interp_kernel_ms"],
                    "touched_faces": T, "bytes_step": B_spread + B_interp,
