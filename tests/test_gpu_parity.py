@@ -369,3 +369,39 @@ def test_shifted_index_space(precision):
     for a, b in zip(out[0][0], out[1][0]):
         assert np.array_equal(a, b)
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("centering", [ia.SIDE, ia.CELL])
+def test_interior_mode_adds_old_value_once(orc, centering):
+    """IBAMR_SPREAD_ACCUMULATE_INTERIOR is LDataManager::spread's zero, spread,
+    add (LDataManager.cpp:594, 661-663): every interior value becomes
+    old + (sum of its contributions from zero), bit for bit, and ghost values
+    keep their old values."""
+    ndim, n, kernel = 3, 32, "IB_4"
+    h = 1.0 / n
+    boxes = [([4, 4, 4], [11, 19, 11]), ([12, 4, 4], [27, 11, 13]), ([16, 20, 18], [23, 27, 29])]
+    patches = [ia.Patch(lo, hi, [l * h for l in lo], [(u + 1) * h for u in hi]) for lo, hi in boxes]
+    level = ia.Level([h] * 3, [0] * 3, [n - 1] * 3, [0.0] * 3, [1.0] * 3, [0, 0, 0], patches)
+    rng = np.random.default_rng(23)
+    X = np.concatenate([np.stack([rng.uniform(lo[d] * h, (hi[d] + 1) * h, 120) for d in range(3)], axis=1)
+                        for lo, hi in boxes])
+    depth = 1 if centering == ia.SIDE else 2
+    plan = ia.LevelPlan(level, kernel, centering, ia.FP64, depth=depth, max_points=X.shape[0])
+    F = rng.standard_normal((X.shape[0], 3 if centering == ia.SIDE else depth))
+    Xd, Fd = _dev(X), _dev(F)
+    plan.bin(Xd)
+    base = _arrays(level, plan, rng)
+    dv = [_dev(a) for a in base]
+    plan.bind(0, dv)
+    plan.spread(0, Fd, Xd, mode=ia.ACCUMULATE_INTERIOR)
+    zero = _oracle_spread(orc, level, plan, [np.zeros_like(a) for a in base], X, F, kernel)
+    torch.cuda.synchronize()
+    g = plan.data_gcw
+    for i, (got, old, s) in enumerate(zip(dv, base, zero)):
+        got = got.cpu().numpy()
+        inner = np.zeros(got.shape, dtype=bool)
+        # array index order (depth,) z, y, x; the interior (side box along the array's axis)
+        sl = [slice(g[2 - ax], got.shape[got.ndim - 3 + ax] - g[2 - ax]) for ax in range(3)]
+        inner[(Ellipsis,) + tuple(sl)] = True
+        assert np.array_equal(got[inner], (old + s)[inner]), i
+        assert np.array_equal(got[~inner], old[~inner]), i
