@@ -42,7 +42,7 @@ C_API = [
     "ibamr_is_known_kernel", "ibamr_get_stencil_size", "ibamr_get_minimum_ghost_width",
     "ibamr_plan_create", "ibamr_plan_create_patch", "ibamr_plan_destroy", "ibamr_plan_bind_data", "ibamr_plan_bin",
     "ibamr_plan_bin_async", "ibamr_plan_check", "ibamr_plan_spread", "ibamr_plan_spread_ds", "ibamr_plan_interp",
-    "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
+    "ibamr_plan_fill_periodic_ghosts", "ibamr_plan_fill_ghosts", "ibamr_forward_euler_step", "ibamr_forward_euler_step_periodic", "ibamr_plan_set_timing", "ibamr_plan_kernel_times",
     "ibamr_plan_info", "ibamr_plan_set_tile_height", "ibamr_plan_get_tile_height", "ibamr_plan_autotune",
     "ibamr_plan_set_tile_shape", "ibamr_plan_get_tile_shape", "ibamr_plan_get_cell_indices",
     "ibamr_plan_redistribute", "ibamr_permute_rows",
@@ -113,6 +113,9 @@ def lib():
     L.ibamr_forces_compute.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     L.ibamr_forces_destroy.argtypes = [vp]
     L.ibamr_forward_euler_step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_long, vp]
+    L.ibamr_forward_euler_step_periodic.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_long,
+                                                    ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), vp]
     L.ibamr_plan_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     L.ibamr_plan_bind_data.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
@@ -575,6 +578,20 @@ def forward_euler_step(X_new, X, U, dt, stream=None):
     assert X.dtype == torch.float64 and X_new.dtype == torch.float64 and X.numel() == U.numel() == X_new.numel()
     _check(lib().ibamr_forward_euler_step(_ptr(X_new), _ptr(X), _ptr(U), prec, float(dt), X.numel(),
                                           _stream_ptr(stream)))
+
+
+def forward_euler_step_periodic(X_new, X, U, dt, level, stream=None):
+    """X_new = X + dt U, then positions moved back into the level's domain by one
+    period along its periodic axes (device tensors [n][ndim])."""
+    import torch
+    prec = FP64 if U.dtype == torch.float64 else FP32C
+    nd = level.ndim
+    assert X.dtype == torch.float64 and X_new.dtype == torch.float64 and X.numel() == U.numel() == X_new.numel()
+    lo = (ctypes.c_double * nd)(*level.domain_x_lower)
+    hi = (ctypes.c_double * nd)(*level.domain_x_upper)
+    per = (ctypes.c_int * nd)(*[1 if s else 0 for s in level.periodic_shift])
+    _check(lib().ibamr_forward_euler_step_periodic(_ptr(X_new), _ptr(X), _ptr(U), prec, float(dt), X.numel() // nd, nd,
+                                                   lo, hi, per, _stream_ptr(stream)))
 
 
 # --------------------------------------------------------------------------- S0: Fortran-kernel drop-in

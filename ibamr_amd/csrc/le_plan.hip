@@ -101,6 +101,30 @@ __global__ void k_euler(double* __restrict__ Xn, const double* __restrict__ X, c
     if (i < n) Xn[i] = X[i] + dt * (double)U[i];
 }
 
+// X + dt U, then a position outside a periodic axis's [lo, hi) moved back by
+// one period L = hi - lo (x < lo: x + L; then x >= hi: x - L), as
+// LDataManager::beginDataRedistribution wraps positions (LDataManager.cpp:1403-1407;
+// one period suffices for a step that moves a point by less than one).
+template <typename TU>
+__global__ void k_euler_wrap(double* __restrict__ Xn, const double* __restrict__ X, const TU* __restrict__ U,
+                             double dt, long n, int ndim, double3 lo, double3 hi, double3 L, int3 per)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = (int)(i % ndim);
+    double x = X[i] + dt * (double)U[i];
+    const int p = d == 0 ? per.x : (d == 1 ? per.y : per.z);
+    if (p)
+    {
+        const double l = d == 0 ? lo.x : (d == 1 ? lo.y : lo.z);
+        const double h = d == 0 ? hi.x : (d == 1 ? hi.y : hi.z);
+        const double len = d == 0 ? L.x : (d == 1 ? L.y : L.z);
+        if (x < l) x = x + len;
+        if (x >= h) x = x - len;
+    }
+    Xn[i] = x;
+}
+
 // Records the start event of launch `which` when timing is on; *end receives
 // the event to record after the launch (nullptr when timing is off).
 int time_begin(ibamr_plan* P, int which, hipStream_t s, hipEvent_t* end)
@@ -529,6 +553,43 @@ int ibamr_forward_euler_step(double* X_new, const double* X, const void* U, int 
         hipLaunchKernelGGL(k_euler<double>, g, dim3(B), 0, s, X_new, X, static_cast<const double*>(U), dt, n);
     else
         hipLaunchKernelGGL(k_euler<float>, g, dim3(B), 0, s, X_new, X, static_cast<const float*>(U), dt, n);
+    HIP_OK(hipGetLastError());
+    return IBAMR_OK;
+}
+
+int ibamr_forward_euler_step_periodic(double* X_new, const double* X, const void* U, int u_precision, double dt,
+                                      long npoints, int ndim, const double* x_lower, const double* x_upper,
+                                      const int* periodic, void* stream)
+{
+    if (npoints < 0) return fail(IBAMR_ERR_INVALID, "npoints < 0");
+    if (ndim != 2 && ndim != 3) return fail(IBAMR_ERR_INVALID, "ndim must be 2 or 3");
+    if (!x_lower || !x_upper || !periodic) return fail(IBAMR_ERR_INVALID, "null domain description");
+    if (npoints == 0) return IBAMR_OK;
+    if (!X_new || !X || !U) return fail(IBAMR_ERR_INVALID, "null array");
+    if (u_precision != IBAMR_FP64 && u_precision != IBAMR_FP32C) return fail(IBAMR_ERR_INVALID, "bad precision");
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, len[3] = {0, 0, 0};
+    int per[3] = {0, 0, 0};
+    for (int d = 0; d < ndim; ++d)
+    {
+        lo[d] = x_lower[d];
+        hi[d] = x_upper[d];
+        len[d] = x_upper[d] - x_lower[d];
+        per[d] = periodic[d] != 0;
+        if (per[d] && !(len[d] > 0.0)) return fail(IBAMR_ERR_INVALID, "periodic axis with x_upper <= x_lower");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long n = npoints * ndim;
+    const int B = 256;
+    const dim3 g((unsigned)((n + B - 1) / B));
+    const double3 l3 = make_double3(lo[0], lo[1], lo[2]), h3 = make_double3(hi[0], hi[1], hi[2]),
+                  L3 = make_double3(len[0], len[1], len[2]);
+    const int3 p3 = make_int3(per[0], per[1], per[2]);
+    if (u_precision == IBAMR_FP64)
+        hipLaunchKernelGGL(k_euler_wrap<double>, g, dim3(B), 0, s, X_new, X, static_cast<const double*>(U), dt, n,
+                           ndim, l3, h3, L3, p3);
+    else
+        hipLaunchKernelGGL(k_euler_wrap<float>, g, dim3(B), 0, s, X_new, X, static_cast<const float*>(U), dt, n, ndim,
+                           l3, h3, L3, p3);
     HIP_OK(hipGetLastError());
     return IBAMR_OK;
 }

@@ -25,8 +25,8 @@ by the periodic ghost fill, so the interpolated velocity depends on the spread.
 
 Graph capture (torch.cuda.CUDAGraph) records only kernels: the force
 generation, point binning (ibamr_plan_bin_async), spread, the stand-in
-combination (a torch elementwise kernel), the periodic ghost fill, interp, the
-Euler update and the periodic wrap (torch elementwise kernels).  No memset and
+combination (a torch elementwise kernel), the periodic ghost fill, interp, and
+the Euler update with the periodic wrap (one library kernel).  No memset and
 no host synchronization happens inside a step.
 """
 import torch
@@ -87,21 +87,9 @@ class StructureSet:
         self.plans[0].fill_periodic_ghosts(1, stream=s)
         for p, a, b in self._slices():
             p.interp(1, U[a:b], X[a:b], stream=s)
-        ia.forward_euler_step(X, X, U, dt, stream=s)
-        self._wrap(X, s)
-
-    def _wrap(self, X, s):
-        """Moves positions back into the periodic box (one period suffices: a
-        step moves a point by less than one period)."""
-        L = self.level
-        with torch.cuda.stream(s):
-            for d in range(L.ndim):
-                if not L.periodic_shift[d]:
-                    continue
-                lo, hi = L.domain_x_lower[d], L.domain_x_upper[d]
-                x = X[:, d]
-                x.copy_(torch.where(x < lo, x + (hi - lo), x))
-                x.copy_(torch.where(x >= hi, x - (hi - lo), x))
+        # X + dt U and the move back into the periodic box in one kernel (one
+        # period suffices: a step moves a point by less than one period)
+        ia.forward_euler_step_periodic(X, X, U, dt, self.level, stream=s)
 
     def autotune(self, X, F, reps=2, stream=None):
         """Picks each plan's spread tile shape for its structures' current

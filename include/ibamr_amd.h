@@ -313,6 +313,17 @@ int ibamr_permute_rows(void* dst, const void* src, const int* order, int n, int 
 int ibamr_forward_euler_step(double* X_new, const double* X, const void* U, int u_precision, double dt, long n,
                              void* stream);
 
+/* The same update over npoints points of ndim coordinates, each new position
+ * then moved back into the domain along the axes with periodic[d] != 0 by one
+ * period (x < x_lower: x + L; then x >= x_upper: x - L; L = x_upper - x_lower),
+ * as LDataManager::beginDataRedistribution wraps positions at a regrid
+ * (LDataManager.cpp:1403-1407), so points stay in the binned domain between
+ * regrids.  x_lower, x_upper, periodic: host arrays of ndim.  Kernel only:
+ * graph-safe. */
+int ibamr_forward_euler_step_periodic(double* X_new, const double* X, const void* U, int u_precision, double dt,
+                                      long npoints, int ndim, const double* x_lower, const double* x_upper,
+                                      const int* periodic, void* stream);
+
 /* Lagrangian force generation, IBStandardForceGen::computeLagrangianForce
  * (src/IB/IBStandardForceGen.cpp:254-305): Hookean springs (default_spring_force,
  * IBSpringForceFunctions.h:104: tension kappa (R - rest)), beams (three-point
