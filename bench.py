@@ -122,8 +122,9 @@ def make_points(torch, n, N, seed):
 
 
 def spread_kernel_name(shape):
-    """The 3-D spread kernel (every tile shape runs the register-column kernel)."""
-    return "k_spread3"
+    """The 3-D spread kernel of a tile shape: z extents of 48 or more are swept
+    z segments (FP32C), shorter ones fixed register columns."""
+    return "k_spread3s" if shape[2] >= 48 else "k_spread3"
 
 
 def touched_faces(torch, X, N, kernel, periodic=True):

@@ -216,8 +216,10 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
             e[d] = s_hi[v][d] - s_lo[v][d] + W;
         }
         const int total = e[0] * e[1] * e[2];
-        const bool staged = (flag & (1 | (2 << ax))) == 0 && total <= lds_elems;
         const size_t plane = (size_t)an[0] * an[1];
+        // staged: one patch, no clipping, the block fits the LDS, and planes below
+        // 2^24 values (the block's offsets then fit 24-bit products and 32 bits)
+        const bool staged = (flag & (1 | (2 << ax))) == 0 && total <= lds_elems && plane < ((size_t)1 << 24);
         const int ncomps = a.side ? 1 : a.depth;
         for (int cd = 0; cd < ncomps; ++cd)
         {
@@ -227,43 +229,29 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                 // Stage the block straight from global memory into LDS (global_load_lds:
                 // no registers in between, every load in flight at once).  Dword q of
                 // the block goes to s_dw[q]; each wave covers 64 consecutive dwords per
-                // instruction, so the LDS destination is wave-uniform.
+                // instruction, so the LDS destination is wave-uniform.  Value v of the
+                // block, (v % e0, (v / e0) % e1, v / (e0 e1)), lies at
+                // v + (an0 - e0) (v / e0) + (plane - e1 an0) (v / (e0 e1)) from the
+                // block's first value.  The quotients are floor((v + 1/2) * fl(1/d)) in
+                // FP32: (v + 1/2) / d is at least 1/(2d) from an integer and the
+                // computed product is within (v / d) 2^-23 of it, so the floor is exact
+                // for v < 2^22 (the block holds at most lds_elems < 2^14 values).
                 const int ndw = total * UNIT;
-                const int q0 = wv * 64 + lane;
-                int el = q0 / UNIT;
-                const int half = q0 % UNIT;
-                constexpr int STEP = I3_THREADS / UNIT;  // values per round per thread
-                int i2 = el / (e[0] * e[1]);
-                int r = el - i2 * e[0] * e[1];
-                int i1 = r / e[0];
-                int i0 = r - i1 * e[0];
-                const int st1 = STEP / e[0], st0 = STEP - st1 * e[0];
-                const T* ub = u + (size_t)(blo[0] - alo[0]) + (size_t)an[0] * (blo[1] - alo[1]) +
-                              plane * (size_t)(blo[2] - alo[2]);
-                // offset of this thread's element in the array, stepped incrementally:
-                // +d0 per round, +c0 when i0 wraps (one more row), +c1 per i1 wrap (one more plane)
-                long off = (long)i0 + (long)an[0] * i1 + (long)plane * i2;
-                const long d0 = (long)st0 + (long)an[0] * st1;
-                const long c0 = (long)an[0] - e[0];
-                const long c1 = (long)plane - (long)e[1] * an[0];
-                for (int base = wv * 64; base < ndw; base += I3_THREADS)
+                const float r0 = 1.0f / (float)e[0], r1 = 1.0f / (float)(e[0] * e[1]);
+                const unsigned c0 = (unsigned)(an[0] - e[0]);
+                const unsigned c1 = (unsigned)(plane - (size_t)e[1] * an[0]);
+                const unsigned int* ub = reinterpret_cast<const unsigned int*>(
+                    u + (size_t)(blo[0] - alo[0]) + (size_t)an[0] * (blo[1] - alo[1]) + plane * (size_t)(blo[2] - alo[2]));
+                const int w0 = __builtin_amdgcn_readfirstlane(wv) * 64;
+                for (int base = w0; base < ndw; base += I3_THREADS)
                 {
-                    if (base + lane < ndw)
-                        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned int*>(ub + off) + half,
-                                                         s_dw + base, 4, 0, 0);
-                    i0 += st0;
-                    i1 += st1;
-                    off += d0;
-                    if (i0 >= e[0])
+                    const unsigned q = (unsigned)(base + lane);
+                    if (q < (unsigned)ndw)
                     {
-                        i0 -= e[0];
-                        ++i1;
-                        off += c0;
-                    }
-                    while (i1 >= e[1])
-                    {
-                        i1 -= e[1];
-                        off += c1;
+                        const unsigned v = q / UNIT;
+                        const float fv = (float)v + 0.5f;
+                        const unsigned off = v + __umul24(c0, (unsigned)(fv * r0)) + __umul24(c1, (unsigned)(fv * r1));
+                        __builtin_amdgcn_global_load_lds(ub + (off * UNIT + q % UNIT), s_dw + base, 4, 0, 0);
                     }
                 }
                 __builtin_amdgcn_s_waitcnt(0);

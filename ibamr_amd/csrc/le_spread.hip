@@ -463,24 +463,40 @@ void fill_spread_args(const ibamr_plan* P, int slot, const void* F, const double
 //   3-D: 8 x 8 or 16 x 4 grid lines (one per lane) times a column of z values
 //     in registers (k_spread3); FP32C keeps two register sets per z value (the
 //     FP64 total and the chunk's FP32 partial), so its columns are shorter.
+//     FP32C z extents of SWEEP_MIN_Z or more are z segments swept through a
+//     register window (k_spread3s).
 using Shape = std::array<int, 3>;
+constexpr int SWEEP_MIN_Z = 48;
 const std::vector<Shape>& tile_shapes(int ndim, int precision)
 {
     static const std::vector<Shape> s2 = {{16, 16, 1}};
     static const std::vector<Shape> s64 = {{8, 8, 32}, {8, 8, 16}, {16, 4, 32}};
-    static const std::vector<Shape> s32 = {{8, 8, 16}, {8, 8, 8}, {8, 8, 24}, {16, 4, 16}};
+    static const std::vector<Shape> s32 = {{8, 8, 16}, {8, 8, 8}, {8, 8, 24}, {16, 4, 16},
+                                           {8, 8, 64}, {8, 8, 128}, {8, 8, 256}};
     if (ndim == 2) return s2;
     return precision == IBAMR_FP64 ? s64 : s32;
 }
 
 // A shape is admissible when each axis of a tile reaches at most 16 (bin,
 // image) entries (the image ranges partition the reachable cells, plus one
-// split per domain side).
+// split per domain side).  A z sweep takes up to S3S_MAXZ z entries and at
+// most 64 (x, y) combinations per z layer, needs every stencil of a bin layer
+// inside its register window (bin + 2 reach <= S3S_ZW) and the set-up pass's
+// stencils (not computed for kernels whose stencil side depends on the
+// unshifted X).
 bool shape_admissible(const ibamr_plan* P, const Shape& sh)
 {
+    const bool sweep = P->ndim == 3 && sh[2] >= SWEEP_MIN_Z;
+    if (sweep && (P->precision == IBAMR_FP64 || P->kernel == K_BS4 || P->kernel == K_BS6 ||
+                  P->bin + 2 * P->reach > S3S_ZW))
+        return false;
+    int n[3] = {1, 1, 1};
     for (int d = 0; d < P->ndim; ++d)
-        if ((sh[d] + 2 * P->reach + P->bin - 1) / P->bin + 3 > 16) return false;
-    return true;
+    {
+        n[d] = (sh[d] + 2 * P->reach + P->bin - 1) / P->bin + 3;
+        if (n[d] > (sweep && d == 2 ? S3S_MAXZ : 16)) return false;
+    }
+    return !sweep || n[0] * n[1] <= 64;
 }
 
 // Work list of one tile shape: every tile of every array of every patch.
@@ -649,6 +665,16 @@ int ibamr_plan_spread_ds(ibamr_plan* P, int slot, const void* F, const double* X
                 IBAMR_S3(8, 8)
                 IBAMR_S3(8, 24)
                 IBAMR_S3(16, 16)
+                if constexpr (!side_by_unshifted_x<KER>())
+                    if (tl.shape[0] == 8 && tl.shape[2] >= SWEEP_MIN_Z)
+                    {
+                        if (P->npoints > 0 && !P->ws_valid)
+                            return fail(IBAMR_ERR_INVALID, "internal: z sweep without stencils");
+                        hipLaunchKernelGGL((k_spread3s<KER, 8>), G, B, 0, s, a, tl.shape[2]);
+                        HIP_OK(hipGetLastError());
+                        if (t_end) HIP_OK(hipEventRecord(t_end, s));
+                        return IBAMR_OK;
+                    }
             }
 #undef IBAMR_S3
             return fail(IBAMR_ERR_INVALID, "internal: tile shape without a kernel");

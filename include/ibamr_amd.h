@@ -372,7 +372,11 @@ int ibamr_plan_info(const ibamr_plan* plan, int* ntiles, int* tile, int* nbins);
  *   2-D: 16 x 16.
  *   3-D FP64: 8 x 8 grid lines times 32 or 16 z values held in registers, or
  *     16 x 4 grid lines times 32.
- *   3-D FP32C: 8 x 8 grid lines times 16, 8 or 24 z values, or 16 x 4 times 16.
+ *   3-D FP32C: 8 x 8 grid lines times 16, 8 or 24 z values, or 16 x 4 times 16;
+ *     and 8 x 8 grid lines times a z segment of 64, 128 or 256 values swept
+ *     through a 16-value register window, layer of bins by layer (each point
+ *     applied once per 8 x 8 column; built when the kernel has precomputed
+ *     stencils, bin + 2 reach <= 16, i.e. drift_cells 0 for IB_6).
  * FP64 results do not depend on the shape (each value is summed in an order
  * fixed by the bins); FP32C results are reproducible for a given shape and
  * differ between shapes within the FP32C tolerance (a shape groups the points

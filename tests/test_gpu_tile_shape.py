@@ -66,7 +66,7 @@ def test_fp64_bitwise_with_every_shape(orc, kernel, centering):
     assert plan.tile_shape == (8, 8, 16)
 
 
-FP32C_SHAPES = [(8, 8, 16), (8, 8, 8), (8, 8, 24), (16, 4, 16)]
+FP32C_SHAPES = [(8, 8, 16), (8, 8, 8), (8, 8, 24), (16, 4, 16), (8, 8, 64), (8, 8, 128), (8, 8, 256)]
 
 
 def _rel(a, b):
