@@ -29,6 +29,9 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Lagrangian points spread+interpolated/s (FP32-batched vs FP64) @<=1e-6 err; HBM GB/s"
 
 
+TILE = None  # --tile
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +55,9 @@ def parse():
     ap.add_argument("--workload", choices=["shell512", "amr", "multi"], default="shell512",
                     help="shell512: periodic 512^3 IB_6 (default); amr: multi-patch fine level, BSPLINE_3; "
                          "multi: 64 shells on 384^3 (IB_4 and BSPLINE_3), 100 graph-captured steps")
+    ap.add_argument("--tile", default=None,
+                    help="X,Y,Z: FP32C spread tile shape instead of the autotuner's choice (the autotuner still "
+                         "runs and reports its times)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 2 GiB (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -228,6 +234,8 @@ def run_form(torch, ia, precision, X, F, N, kernel, steps, warmup):
     bin_ms = eb0.elapsed_time(eb1)
     # spread tile height chosen by the library's autotuner on these points (untimed)
     tune = plan.autotune(0, Fd, X, reps=3, stream=s)
+    if TILE and precision != ia.FP64:
+        plan.tile_shape = TILE
 
     def step(ev=None):
         if ev:
@@ -386,6 +394,8 @@ def run_amr_form(torch, ia, precision, level, X, F, kernel, steps, warmup, seed=
     U = torch.zeros((n, 3), dtype=dt, device="cuda")
     plan.bin(X, stream=s)
     tune = plan.autotune(0, Fd, X, reps=3, stream=s)  # spread tile height for these patches and points
+    if TILE and precision != ia.FP64:
+        plan.tile_shape = TILE
     for _ in range(warmup):
         plan.spread(0, Fd, X, mode=ia.OVERWRITE, stream=s)
         plan.interp(1, U, X, stream=s)
@@ -629,7 +639,10 @@ def summary(rec):
 
 
 def main():
+    global TILE
     args = parse()
+    if args.tile:
+        TILE = tuple(int(v) for v in args.tile.split(","))
     if args.gpus != 1:
         print("bench.py measures one GPU only (the library has no multi-GPU path); got --gpus %d" % args.gpus,
               file=sys.stderr)

@@ -38,6 +38,12 @@ struct InterpArgs
     int nvar;
     double lev_xlo[3];
     int lev_lo[3];
+    // 3-D: superbins with a static owner and block (see build_superbin_owners)
+    const int* sb_owner;
+    int sb_off[2][2];
+    int sbc;         // cells per superbin edge
+    int nsy;         // superbins along y
+    int dom_hi[3];
 };
 
 // Compensated (FP32) or plain (FP64, same order as the reference) sum.
@@ -197,6 +203,81 @@ int interp_prepare(int ndim, int kernel, int precision)
     });
 }
 
+// Lowest and highest stencil start minus the point's cell over positions in the
+// cell, for the cell-centred [0] and the side-centred [1] placement along an
+// axis (weights_1d: IB_4/IB_6 nint((X - x_lower)/dx) - 2/3, BSPLINE floor based,
+// PIECEWISE_LINEAR by the nearest cell centre).  Rounding of X / dx at a cell
+// boundary can move a start by one more; the kernel checks each point against
+// its staged block and reads the others from the array.
+void stencil_start_offsets(int k, int off[2][2])
+{
+    static const int tab[7][2][2] = {
+        {{-1, 0}, {0, 0}},     // PIECEWISE_LINEAR
+        {{-2, -1}, {-1, -1}},  // IB_4
+        {{-3, -2}, {-2, -2}},  // IB_6
+        {{-1, -1}, {-1, 0}},   // BSPLINE_3
+        {{-2, -1}, {-1, -1}},  // BSPLINE_4
+        {{-2, -2}, {-2, -1}},  // BSPLINE_5
+        {{-3, -2}, {-2, -2}},  // BSPLINE_6
+    };
+    for (int v = 0; v < 2; ++v)
+        for (int i = 0; i < 2; ++i) off[v][i] = tab[k][v][i];
+}
+
+// Static superbins: the 3-D interpolation stages a superbin's block of each
+// array from the superbin's cells when one patch owns all of them and every
+// block lies in that patch's ghosted arrays and fits the LDS block (edge 16 + W);
+// the kernel then skips the pass over the points that finds the owner and the
+// exact extents.
+int build_superbin_owners(ibamr_plan* P)
+{
+    if (P->ndim != 3) return IBAMR_OK;
+    int off[2][2];
+    stencil_start_offsets(P->kernel, off);
+    for (int v = 0; v < 2; ++v)
+    {
+        P->sb_off[v][0] = off[v][0] - P->drift;
+        P->sb_off[v][1] = off[v][1] + P->drift;
+    }
+    const int W = kernel_width(P->kernel), edge = 2 * P->bin + W, sbc = 2 * P->bin;
+    const int nsb = P->nsb[0] * P->nsb[1] * P->nsb[2];
+    std::vector<int> owner(nsb, -1);
+    const bool side = P->centering == IBAMR_SIDE;
+    for (int i = 0; i < P->npatches; ++i)
+    {
+        const PatchDev& g = P->patches_h[i];
+        int s_lo[3], s_hi[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            s_lo[d] = std::max(0, (g.slo[d] - P->dom_lo[d]) / sbc);
+            s_hi[d] = std::min(P->nsb[d] - 1, (g.shi[d] - P->dom_lo[d]) / sbc);
+        }
+        for (int sz = s_lo[2]; sz <= s_hi[2]; ++sz)
+            for (int sy = s_lo[1]; sy <= s_hi[1]; ++sy)
+                for (int sx = s_lo[0]; sx <= s_hi[0]; ++sx)
+                {
+                    const int sc[3] = {sx, sy, sz};
+                    bool ok = true;
+                    for (int d = 0; d < 3 && ok; ++d)
+                    {
+                        const int clo = P->dom_lo[d] + sbc * sc[d], chi = std::min(clo + sbc - 1, P->dom_hi[d]);
+                        ok = clo >= g.slo[d] && chi <= g.shi[d];
+                        for (int ax = 0; ax < (side ? 3 : 1) && ok; ++ax)
+                        {
+                            const int v = side && d == ax ? 1 : 0;
+                            const int blo = clo + P->sb_off[v][0], bhi = chi + P->sb_off[v][1] + W - 1;
+                            ok = blo >= g.ilo[d] - P->data_gcw[d] && bhi <= g.ihi[d] + P->data_gcw[d] + v &&
+                                 bhi - blo + 1 <= edge;
+                        }
+                    }
+                    if (ok) owner[(sz * P->nsb[1] + sy) * P->nsb[0] + sx] = i;
+                }
+    }
+    HIP_OK(hipMalloc(&P->sb_owner_d, sizeof(int) * std::max(1, nsb)));
+    HIP_OK(hipMemcpy(P->sb_owner_d, owner.data(), sizeof(int) * nsb, hipMemcpyHostToDevice));
+    return IBAMR_OK;
+}
+
 } // namespace ibamr
 
 extern "C" {
@@ -253,6 +334,12 @@ int ibamr_plan_interp(ibamr_plan* P, int slot, void* U, const double* X, void* s
             a.dx[d] = P->dx[d];
             a.gcw[d] = P->data_gcw[d];
         }
+        a.sb_owner = P->sb_owner_d;
+        for (int v = 0; v < 2; ++v)
+            for (int i = 0; i < 2; ++i) a.sb_off[v][i] = P->sb_off[v][i];
+        a.sbc = 2 * P->bin;
+        a.nsy = P->nsb[1];
+        for (int d = 0; d < 3; ++d) a.dom_hi[d] = P->dom_hi[d];
         a.narr = P->narr;
         a.fdepth = P->fdepth;
         a.depth = P->depth;

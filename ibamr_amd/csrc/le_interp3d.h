@@ -85,114 +85,136 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
 
-    if (tid < 6)
+    const int owner = a.sb_owner ? a.sb_owner[sb] : -1;
+    if (owner >= 0)
     {
-        s_lo[tid / 3][tid % 3] = 1 << 30;
-        s_hi[tid / 3][tid % 3] = -(1 << 30);
-    }
-    if (tid == 0)
-    {
-        s_pid = -1;
-        s_flag = 0;
-    }
-    __syncthreads();
-    // pass 1 (all arrays at once): owning patch and stencil bases of every
-    // point -> block extents; flags: bit 0 several patches, bit 1+a array a
-    // needs clipping (thread-local min/max, then one LDS atomic per wavefront)
-    {
-        int lmin[2][3], lmax[2][3];
-#pragma unroll
-        for (int v = 0; v < 2; ++v)
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-            {
-                lmin[v][d] = 1 << 30;
-                lmax[v][d] = -(1 << 30);
-            }
-        int lpid = -1, lflag = 0;
-        for (int k = s0 + tid; k < s1; k += I3_THREADS)
+        // a static superbin (one owning patch, blocks inside its ghosted arrays):
+        // the blocks from the superbin's cells and the kernel's start offsets
+        if (tid < 6)
         {
-            const int4 c4 = a.cells[k];
-            const int pid = interp_owner(a, k, c4);
-            if (pid < 0) continue;
-            if (lpid >= 0 && lpid != pid) lflag |= 1;
-            lpid = pid;
-            const PatchDev P = a.patches[pid];
-            int bc[3], bs[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-            {
-                if (!PRE)
-                {
-                    T w[W];
-                    const double X = a.X[3 * (size_t)k + d];
-                    bc[d] = P.ilo[d] + weights_1d<KER>(X, P.xlo[d], a.dx[d], w);
-                    bs[d] = a.side ? P.ilo[d] + weights_1d<KER>(X, P.xlo[d] - 0.5 * a.dx[d], a.dx[d], w) : bc[d];
-                }
-                else if (pre_ld)
-                {
-                    // the starts in the level's frame (k_point_weights)
-                    bc[d] = a.st[(size_t)(d * a.nvar) * a.n + k];
-                    bs[d] = a.side ? a.st[(size_t)(d * a.nvar + 1) * a.n + k] : bc[d];
-                }
-                else
-                {
-                    T w[W];
-                    const double X = a.X[3 * (size_t)k + d];
-                    bc[d] = a.lev_lo[d] + weights_1d<KER>(X, a.lev_xlo[d], a.dx[d], w);
-                    bs[d] = a.side ? a.lev_lo[d] + weights_1d<KER>(X, a.lev_xlo[d] - 0.5 * a.dx[d], a.dx[d], w)
-                                   : bc[d];
-                }
-                lmin[0][d] = min(lmin[0][d], bc[d]);
-                lmax[0][d] = max(lmax[0][d], bc[d]);
-                lmin[1][d] = min(lmin[1][d], bs[d]);
-                lmax[1][d] = max(lmax[1][d], bs[d]);
-            }
-            const int narr = a.side ? 3 : 1;
-            for (int ax = 0; ax < narr; ++ax)
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                {
-                    const bool sh = a.side && d == ax;
-                    const int b = sh ? bs[d] : bc[d];
-                    const int alo = P.ilo[d] - a.gcw[d];
-                    const int ahi = P.ihi[d] + a.gcw[d] + (sh ? 1 : 0);
-                    if (b < alo || b + W - 1 > ahi) lflag |= 2 << ax;
-                }
+            const int v = tid / 3, d = tid % 3;
+            const int sc = d == 0 ? sb % nsx : (d == 1 ? (sb / nsx) % a.nsy : sb / nsx / a.nsy);
+            const int clo = a.lev_lo[d] + a.sbc * sc, chi = min(clo + a.sbc - 1, a.dom_hi[d]);
+            s_lo[v][d] = clo + a.sb_off[v][0];
+            s_hi[v][d] = chi + a.sb_off[v][1];
         }
-        int pmin = lpid < 0 ? (1 << 30) : lpid, pmax = lpid;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
+        if (tid == 0)
         {
+            s_pid = owner;
+            s_flag = 0;
+        }
+    }
+    else
+    {
+        if (tid < 6)
+        {
+            s_lo[tid / 3][tid % 3] = 1 << 30;
+            s_hi[tid / 3][tid % 3] = -(1 << 30);
+        }
+        if (tid == 0)
+        {
+            s_pid = -1;
+            s_flag = 0;
+        }
+        __syncthreads();
+        // pass 1 (all arrays at once): owning patch and stencil bases of every
+        // point -> block extents; flags: bit 0 several patches, bit 1+a array a
+        // needs clipping (thread-local min/max, then one LDS atomic per wavefront)
+        {
+            int lmin[2][3], lmax[2][3];
 #pragma unroll
             for (int v = 0; v < 2; ++v)
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
                 {
-                    lmin[v][d] = min(lmin[v][d], __shfl_xor(lmin[v][d], off));
-                    lmax[v][d] = max(lmax[v][d], __shfl_xor(lmax[v][d], off));
+                    lmin[v][d] = 1 << 30;
+                    lmax[v][d] = -(1 << 30);
                 }
-            pmin = min(pmin, __shfl_xor(pmin, off));
-            pmax = max(pmax, __shfl_xor(pmax, off));
-            lflag |= __shfl_xor(lflag, off);
-        }
-        if (lane == 0)
-        {
-#pragma unroll
-            for (int v = 0; v < 2; ++v)
+            int lpid = -1, lflag = 0;
+            for (int k = s0 + tid; k < s1; k += I3_THREADS)
+            {
+                const int4 c4 = a.cells[k];
+                const int pid = interp_owner(a, k, c4);
+                if (pid < 0) continue;
+                if (lpid >= 0 && lpid != pid) lflag |= 1;
+                lpid = pid;
+                const PatchDev P = a.patches[pid];
+                int bc[3], bs[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
                 {
-                    atomicMin(&s_lo[v][d], lmin[v][d]);
-                    atomicMax(&s_hi[v][d], lmax[v][d]);
+                    if (!PRE)
+                    {
+                        T w[W];
+                        const double X = a.X[3 * (size_t)k + d];
+                        bc[d] = P.ilo[d] + weights_1d<KER>(X, P.xlo[d], a.dx[d], w);
+                        bs[d] = a.side ? P.ilo[d] + weights_1d<KER>(X, P.xlo[d] - 0.5 * a.dx[d], a.dx[d], w) : bc[d];
+                    }
+                    else if (pre_ld)
+                    {
+                        // the starts in the level's frame (k_point_weights)
+                        bc[d] = a.st[(size_t)(d * a.nvar) * a.n + k];
+                        bs[d] = a.side ? a.st[(size_t)(d * a.nvar + 1) * a.n + k] : bc[d];
+                    }
+                    else
+                    {
+                        T w[W];
+                        const double X = a.X[3 * (size_t)k + d];
+                        bc[d] = a.lev_lo[d] + weights_1d<KER>(X, a.lev_xlo[d], a.dx[d], w);
+                        bs[d] = a.side ? a.lev_lo[d] + weights_1d<KER>(X, a.lev_xlo[d] - 0.5 * a.dx[d], a.dx[d], w)
+                                       : bc[d];
+                    }
+                    lmin[0][d] = min(lmin[0][d], bc[d]);
+                    lmax[0][d] = max(lmax[0][d], bc[d]);
+                    lmin[1][d] = min(lmin[1][d], bs[d]);
+                    lmax[1][d] = max(lmax[1][d], bs[d]);
                 }
-            if (pmax >= 0)
-            {
-                if (pmin != pmax) lflag |= 1;
-                const int prev = atomicCAS(&s_pid, -1, pmin);
-                if (prev != -1 && prev != pmin) lflag |= 1;
+                const int narr = a.side ? 3 : 1;
+                for (int ax = 0; ax < narr; ++ax)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        const bool sh = a.side && d == ax;
+                        const int b = sh ? bs[d] : bc[d];
+                        const int alo = P.ilo[d] - a.gcw[d];
+                        const int ahi = P.ihi[d] + a.gcw[d] + (sh ? 1 : 0);
+                        if (b < alo || b + W - 1 > ahi) lflag |= 2 << ax;
+                    }
             }
-            if (lflag) atomicOr(&s_flag, lflag);
+            int pmin = lpid < 0 ? (1 << 30) : lpid, pmax = lpid;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+            {
+#pragma unroll
+                for (int v = 0; v < 2; ++v)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        lmin[v][d] = min(lmin[v][d], __shfl_xor(lmin[v][d], off));
+                        lmax[v][d] = max(lmax[v][d], __shfl_xor(lmax[v][d], off));
+                    }
+                pmin = min(pmin, __shfl_xor(pmin, off));
+                pmax = max(pmax, __shfl_xor(pmax, off));
+                lflag |= __shfl_xor(lflag, off);
+            }
+            if (lane == 0)
+            {
+#pragma unroll
+                for (int v = 0; v < 2; ++v)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        atomicMin(&s_lo[v][d], lmin[v][d]);
+                        atomicMax(&s_hi[v][d], lmax[v][d]);
+                    }
+                if (pmax >= 0)
+                {
+                    if (pmin != pmax) lflag |= 1;
+                    const int prev = atomicCAS(&s_pid, -1, pmin);
+                    if (prev != -1 && prev != pmin) lflag |= 1;
+                }
+                if (lflag) atomicOr(&s_flag, lflag);
+            }
         }
     }
     __syncthreads();
@@ -297,7 +319,13 @@ __global__ __launch_bounds__(I3_THREADS) void k_interp3(InterpArgs<T> a, int lds
                     }
                 }
                 Sum<T> acc;
-                if (staged)
+                // a static superbin's block holds the stencils of positions
+                // inside its cells up to rounding at cell boundaries: others are
+                // read from the array
+                bool in_block = staged;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) in_block = in_block && (unsigned)(bse[d] - blo[d]) <= (unsigned)(e[d] - W);
+                if (in_block)
                 {
                     const T* base = s_u + (bse[0] - blo[0]) + e[0] * ((bse[1] - blo[1]) + e[1] * (bse[2] - blo[2]));
 #pragma unroll

@@ -180,6 +180,7 @@ int ibamr_plan_destroy(ibamr_plan* P)
     dfree(P->patches_d);
     for (TileList& t : P->tlists) dfree(t.d);
     dfree(P->wt_d);
+    dfree(P->sb_owner_d);
     dfree(P->st_d);
     dfree(P->bin_patch_start_d);
     dfree(P->fill_segs_d);
@@ -444,7 +445,8 @@ static int plan_create_impl(ibamr_plan** out, const ibamr_level_geom* L, const i
     {
         // raise the dynamic LDS limit of this plan's interpolation kernel here, not
         // per launch: the launches may be captured into a HIP graph
-        const int rc = interp_prepare(3, k, C->precision);
+        int rc = interp_prepare(3, k, C->precision);
+        if (rc == IBAMR_OK) rc = build_superbin_owners(P);
         if (rc != IBAMR_OK) return cleanup_fail(rc);
     }
     *out = P;

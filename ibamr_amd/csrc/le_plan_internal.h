@@ -150,6 +150,13 @@ struct ibamr_plan
     int* st_d = nullptr;
     int nvar = 1;                      // variants per axis: 2 for side data
     bool ws_valid = false;             // wt_d/st_d hold the stencils of Xs
+    // 3-D interpolation: per superbin, the patch owning all its cells when every
+    // staged block of it (the superbin's cells widened by sb_off, the kernel's
+    // stencil start offsets per variant plus the drift) lies inside that
+    // patch's ghosted arrays and fits the LDS block; else -1 (extents from the
+    // points)
+    int* sb_owner_d = nullptr;
+    int sb_off[2][2] = {};
     int* bin_patch_start_d = nullptr;  // [nmem+2]
     int* bin_patches_d = nullptr;
     int4* fill_segs_d = nullptr;       // level ghost fill: (patch, array, first element, count) segments
@@ -287,10 +294,11 @@ int radix_sort_pairs(const ibamr_plan* P, const int* kin, const int* vin, int n,
                      int* kt, int* vt, hipStream_t s);
 int bits_for(int maxkey);
 // le_spread.hip: the spread's work lists for every admissible tile shape of the
-// plan's form, and the fixed-point bound factor
+// plan's form
 int build_tile_lists(ibamr_plan* P);
-// le_interp.hip: per-plan interpolation set-up (dynamic LDS limits)
+// le_interp.hip: per-plan interpolation set-up (dynamic LDS limits; superbin owners)
 int interp_prepare(int ndim, int kernel, int precision);
+int build_superbin_owners(ibamr_plan* P);
 // le_plan.hip: timing of the main spread / interpolation launches
 int time_begin(ibamr_plan* P, int which, hipStream_t s, hipEvent_t* end);
 } // namespace ibamr

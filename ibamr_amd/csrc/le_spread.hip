@@ -739,32 +739,44 @@ int ibamr_plan_autotune(ibamr_plan* P, int slot, const void* F, const double* X,
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int best = start;
     float best_ms = std::numeric_limits<float>::infinity();
+    // rounds over all shapes, each timing `reps` spreads per shape; a shape's
+    // time is its fastest round (the shapes are interleaved, so a slow spell of
+    // the device does not fall on one shape only)
+    constexpr int ROUNDS = 3;
     auto run = [&]() -> int {
         HIP_OK(hipEventCreate(&e0));
         HIP_OK(hipEventCreate(&e1));
-        for (size_t i = 0; i < P->tlists.size(); ++i)
+        const size_t ns = P->tlists.size();
+        for (size_t i = 0; i < ns; ++i)
         {
             for (int d = 0; d < 3; ++d) shapes[3 * i + d] = P->tlists[i].shape[d];
+            ms[i] = std::numeric_limits<float>::infinity();
             P->tcur = (int)i;
-            int r = ibamr_plan_spread(P, slot, F, X, IBAMR_SPREAD_OVERWRITE, stream);
+            const int r = ibamr_plan_spread(P, slot, F, X, IBAMR_SPREAD_OVERWRITE, stream);  // untimed
             if (r != IBAMR_OK) return r;
-            HIP_OK(hipEventRecord(e0, s));
-            for (int k = 0; k < reps; ++k)
+        }
+        for (int round = 0; round < ROUNDS; ++round)
+            for (size_t i = 0; i < ns; ++i)
             {
-                r = ibamr_plan_spread(P, slot, F, X, IBAMR_SPREAD_OVERWRITE, stream);
-                if (r != IBAMR_OK) return r;
+                P->tcur = (int)i;
+                HIP_OK(hipEventRecord(e0, s));
+                for (int k = 0; k < reps; ++k)
+                {
+                    const int r = ibamr_plan_spread(P, slot, F, X, IBAMR_SPREAD_OVERWRITE, stream);
+                    if (r != IBAMR_OK) return r;
+                }
+                HIP_OK(hipEventRecord(e1, s));
+                HIP_OK(hipEventSynchronize(e1));
+                float t = 0.0f;
+                HIP_OK(hipEventElapsedTime(&t, e0, e1));
+                ms[i] = std::min(ms[i], t / reps);
             }
-            HIP_OK(hipEventRecord(e1, s));
-            HIP_OK(hipEventSynchronize(e1));
-            float t = 0.0f;
-            HIP_OK(hipEventElapsedTime(&t, e0, e1));
-            ms[i] = t / reps;
+        for (size_t i = 0; i < ns; ++i)
             if (ms[i] < best_ms)
             {
                 best_ms = ms[i];
                 best = (int)i;
             }
-        }
         return IBAMR_OK;
     };
     const int rc = run();

@@ -64,8 +64,9 @@ enum
 /* Kernel registry, LEInteractor::isKnownKernel / getStencilSize /
  * getMinimumGhostWidth (LEInteractor.cpp:2035-2114): the reference's 22 names
  * with their stencil sizes (USER_DEFINED reports 0: no user kernel is
- * registered).  "IB_4", "IB_6" and "BSPLINE_3" have GPU spread/interp; plans
- * for the other names fail with IBAMR_ERR_BAD_KERNEL.  Unknown names: 0 from
+ * registered).  "IB_4", "IB_6", "BSPLINE_3".."BSPLINE_6" and
+ * "PIECEWISE_LINEAR" have GPU spread/interp; plans for the other names fail
+ * with IBAMR_ERR_BAD_KERNEL.  Unknown names: 0 from
  * ibamr_is_known_kernel, IBAMR_ERR_BAD_KERNEL from the other two. */
 int ibamr_is_known_kernel(const char* kernel_fcn);
 int ibamr_get_stencil_size(const char* kernel_fcn);
@@ -386,12 +387,13 @@ int ibamr_plan_info(const ibamr_plan* plan, int* ntiles, int* tile, int* nbins);
  *   set_tile_shape / get_tile_shape: shape[3].
  *   set_tile_height / get_tile_height: the first shape with that z extent / the
  *     z extent of the shape in use.
- *   autotune: for each shape, one untimed and `reps` timed spreads of F at X
- *     into the arrays bound to `slot` (IBAMR_SPREAD_OVERWRITE: their values are
- *     replaced), then keeps the fastest.  shapes[3*i .. 3*i+2] and ms[i]
- *     (capacity IBAMR_MAX_TILE_SHAPES) receive each candidate and its mean time
- *     per spread call, *ncand their number; synchronises the stream.  Points
- *     must be binned. */
+ *   autotune: for each shape one untimed spread of F at X into the arrays
+ *     bound to `slot` (IBAMR_SPREAD_OVERWRITE: their values are replaced), then
+ *     3 rounds over all shapes of `reps` timed spreads each, and keeps the
+ *     shape with the fastest round.  shapes[3*i .. 3*i+2] and ms[i] (capacity
+ *     IBAMR_MAX_TILE_SHAPES) receive each candidate and its best round's mean
+ *     time per spread call, *ncand their number; synchronises the stream.
+ *     Points must be binned. */
 #define IBAMR_MAX_TILE_SHAPES 8
 int ibamr_plan_set_tile_shape(ibamr_plan* plan, const int* shape);
 int ibamr_plan_get_tile_shape(const ibamr_plan* plan, int* shape);
