@@ -501,21 +501,27 @@ __global__ __launch_bounds__(64) void k_spread3s(SpreadArgs<float> a, int seg)
     constexpr int ZW = S3S_ZW;
     constexpr int NZZ = ZW - W + 1;  // window positions of a stencil's first z value
     static_assert(TX * TY == 64, "one lane per grid line");
-    __shared__ T s_rx[64 * PX];
-    __shared__ T s_ry[64 * PY];
-    __shared__ __align__(16) T s_wz[64 * 8];
+    // staged rows of the chunk's 64 points, plus row 64 of zeros (the partner
+    // of the last point of a z start with an odd count)
+    __shared__ T s_rx[65 * PX];
+    __shared__ T s_ry[65 * PY];
+    __shared__ __align__(16) T s_wz[65 * 8];
     __shared__ int s_lb[2][16];
     __shared__ int s_lo[2][16];
     __shared__ int s_ln[3];
     __shared__ int s_zb[S3S_MAXZ];  // z layers: bin, image offset; increasing shifted z
     __shared__ int s_zo[S3S_MAXZ];
     __shared__ int s_cd[64];        // per (x, y) combination of the layer: first sorted point - first position
-    __shared__ int s_co[64];        //                                      image offset code
+    __shared__ int s_sx[64];        //                                      image shift in x (cells)
+    __shared__ int s_sy[64];        //                                      image shift in y
 
     const int lane = threadIdx.x;
     const int x = lane % TX, y = lane / TX;
     const TileDesc td = a.tiles[blockIdx.x];
     if (td.patch < 0) return;  // padding of the XCD-ordered work list
+    if (lane < PX) s_rx[64 * PX + lane] = T(0);
+    if (lane < PY) s_ry[64 * PY + lane] = T(0);
+    if (lane < 8) s_wz[64 * 8 + lane] = T(0);
     const PatchDev P = a.patches[td.patch];
     const int axis = a.side ? td.comp : -1;
     const int tsz[3] = {TX, TY, seg};
@@ -707,7 +713,8 @@ __global__ __launch_bounds__(64) void k_spread3s(SpreadArgs<float> a, int seg)
             }
             lex = inc - ln;
             s_cd[lane] = st - lex;
-            s_co[lane] = lane < nxy ? (s_lo[0][lane % nx] + 1) + 3 * (s_lo[1][lane / nx] + 1) + 9 * (s_zo[ez] + 1) : 13;
+            s_sx[lane] = lane < nxy ? s_lo[0][lane % nx] * a.pshift[0] : 0;
+            s_sy[lane] = lane < nxy ? s_lo[1][lane / nx] * a.pshift[1] : 0;
             __syncthreads();
             gtotal = __builtin_amdgcn_readlane(inc, 63);
             cf = 0;
@@ -717,7 +724,7 @@ __global__ __launch_bounds__(64) void k_spread3s(SpreadArgs<float> a, int seg)
     };
     // the lane's point of the chunk at cq: its index, image code, cell, stencil and F
     int cq = 0;
-    int nk = -1, noc = 13;
+    int nk = -1, nsx = 0, nsy = 0;
     int4 nc4 = make_int4(0, 0, 0, 0);
     int nst[3] = {0, 0, 0};
     T nw[3][W];
@@ -738,7 +745,8 @@ __global__ __launch_bounds__(64) void k_spread3s(SpreadArgs<float> a, int seg)
         nk = -1;
         if (q >= gtotal) return;
         nk = q + s_cd[comb];
-        noc = s_co[comb];
+        nsx = s_sx[comb];
+        nsy = s_sy[comb];
         nc4 = a.cells[nk];
         // precomputed in the level's frame (k_point_weights)
 #pragma unroll
@@ -755,25 +763,29 @@ __global__ __launch_bounds__(64) void k_spread3s(SpreadArgs<float> a, int seg)
     bool have = next_layer();
     if (have) fetch();
     const T iv = T(1.0 / a.vol);
+    int zsh = 0;  // the layer's image shift in z (cells)
     while (have)
     {
         // a layer's first chunk: place the window for it
         if (cq == 0)
-            shift_to(max(a.dom_lo[2] + s_zb[ez] * a.bin + s_zo[ez] * a.pshift[2], clo[2]) - a.reach);
+        {
+            zsh = s_zo[ez] * a.pshift[2];
+            shift_to(max(a.dom_lo[2] + s_zb[ez] * a.bin + zsh, clo[2]) - a.reach);
+        }
         // ---- stage this chunk
         int my_zz = 1 << 20;
         if (nk >= 0)
         {
-            const int o[3] = {noc % 3 - 1, (noc / 3) % 3 - 1, noc / 9 - 1};
+            const int sh[3] = {nsx, nsy, zsh};
             const int cc[3] = {nc4.x, nc4.y, nc4.z};
             bool ok = true;
             int bse[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d)
             {
-                const int ci = cc[d] + o[d] * a.pshift[d];
+                const int ci = cc[d] + sh[d];
                 ok = ok && ci >= clo[d] && ci <= chi[d];
-                bse[d] = nst[d] + o[d] * a.pshift[d];  // an image moves the stencil by whole periods
+                bse[d] = nst[d] + sh[d];  // an image moves the stencil by whole periods
             }
             const int dx0 = bse[0] - flo[0], dy0 = bse[1] - flo[1], zz = bse[2] - wb;
             if (ok && dx0 > -W && dx0 < TX && dy0 > -W && dy0 < TY && zz >= 0 && zz < NZZ)
@@ -823,9 +835,8 @@ __global__ __launch_bounds__(64) void k_spread3s(SpreadArgs<float> a, int seg)
                     // two points per step so their LDS reads overlap
                     const int j = __builtin_ctzll(m);
                     m &= m - 1;
-                    const int j2 = m ? __builtin_ctzll(m) : j;
-                    const bool two = m != 0;
-                    if (two) m &= m - 1;
+                    const int j2 = m ? __builtin_ctzll(m) : 64;
+                    m &= m - 1;
                     const T wx = s_rx[j * PX + x];
                     const T wy = s_ry[j * PY + y];
                     const T wx2 = s_rx[j2 * PX + x];
@@ -838,7 +849,7 @@ __global__ __launch_bounds__(64) void k_spread3s(SpreadArgs<float> a, int seg)
                         wz2[i] = s_wz[j2 * 8 + i];
                     }
                     const T b = wx * wy;
-                    const T b2 = two ? wx2 * wy2 : T(0);
+                    const T b2 = wx2 * wy2;
 #pragma unroll
                     for (int i = 0; i < W; ++i) part[zz + i] = fmaf(b2, wz2[i], fmaf(b, wz[i], part[zz + i]));
                 }
