@@ -88,7 +88,11 @@ def counters_from_profile(kernel, workload_tag):
     with open(path) as fh:
         for line in fh:
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 3 and cells[0].startswith("VALU busy"):
+            if len(cells) == 3 and cells[0] == "counter":
+                # the summary's kernels (`name<template args>`) must be the ones measured now
+                if cells[1 + col].strip("`").split("<")[0] != kernel:
+                    return None
+            elif len(cells) == 3 and cells[0].startswith("VALU busy"):
                 out["valu_busy_frac"] = float(cells[1 + col].rstrip(" %")) / 100.0
             elif len(cells) == 3 and cells[0].startswith("LDS bank-conflict rate"):
                 out["lds_bank_conflict_frac"] = float(cells[1 + col].rstrip(" %")) / 100.0
