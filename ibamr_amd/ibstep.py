@@ -85,8 +85,10 @@ class StructureSet:
             for d in range(self.level.ndim):
                 torch.add(self.u_bg[d], self.f[d], alpha=self.alpha, out=self.u[d])
         self.plans[0].fill_periodic_ghosts(1, stream=s)
+        # X has not moved since the spread: each plan interpolates from the
+        # bin-ordered copy (and, FP32C, the stencils) its spread set up
         for p, a, b in self._slices():
-            p.interp(1, U[a:b], X[a:b], stream=s)
+            p.interp(1, U[a:b], None, stream=s)
         # X + dt U and the move back into the periodic box in one kernel (one
         # period suffices: a step moves a point by less than one period)
         ia.forward_euler_step_periodic(X, X, U, dt, self.level, stream=s)
