@@ -46,7 +46,8 @@ C_API = [
     "ibamr_plan_info", "ibamr_plan_set_tile_height", "ibamr_plan_get_tile_height", "ibamr_plan_autotune",
     "ibamr_plan_set_tile_shape", "ibamr_plan_get_tile_shape", "ibamr_plan_get_cell_indices",
     "ibamr_plan_redistribute", "ibamr_permute_rows",
-    "ibamr_plan_set_physical_boundary", "ibamr_plan_accumulate_physical_boundary", "ibamr_plan_fill_physical_boundary", "ibamr_fe_quadrature_points",
+    "ibamr_plan_set_physical_boundary", "ibamr_plan_set_physical_boundary_coefs",
+    "ibamr_plan_accumulate_physical_boundary", "ibamr_plan_fill_physical_boundary", "ibamr_fe_quadrature_points",
     "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
     "ibamr_s0_interp", "ibamr_s0_spread", "ibamr_s0_interp_noalloc",
@@ -131,6 +132,8 @@ def lib():
     L.ibamr_permute_rows.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     L.ibamr_plan_set_physical_boundary.argtypes = [vp, ctypes.POINTER(BdryBox), ctypes.c_int,
                                                    ctypes.POINTER(ctypes.c_double)]
+    L.ibamr_plan_set_physical_boundary_coefs.argtypes = [vp, ctypes.POINTER(BdryBox), ctypes.c_int,
+                                                         ctypes.POINTER(ctypes.c_double), ctypes.c_long]
     L.ibamr_plan_accumulate_physical_boundary.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_plan_fill_physical_boundary.argtypes = [vp, ctypes.c_int, vp]
     L.ibamr_fe_quadrature_points.argtypes = [ctypes.c_int] * 6 + [vp] * 7 + [ctypes.c_int, vp]
@@ -397,21 +400,42 @@ class LevelPlan:
                                              _ptr(order), _ptr(poff), _ptr(offs), _stream_ptr(stream)))
         return order[:n], poff, (offs[:n] if offs is not None else None)
 
-    def set_physical_boundary(self, boxes, abg):
-        """boxes: [(patch, codim, location, lower, upper)] in each patch's order;
-        abg: [nboxes][arrays][3] Robin coefficients (a, b, g) of codim-1 boxes."""
-        import numpy as np
+    @staticmethod
+    def _bdry_boxes(boxes):
         arr = (BdryBox * max(1, len(boxes)))()
         for k, (p, c, l, lo, hi) in enumerate(boxes):
             arr[k].patch, arr[k].codim, arr[k].location = p, c, l
             for d in range(len(lo)):
                 arr[k].lower[d], arr[k].upper[d] = lo[d], hi[d]
+        return arr
+
+    def set_physical_boundary(self, boxes, abg):
+        """boxes: [(patch, codim, location, lower, upper)] in each patch's order;
+        abg: [nboxes][arrays][3] Robin coefficients (a, b, g) of codim-1 boxes."""
+        import numpy as np
+        arr = self._bdry_boxes(boxes)
         ab = np.ascontiguousarray(abg, dtype=np.float64).reshape(-1)
         if ab.size == 0:
             ab = np.zeros(3)
         self._bdry_keep = ab
         _check(lib().ibamr_plan_set_physical_boundary(self._h, arr, len(boxes),
                                                       ab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def set_physical_boundary_coefs(self, boxes, coefs):
+        """The same boxes with coefficients varying over each codim-1 box: coefs
+        lists, for each codim-1 box in order and each array j, the (a, b, g)
+        arrays over the box's columns (lower tangential axis fastest; side data:
+        one more column along j when j is tangential to the box), each a
+        flat sequence or an array of shape (3, n)."""
+        import numpy as np
+        arr = self._bdry_boxes(boxes)
+        flat = [np.ascontiguousarray(c, dtype=np.float64).reshape(-1) for c in coefs]
+        cf = np.concatenate(flat) if flat else np.zeros(0)
+        keep = cf if cf.size else np.zeros(1)
+        self._bdry_keep = keep
+        _check(lib().ibamr_plan_set_physical_boundary_coefs(self._h, arr, len(boxes),
+                                                            keep.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                            int(cf.size)))
 
     def accumulate_physical_boundary(self, slot, stream=None):
         """accumulateFromPhysicalBoundaryData on every patch (after a spread)."""

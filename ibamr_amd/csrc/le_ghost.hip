@@ -178,10 +178,13 @@ struct BdryArgs
 {
     const int4* patches_list;
     const int* boxes;
-    const double* abg;
+    // Robin coefficients of codim-1 boxes: for box slot k and array j, a, b and g
+    // over the box's columns start at coef + coef_off[k * narr + j]
+    const double* coef;
+    const long* coef_off;
     const PatchDev* patches;
     T* const* arrays;
-    int ndim, side, depth, narr, ncoef, gcw, adjoint;  // ncoef: coefficient triples per box
+    int ndim, side, depth, narr, gcw, adjoint;
     double dx[3];
 };
 
@@ -233,7 +236,7 @@ __device__ inline long face_columns(int ndim, int dir, const int* blo, const int
 
 template <typename T>
 __device__ void bd_cc_op1(const BPatch<T>& P, int a, int comp, int loc, const int* dhi, const int* blo,
-                          const int* bhi, double ca, double cb, double cg, int adjoint, const double* dx)
+                          const int* bhi, const double* cf, int adjoint, const double* dx)
 {
     const int ndim = P.ndim, dir = loc / 2, lower = (loc % 2) == 0;
     const double h = dx[loc / ndim];  // as the Fortran indexes it
@@ -244,6 +247,8 @@ __device__ void bd_cc_op1(const BPatch<T>& P, int a, int comp, int loc, const in
     {
         int x[3] = {0, 0, 0};
         face_column(ndim, dir, blo, bhi, c, x);
+        // column c of the coefficient box (the lower tangential axis fastest, as acoef(j,k))
+        const double ca = cf[c], cb = cf[ncol + c], cg = cf[2 * ncol + c];
         for (int i = 0; i < P.gcw; ++i)
         {
             const double n = 1.0 + 2.0 * i;
@@ -268,19 +273,20 @@ __device__ void bd_cc_op1(const BPatch<T>& P, int a, int comp, int loc, const in
 }
 
 template <typename T>
-__device__ void bd_sc_op1(const BPatch<T>& P, int loc, const int* blo, const int* bhi, double ca, double cb, double cg,
-                          int adjoint, const double* dx)
+__device__ void bd_sc_op1(const BPatch<T>& P, int loc, const int* blo, const int* bhi, const double* cf, int adjoint,
+                          const double* dx)
 {
     const int ndim = P.ndim, dir = loc / 2, lower = (loc % 2) == 0;
     const double h = dx[loc / ndim];
     const int sgn = lower ? -1 : 1;
     const int ib = lower ? P.ilo[dir] : P.ihi[dir] + 1;
-    const bool dirichlet = fabs(cb) < 1.0e-12;
     const long ncol = face_columns(ndim, dir, blo, bhi);
     for (long c = threadIdx.x; c < ncol; c += blockDim.x)
     {
         int x[3] = {0, 0, 0};
         face_column(ndim, dir, blo, bhi, c, x);
+        const double ca = cf[c], cb = cf[ncol + c], cg = cf[2 * ncol + c];
+        const bool dirichlet = fabs(cb) < 1.0e-12;  // per face (scrobinphysbdryop1: abs(b) .lt. 1.d-12)
         int xb[3] = {x[0], x[1], x[2]};
         xb[dir] = ib;
         T* ub = P.at(dir, 0, xb);
@@ -605,14 +611,15 @@ __global__ __launch_bounds__(256) void k_phys_bdry(BdryArgs<T> a)
             else if ((phase == 10 || phase == 11) && codim == 1)
             {
                 const int dir = loc / 2;
-                const double* q = a.abg + 3 * (long)k * a.ncoef;
+                const double* q = a.coef;
+                // coefficient arrays per box: one per axis (side data) or per component (cell data)
+                const long* qo = a.coef_off + (long)k * (a.side ? a.ndim : a.depth);
                 if (!a.side)
                 {
                     if (phase == 10)
                         for (int c = 0; c < a.depth; ++c)
                         {
-                            bd_cc_op1(P, 0, c, loc, P.ihi, lo, hi, q[3 * c], q[3 * c + 1], q[3 * c + 2], a.adjoint,
-                                      a.dx);
+                            bd_cc_op1(P, 0, c, loc, P.ihi, lo, hi, q + qo[c], a.adjoint, a.dx);
                             __syncthreads();
                         }
                 }
@@ -624,14 +631,13 @@ __global__ __launch_bounds__(256) void k_phys_bdry(BdryArgs<T> a)
                         int shi[3] = {P.ihi[0], P.ihi[1], P.ihi[2]}, bhi[3] = {hi[0], hi[1], hi[2]};
                         shi[ax] += 1;
                         bhi[ax] += 1;
-                        bd_cc_op1(P, ax, 0, loc, shi, lo, bhi, q[3 * ax], q[3 * ax + 1], q[3 * ax + 2], a.adjoint,
-                                  a.dx);
+                        bd_cc_op1(P, ax, 0, loc, shi, lo, bhi, q + qo[ax], a.adjoint, a.dx);
                         __syncthreads();
                     }
                 }
                 else
                 {
-                    bd_sc_op1(P, loc, lo, hi, q[3 * dir], q[3 * dir + 1], q[3 * dir + 2], a.adjoint, a.dx);
+                    bd_sc_op1(P, loc, lo, hi, q + qo[dir], a.adjoint, a.dx);
                     __syncthreads();
                 }
                 did = true;
@@ -730,10 +736,28 @@ int ibamr_plan_fill_ghosts(ibamr_plan* P, int slot, void* stream)
     return P->precision == IBAMR_FP64 ? go((double)0) : go((float)0);
 }
 
-int ibamr_plan_set_physical_boundary(ibamr_plan* P, const ibamr_bdry_box* boxes, int nboxes, const double* abg)
+// Coefficient values of a codim-1 box for array j: its tangential cells, one more
+// along j for a side array tangential to the box (the coefficient box's
+// tangential extension, CartSideRobinPhysBdryOp.cpp:913).
+static long bdry_ncoef(const ibamr_plan* P, const ibamr_bdry_box& b, int j)
+{
+    const int dir = b.location / 2;
+    long m = 1;
+    for (int d = 0; d < P->ndim; ++d)
+        if (d != dir) m *= (long)(b.upper[d] - b.lower[d] + 1 + (P->centering == IBAMR_SIDE && d == j ? 1 : 0));
+    return m;
+}
+
+// Boxes with uniform coefficients (abg: [nboxes][arrays][3]) or coefficients over
+// each codim-1 box (coefs: per codim-1 box in the given order and per array, a, b
+// and g over the box's columns); exactly one of abg and coefs is given.
+static int set_physical_boundary(ibamr_plan* P, const ibamr_bdry_box* boxes, int nboxes, const double* abg,
+                                 const double* coefs, long ncoefs, bool varying)
 {
     if (!P) return fail(IBAMR_ERR_INVALID, "null plan");
-    if (nboxes < 0 || (nboxes > 0 && (!boxes || !abg))) return fail(IBAMR_ERR_INVALID, "null boundary boxes");
+    if (nboxes < 0 || (nboxes > 0 && !boxes)) return fail(IBAMR_ERR_INVALID, "null boundary boxes");
+    if (nboxes > 0 && !(varying ? (const void*)coefs : (const void*)abg))
+        return fail(IBAMR_ERR_INVALID, "null boundary coefficients");
     for (int d = 1; d < P->ndim; ++d)
         if (P->data_gcw[d] != P->data_gcw[0])
             return fail(IBAMR_ERR_GHOST_WIDTH, "physical boundary operators need a uniform ghost width");
@@ -798,10 +822,24 @@ int ibamr_plan_set_physical_boundary(ibamr_plan* P, const ibamr_bdry_box* boxes,
         }
         per[b.patch].push_back(k);
     }
+    const int narr = P->centering == IBAMR_SIDE ? P->ndim : P->depth;
+    // where box k's coefficients for array j start in the caller's coefs
+    std::vector<long> uoff((size_t)nboxes * narr, 0);
+    long need = 0;
+    for (int k = 0; k < nboxes; ++k)
+        if (boxes[k].codim == 1)
+            for (int j = 0; j < narr; ++j)
+            {
+                uoff[(size_t)k * narr + j] = need;
+                need += 3 * bdry_ncoef(P, boxes[k], j);
+            }
+    if (varying && ncoefs != need)
+        return fail(IBAMR_ERR_INVALID, "boundary coefficients: " + std::to_string(ncoefs) + " values given, the boxes "
+                                       "need " + std::to_string(need));
     std::vector<int4> pl;
     std::vector<int> bx;
-    std::vector<double> ab;
-    const int narr = P->centering == IBAMR_SIDE ? P->ndim : P->depth;
+    std::vector<double> cf;
+    std::vector<long> cfo;
     for (int p = 0; p < P->npatches; ++p)
     {
         if (per[p].empty()) continue;
@@ -812,24 +850,52 @@ int ibamr_plan_set_physical_boundary(ibamr_plan* P, const ibamr_bdry_box* boxes,
             const int v[8] = {b.codim, b.location, b.lower[0], b.lower[1], P->ndim == 3 ? b.lower[2] : 0,
                               b.upper[0], b.upper[1], P->ndim == 3 ? b.upper[2] : 0};
             bx.insert(bx.end(), v, v + 8);
-            ab.insert(ab.end(), abg + 3 * (long)k * narr, abg + 3 * (long)(k + 1) * narr);
+            for (int j = 0; j < narr; ++j)
+            {
+                cfo.push_back((long)cf.size());
+                if (b.codim != 1) continue;
+                const long m = bdry_ncoef(P, b, j);
+                if (varying)
+                {
+                    const double* u = coefs + uoff[(size_t)k * narr + j];
+                    cf.insert(cf.end(), u, u + 3 * m);
+                }
+                else
+                    for (int c = 0; c < 3; ++c) cf.insert(cf.end(), (size_t)m, abg[3 * ((long)k * narr + j) + c]);
+            }
         }
     }
     dfree(P->bdry_patch_d);
     dfree(P->bdry_box_d);
-    dfree(P->bdry_abg_d);
+    dfree(P->bdry_coef_d);
+    dfree(P->bdry_coef_off_d);
     P->bdry_patch_d = nullptr;
     P->bdry_box_d = nullptr;
-    P->bdry_abg_d = nullptr;
+    P->bdry_coef_d = nullptr;
+    P->bdry_coef_off_d = nullptr;
     P->nbdry_patches = (int)pl.size();
     if (pl.empty()) return IBAMR_OK;
+    if (cf.empty()) cf.push_back(0.0);
     HIP_OK(hipMalloc(&P->bdry_patch_d, sizeof(int4) * pl.size()));
     HIP_OK(hipMalloc(&P->bdry_box_d, sizeof(int) * bx.size()));
-    HIP_OK(hipMalloc(&P->bdry_abg_d, sizeof(double) * ab.size()));
+    HIP_OK(hipMalloc(&P->bdry_coef_d, sizeof(double) * cf.size()));
+    HIP_OK(hipMalloc(&P->bdry_coef_off_d, sizeof(long) * cfo.size()));
     HIP_OK(hipMemcpy(P->bdry_patch_d, pl.data(), sizeof(int4) * pl.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(P->bdry_box_d, bx.data(), sizeof(int) * bx.size(), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(P->bdry_abg_d, ab.data(), sizeof(double) * ab.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(P->bdry_coef_d, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(P->bdry_coef_off_d, cfo.data(), sizeof(long) * cfo.size(), hipMemcpyHostToDevice));
     return IBAMR_OK;
+}
+
+int ibamr_plan_set_physical_boundary(ibamr_plan* P, const ibamr_bdry_box* boxes, int nboxes, const double* abg)
+{
+    return set_physical_boundary(P, boxes, nboxes, abg, nullptr, 0, false);
+}
+
+int ibamr_plan_set_physical_boundary_coefs(ibamr_plan* P, const ibamr_bdry_box* boxes, int nboxes,
+                                           const double* coefs, long ncoefs)
+{
+    return set_physical_boundary(P, boxes, nboxes, nullptr, coefs, ncoefs, true);
 }
 
 static int phys_bdry_launch(ibamr_plan* P, int slot, int adjoint, void* stream)
@@ -843,14 +909,14 @@ static int phys_bdry_launch(ibamr_plan* P, int slot, int adjoint, void* stream)
         BdryArgs<T> a{};
         a.patches_list = P->bdry_patch_d;
         a.boxes = P->bdry_box_d;
-        a.abg = P->bdry_abg_d;
+        a.coef = P->bdry_coef_d;
+        a.coef_off = P->bdry_coef_off_d;
         a.patches = P->patches_d;
         a.arrays = reinterpret_cast<T* const*>(P->arrays_d[slot]);
         a.ndim = P->ndim;
         a.side = P->centering == IBAMR_SIDE;
         a.depth = P->depth;
         a.narr = P->narr;
-        a.ncoef = P->centering == IBAMR_SIDE ? P->ndim : P->depth;
         a.gcw = P->data_gcw[0];
         a.adjoint = adjoint;
         for (int d = 0; d < 3; ++d) a.dx[d] = P->dx[d];

@@ -199,7 +199,8 @@ int ibamr_plan_destroy(ibamr_plan* P)
     dfree(P->pcell_off_d);
     dfree(P->bdry_patch_d);
     dfree(P->bdry_box_d);
-    dfree(P->bdry_abg_d);
+    dfree(P->bdry_coef_d);
+    dfree(P->bdry_coef_off_d);
     dfree(P->status_d);
     dfree(P->rs_hist_d);
     dfree(P->rs_off_d);

@@ -173,7 +173,8 @@ struct ibamr_plan
     int* status_d = nullptr;           // [8] redistribution counters
     int4* bdry_patch_d = nullptr;      // physical boundary: (patch, first box, number of boxes, 0) per patch
     int* bdry_box_d = nullptr;         // [nbdry][8]: codim, location, lower[3], upper[3]
-    double* bdry_abg_d = nullptr;      // [nbdry][narr][3] Robin coefficients
+    double* bdry_coef_d = nullptr;     // Robin a, b, g over each codim-1 box's columns, per array
+    long* bdry_coef_off_d = nullptr;   // [nbdry][narr] where each (box, array)'s coefficients start
     int nbdry_patches = 0;
     int* rs_hist_d = nullptr;          // [RS_DIGITS * blocks] (digit, block) counts
     int* rs_off_d = nullptr;           // [RS_DIGITS * blocks + 1] their exclusive scan

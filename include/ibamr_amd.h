@@ -300,6 +300,19 @@ typedef struct
     int upper[3];
 } ibamr_bdry_box;
 int ibamr_plan_set_physical_boundary(ibamr_plan* plan, const ibamr_bdry_box* boxes, int nboxes, const double* abg);
+/* The same boxes with coefficients that vary over each codim-1 box, as a
+ * RobinBcCoefStrategy::setBcCoefs fills acoef/bcoef/gcoef over the coefficient
+ * box (CartCellRobinPhysBdryOp.cpp:518-531; side data: the normal axis at
+ * CartSideRobinPhysBdryOp.cpp:664-677, the tangential axes over the coefficient
+ * box extended by one cell along the array's axis, :913-947).  coefs holds, for
+ * each codim-1 box in the given order and each array j (as for abg), a[n],
+ * b[n], g[n] over the box's n columns, the lower tangential axis fastest (the
+ * Fortran's acoef(j,k) order); n is the product of the box's tangential extents,
+ * plus one along j for a side array tangential to the box.  ncoefs: the number
+ * of doubles in coefs, checked against what the boxes need.  The Dirichlet test
+ * |b| < 1e-12 is taken per face, as the reference's kernels take it. */
+int ibamr_plan_set_physical_boundary_coefs(ibamr_plan* plan, const ibamr_bdry_box* boxes, int nboxes,
+                                           const double* coefs, long ncoefs);
 int ibamr_plan_accumulate_physical_boundary(ibamr_plan* plan, int slot, void* stream);
 int ibamr_plan_fill_physical_boundary(ibamr_plan* plan, int slot, void* stream);
 

@@ -1065,7 +1065,7 @@ static int bp_tangential(int ndim, int dir, int* t)
 /* ccrobinphysbdryop1{x,y,z}: data box [dlo, dhi] (the patch box, or the side box of
  * array a for the transverse use), tangential ranges from the coefficient box */
 static void bp_cc_op1(const orc_bpatch* P, int a, int comp, int loc, const int* dlo, const int* dhi, const int* blo,
-                      const int* bhi, double ca, double cb, double cg, int adjoint)
+                      const int* bhi, const double* cf, long ncf, int adjoint)
 {
     const int ndim = P->ndim, dir = loc / 2, lower = (loc % 2) == 0;
     const double h = P->dx[loc / ndim]; /* as the Fortran indexes it */
@@ -1075,11 +1075,13 @@ static void bp_cc_op1(const orc_bpatch* P, int a, int comp, int loc, const int* 
     const int nt = bp_tangential(ndim, dir, t);
     int x[3] = {0, 0, 0};
     const int o_lo = nt > 1 ? blo[t[0]] : 0, o_hi = nt > 1 ? bhi[t[0]] : 0;
+    long col = 0; /* the coefficient arrays' index: the lower tangential axis fastest */
     for (int o = o_lo; o <= o_hi; ++o)
-        for (int q = blo[t[nt - 1]]; q <= bhi[t[nt - 1]]; ++q)
+        for (int q = blo[t[nt - 1]]; q <= bhi[t[nt - 1]]; ++q, ++col)
         {
             if (nt > 1) x[t[0]] = o;
             x[t[nt - 1]] = q;
+            const double ca = cf[col], cb = cf[ncf + col], cg = cf[2 * ncf + col];
             for (int i = 0; i < P->gcw; ++i)
             {
                 const double n = 1.0 + 2.0 * i;
@@ -1103,7 +1105,7 @@ static void bp_cc_op1(const orc_bpatch* P, int a, int comp, int loc, const int* 
 }
 
 /* scrobinphysbdryop1{x,y,z}: the normal component on its boundary face */
-static void bp_sc_op1(const orc_bpatch* P, int loc, const int* blo, const int* bhi, double ca, double cb, double cg,
+static void bp_sc_op1(const orc_bpatch* P, int loc, const int* blo, const int* bhi, const double* cf, long ncf,
                       int adjoint)
 {
     const int ndim = P->ndim, dir = loc / 2, lower = (loc % 2) == 0;
@@ -1114,11 +1116,13 @@ static void bp_sc_op1(const orc_bpatch* P, int loc, const int* blo, const int* b
     const int nt = bp_tangential(ndim, dir, t);
     int x[3] = {0, 0, 0};
     const int o_lo = nt > 1 ? blo[t[0]] : 0, o_hi = nt > 1 ? bhi[t[0]] : 0;
+    long col = 0;
     for (int o = o_lo; o <= o_hi; ++o)
-        for (int q = blo[t[nt - 1]]; q <= bhi[t[nt - 1]]; ++q)
+        for (int q = blo[t[nt - 1]]; q <= bhi[t[nt - 1]]; ++q, ++col)
         {
             if (nt > 1) x[t[0]] = o;
             x[t[nt - 1]] = q;
+            const double ca = cf[col], cb = cf[ncf + col], cg = cf[2 * ncf + col];
             int xb[3] = {x[0], x[1], x[2]};
             xb[dir] = ib;
             double* ub = bp_at(P, dir, 0, xb);
@@ -1366,8 +1370,30 @@ static void bp_sc_corner(const orc_bpatch* P, int loc, const int* flo, const int
  * side: axis).  adjoint = 1: accumulateFromPhysicalBoundaryData (codim 3, 2,
  * then 1; side: transverse then normal); 0: setPhysicalBoundaryConditions
  * (codim 1 (side: normal, then transverse), 2, 3). */
-void orc_phys_bdry(int ndim, int side, int depth, int gcw, const int* ilo, const int* ihi, const double* dx,
-                   double** arrays, int nboxes, const int* boxes, const double* abg, int adjoint)
+/* Coefficient values of codim-1 box k (boxes: nboxes x {codim, location, lo[3],
+ * hi[3]}) for array j: its tangential cells, one more along j for a side
+ * array tangential to the box (the coefficient box's tangential extension). */
+static long bp_ncoef(int ndim, int side, const int* bx, int j)
+{
+    const int dir = bx[1] / 2;
+    long m = 1;
+    for (int d = 0; d < ndim; ++d)
+        if (d != dir) m *= (long)(bx[5 + d] - bx[2 + d] + 1 + (side && d == j ? 1 : 0));
+    return m;
+}
+
+long orc_phys_bdry_ncoefs(int ndim, int side, int depth, int nboxes, const int* boxes)
+{
+    const int narr = side ? ndim : depth;
+    long n = 0;
+    for (int k = 0; k < nboxes; ++k)
+        if (boxes[8 * k] == 1)
+            for (int j = 0; j < narr; ++j) n += 3 * bp_ncoef(ndim, side, boxes + 8 * k, j);
+    return n;
+}
+
+void orc_phys_bdry_coefs(int ndim, int side, int depth, int gcw, const int* ilo, const int* ihi, const double* dx,
+                         double** arrays, int nboxes, const int* boxes, const double* coefs, int adjoint)
 {
     orc_bpatch P;
     P.ndim = ndim;
@@ -1382,6 +1408,17 @@ void orc_phys_bdry(int ndim, int side, int depth, int gcw, const int* ilo, const
         P.arr[d] = d < (side ? ndim : 1) ? arrays[d] : 0;
     }
     const int narr = side ? ndim : depth;
+    /* each codim-1 box's coefficient arrays (a, b, g over its columns) per array, in box order */
+    long* off = (long*)malloc(sizeof(long) * (size_t)(nboxes * narr + 1));
+    long* ncf = (long*)malloc(sizeof(long) * (size_t)(nboxes * narr + 1));
+    long o = 0;
+    for (int k = 0; k < nboxes; ++k)
+        for (int j = 0; j < narr; ++j)
+        {
+            off[k * narr + j] = o;
+            ncf[k * narr + j] = boxes[8 * k] == 1 ? bp_ncoef(ndim, side, boxes + 8 * k, j) : 0;
+            o += 3 * ncf[k * narr + j];
+        }
     /* phases: adjoint 3, 2, 1T, 1N; forward 1N, 1T, 2, 3 (cell: 1T is the cell op, 1N empty) */
     const int phases_adj[4] = {3, 2, 10, 11}, phases_fwd[4] = {11, 10, 2, 3};
     for (int ph = 0; ph < 4; ++ph)
@@ -1415,10 +1452,8 @@ void orc_phys_bdry(int ndim, int side, int depth, int gcw, const int* ilo, const
                 {
                     if (phase == 10)
                         for (int c = 0; c < depth; ++c)
-                        {
-                            const double* q = abg + 3 * ((long)k * narr + c);
-                            bp_cc_op1(&P, 0, c, loc, P.ilo, P.ihi, lo, hi, q[0], q[1], q[2], adjoint);
-                        }
+                            bp_cc_op1(&P, 0, c, loc, P.ilo, P.ihi, lo, hi, coefs + off[k * narr + c],
+                                      ncf[k * narr + c], adjoint);
                 }
                 else if (phase == 10)
                 {
@@ -1428,16 +1463,36 @@ void orc_phys_bdry(int ndim, int side, int depth, int gcw, const int* ilo, const
                         int shi[3] = {P.ihi[0], P.ihi[1], P.ihi[2]}, bhi[3] = {hi[0], hi[1], hi[2]};
                         shi[a] += 1;  /* the side box of array a */
                         bhi[a] += 1;  /* tangential extension of the coefficient box */
-                        const double* q = abg + 3 * ((long)k * narr + a);
-                        bp_cc_op1(&P, a, 0, loc, P.ilo, shi, lo, bhi, q[0], q[1], q[2], adjoint);
+                        bp_cc_op1(&P, a, 0, loc, P.ilo, shi, lo, bhi, coefs + off[k * narr + a], ncf[k * narr + a],
+                                  adjoint);
                     }
                 }
                 else
-                {
-                    const double* q = abg + 3 * ((long)k * narr + dir);
-                    bp_sc_op1(&P, loc, lo, hi, q[0], q[1], q[2], adjoint);
-                }
+                    bp_sc_op1(&P, loc, lo, hi, coefs + off[k * narr + dir], ncf[k * narr + dir], adjoint);
             }
         }
     }
+    free(off);
+    free(ncf);
+}
+
+void orc_phys_bdry(int ndim, int side, int depth, int gcw, const int* ilo, const int* ihi, const double* dx,
+                   double** arrays, int nboxes, const int* boxes, const double* abg, int adjoint)
+{
+    /* uniform coefficients: every column of a box and array gets its (a, b, g) */
+    const int narr = side ? ndim : depth;
+    const long n = orc_phys_bdry_ncoefs(ndim, side, depth, nboxes, boxes);
+    double* coefs = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    long o = 0;
+    for (int k = 0; k < nboxes; ++k)
+        if (boxes[8 * k] == 1)
+            for (int j = 0; j < narr; ++j)
+            {
+                const long m = bp_ncoef(ndim, side, boxes + 8 * k, j);
+                for (int c = 0; c < 3; ++c)
+                    for (long i = 0; i < m; ++i) coefs[o + c * m + i] = abg[3 * ((long)k * narr + j) + c];
+                o += 3 * m;
+            }
+    orc_phys_bdry_coefs(ndim, side, depth, gcw, ilo, ihi, dx, arrays, nboxes, boxes, coefs, adjoint);
+    free(coefs);
 }

@@ -171,9 +171,20 @@ void orc_fe_integrate_rhs(int nvars, int nelem, int nbasis, int nqp, const int* 
 
 /* CartCellRobinPhysBdryOp / CartSideRobinPhysBdryOp LINEAR ghost operators on one
  * patch: adjoint = accumulateFromPhysicalBoundaryData, else setPhysicalBoundaryConditions.
- * boxes: nboxes x {codim, location, lo[3], hi[3]}; abg: [nboxes][arrays][3]. */
+ * boxes: nboxes x {codim, location, lo[3], hi[3]}; abg: [nboxes][arrays][3], one
+ * (a, b, g) per codim-1 box and array. */
 void orc_phys_bdry(int ndim, int side, int depth, int gcw, const int* ilo, const int* ihi, const double* dx,
                    double** arrays, int nboxes, const int* boxes, const double* abg, int adjoint);
+/* The same with the Robin coefficients varying over each codim-1 box, as
+ * RobinBcCoefStrategy::setBcCoefs fills them on the coefficient box
+ * (CartCellRobinPhysBdryOp.cpp:518-531, CartSideRobinPhysBdryOp.cpp:664-677, 913):
+ * for every codim-1 box in order and every array j (cell data: component; side
+ * data: axis), a[n], b[n], g[n] over the box's tangential cells (one more along
+ * j for a side array tangential to the box), the lower tangential axis fastest
+ * (the Fortran acoef(j,k) order).  orc_phys_bdry_ncoefs: the doubles coefs holds. */
+long orc_phys_bdry_ncoefs(int ndim, int side, int depth, int nboxes, const int* boxes);
+void orc_phys_bdry_coefs(int ndim, int side, int depth, int gcw, const int* ilo, const int* ihi, const double* dx,
+                         double** arrays, int nboxes, const int* boxes, const double* coefs, int adjoint);
 
 void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
                           const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,

@@ -442,3 +442,44 @@ def phys_bdry(arrays, ilower, iupper, gcw, dx, boxes, abg, side, adjoint, depth=
     L.orc_phys_bdry(ndim, 1 if side else 0, depth, gcw, _i(_iv(ilower)), _i(_iv(iupper)), _d(_dv(dx)), ptrs,
                     len(boxes), _i(bx), _d(ab), 1 if adjoint else 0)
     return arrays
+
+
+def phys_bdry_ncoef(ndim, box, j, side):
+    """Coefficient values of a codim-1 box (codim, location, lo, hi) for array j."""
+    _, loc, lo, hi = box
+    m = 1
+    for d in range(ndim):
+        if d != loc // 2:
+            m *= hi[d] - lo[d] + 1 + (1 if side and d == j else 0)
+    return m
+
+
+def phys_bdry_coefs(arrays, ilower, iupper, gcw, dx, boxes, coefs, side, adjoint, depth=1):
+    """orc_phys_bdry_coefs: the operators with Robin coefficients varying over
+    each codim-1 box.  coefs: for every codim-1 box in order and every array,
+    a, b, g over the box's tangential cells (phys_bdry_ncoef values each, the
+    lower tangential axis fastest), concatenated."""
+    ndim = len(ilower)
+    for a in arrays:
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    bx = np.zeros((max(len(boxes), 1), 8), dtype=np.int32)
+    for k, (c, l, lo, hi) in enumerate(boxes):
+        bx[k, 0], bx[k, 1] = c, l
+        bx[k, 2:2 + ndim] = lo
+        bx[k, 5:5 + ndim] = hi
+    cf = _dv(coefs) if len(coefs) else np.zeros(3)
+    L = lib()
+    if not getattr(L, "_pbc_types", False):
+        L.orc_phys_bdry_ncoefs.restype = ctypes.c_long
+        L.orc_phys_bdry_ncoefs.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)]
+        L.orc_phys_bdry_coefs.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 2 + \
+            [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.c_int,
+             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L._pbc_types = True
+    need = L.orc_phys_bdry_ncoefs(ndim, 1 if side else 0, depth, len(boxes), _i(bx))
+    if len(coefs) != need:
+        raise ValueError("coefs holds %d values, the boxes need %d" % (len(coefs), need))
+    ptrs = (ctypes.POINTER(ctypes.c_double) * len(arrays))(*[_d(a) for a in arrays])
+    L.orc_phys_bdry_coefs(ndim, 1 if side else 0, depth, gcw, _i(_iv(ilower)), _i(_iv(iupper)), _d(_dv(dx)), ptrs,
+                          len(boxes), _i(bx), _d(cf), 1 if adjoint else 0)
+    return arrays
