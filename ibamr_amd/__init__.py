@@ -50,6 +50,7 @@ C_API = [
     "ibamr_plan_accumulate_physical_boundary", "ibamr_plan_fill_physical_boundary", "ibamr_fe_quadrature_points",
     "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
+    "ibamr_forces_register_spring_function", "ibamr_forces_set_spring_parameters",
     "ibamr_s0_interp", "ibamr_s0_spread", "ibamr_s0_interp_noalloc",
 ] + ["lagrangian_%s_%s%s_" % (k, op, d) for k in ("ib_4", "ib_6", "bspline_3", "piecewise_linear",
                                                         "bspline_4", "bspline_5", "bspline_6")
@@ -113,6 +114,8 @@ def lib():
                                       ctypes.c_int, ip_, ip_, ip_, dp_, dp_, ctypes.c_int, ip_, dp_, dp_, dp_]
     L.ibamr_forces_compute.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     L.ibamr_forces_destroy.argtypes = [vp]
+    L.ibamr_forces_register_spring_function.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.ibamr_forces_set_spring_parameters.argtypes = [vp, ip_, ip_, dp_]
     L.ibamr_forward_euler_step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_long, vp]
     L.ibamr_forward_euler_step_periodic.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_long,
                                                     ctypes.c_int, ctypes.POINTER(ctypes.c_double),

@@ -1,8 +1,16 @@
-// ib_forces.hip -- Lagrangian force generation on the GPU: Hookean springs,
-// beams (three-point bending) and target points, the forces IBStandardForceGen
+// ib_forces.hip -- Lagrangian force generation on the GPU: springs, beams
+// (three-point bending) and target points, the forces IBStandardForceGen
 // computes before the spread (src/IB/IBStandardForceGen.cpp:254-305; springs
-// :814-930 with default_spring_force, include/ibamr/IBSpringForceFunctions.h:104;
-// beams :1038-1147; target points :1202-1299).
+// :814-930 with default_spring_force, include/ibamr/IBSpringForceFunctions.h:104,
+// or another registered force function; beams :1038-1147; target points
+// :1202-1299).
+//
+// The reference keeps a map force_fcn_idx -> function pointer
+// (registerSpringForceFunction, :133-140) and resolves every spring's index to
+// its function when the level's data is set up (:728-790).  Host function
+// pointers cannot run on the GPU, so the map here holds built-in device laws
+// (spring_tension below), resolved per spring the same way, when the spring
+// parameters are set.
 //
 // The reference loops over springs, then beams, then target points, adding each
 // element's force into its nodes in element order (a scatter into F_ghost,
@@ -18,6 +26,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -47,11 +56,32 @@ enum Role : int
         if (e_ != hipSuccess) return set_error(IBAMR_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-// Spring k: D = X_slave - X_master, R = |D|, T/R = kappa (R - rest) / R,
-// F = (T/R) D; skipped when R < DBL_EPSILON (IBStandardForceGen.cpp:903-911).
+// Tension T(R) of spring law `law` with parameters p[0 .. np):
+//   IBAMR_SPRING_HOOKEAN: p[0] (R - p[1]) (default_spring_force,
+//     IBSpringForceFunctions.h:104-107);
+//   IBAMR_SPRING_LINEAR_STRAIN: p[0] (R / p[1] - 1) (the linear_spring_force
+//     examples/IB/explicit/ex3/example.cpp:58-63 registers);
+//   IBAMR_SPRING_POLYNOMIAL: x (p[0] + x (p[2] + ... + x p[np-1])), x = R - p[1].
+__device__ inline double spring_tension(int law, const double* __restrict__ p, int np, double R)
+{
+    if (law == IBAMR_SPRING_LINEAR_STRAIN) return p[0] * (R / p[1] - 1.0);
+    if (law == IBAMR_SPRING_POLYNOMIAL)
+    {
+        const double x = R - p[1];
+        double h = np > 2 ? p[np - 1] : p[0];
+        for (int m = np - 2; m >= 2; --m) h = p[m] + x * h;
+        if (np > 2) h = p[0] + x * h;
+        return x * h;
+    }
+    return p[0] * (R - p[1]);
+}
+
+// Spring k: D = X_slave - X_master, R = |D|, T/R = T(R) / R, F = (T/R) D;
+// skipped when R < DBL_EPSILON (IBStandardForceGen.cpp:903-911).
 __global__ void k_spring_elem(int ns, int ndim, const int* __restrict__ m, const int* __restrict__ sl,
-                              const double* __restrict__ kappa, const double* __restrict__ rest,
-                              const double* __restrict__ X, double* __restrict__ Fs, unsigned char* __restrict__ skip)
+                              const int* __restrict__ law, const int* __restrict__ poff,
+                              const double* __restrict__ par, const double* __restrict__ X, double* __restrict__ Fs,
+                              unsigned char* __restrict__ skip)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
@@ -63,7 +93,8 @@ __global__ void k_spring_elem(int ns, int ndim, const int* __restrict__ m, const
     const double R = sqrt(R2);
     const bool sk = R < DBL_EPSILON;
     skip[k] = sk ? 1 : 0;
-    const double T_over_R = sk ? 0.0 : (kappa[k] * (R - rest[k])) / R;
+    const int o = poff[k];
+    const double T_over_R = sk ? 0.0 : spring_tension(law[k], par + o, poff[k + 1] - o, R) / R;
     for (int d = 0; d < ndim; ++d) Fs[(size_t)ndim * k + d] = T_over_R * D[d];
 }
 
@@ -148,7 +179,9 @@ struct ibamr_forces
 {
     int ndim = 3, n = 0, ns = 0, nb = 0, nt = 0;
     int *s_m = nullptr, *s_s = nullptr;
-    double *s_k = nullptr, *s_r = nullptr;
+    int *s_law = nullptr, *s_off = nullptr;  // per spring: law, first parameter ([ns + 1])
+    double* s_par = nullptr;
+    std::map<int, int> registry{{0, IBAMR_SPRING_HOOKEAN}};  // force function index -> law
     int *b_m = nullptr, *b_n = nullptr, *b_p = nullptr;
     double *b_K = nullptr, *b_c = nullptr;
     int* t_i = nullptr;
@@ -164,7 +197,7 @@ extern "C" {
 int ibamr_forces_destroy(ibamr_forces* f)
 {
     if (!f) return IBAMR_OK;
-    for (void* p : {(void*)f->s_m, (void*)f->s_s, (void*)f->s_k, (void*)f->s_r, (void*)f->b_m, (void*)f->b_n,
+    for (void* p : {(void*)f->s_m, (void*)f->s_s, (void*)f->s_law, (void*)f->s_off, (void*)f->s_par, (void*)f->b_m, (void*)f->b_n,
                     (void*)f->b_p, (void*)f->b_K, (void*)f->b_c, (void*)f->t_i, (void*)f->t_K, (void*)f->t_E,
                     (void*)f->t_X0, (void*)f->start, (void*)f->ent, (void*)f->Fs, (void*)f->Fb, (void*)f->skip})
         dfree(p);
@@ -233,8 +266,20 @@ int ibamr_forces_create(ibamr_forces** out, int ndim, int npoints, int nsprings,
     };
     chk(dev_copy(&f->s_m, spring_master, (size_t)nsprings));
     chk(dev_copy(&f->s_s, spring_slave, (size_t)nsprings));
-    chk(dev_copy(&f->s_k, spring_kappa, (size_t)nsprings));
-    chk(dev_copy(&f->s_r, spring_rest, (size_t)nsprings));
+    {
+        // the default force function: law IBAMR_SPRING_HOOKEAN with {kappa, rest}
+        std::vector<int> law((size_t)nsprings, IBAMR_SPRING_HOOKEAN), off((size_t)nsprings + 1);
+        std::vector<double> par((size_t)nsprings * 2);
+        for (int k = 0; k <= nsprings; ++k) off[k] = 2 * k;
+        for (int k = 0; k < nsprings; ++k)
+        {
+            par[2 * (size_t)k] = spring_kappa[k];
+            par[2 * (size_t)k + 1] = spring_rest[k];
+        }
+        chk(dev_copy(&f->s_law, law.data(), law.size()));
+        chk(dev_copy(&f->s_off, off.data(), nsprings > 0 ? off.size() : 0));
+        chk(dev_copy(&f->s_par, par.data(), par.size()));
+    }
     chk(dev_copy(&f->b_m, beam_master, (size_t)nbeams));
     chk(dev_copy(&f->b_n, beam_next, (size_t)nbeams));
     chk(dev_copy(&f->b_p, beam_prev, (size_t)nbeams));
@@ -263,6 +308,59 @@ int ibamr_forces_create(ibamr_forces** out, int ndim, int npoints, int nsprings,
     return IBAMR_OK;
 }
 
+int ibamr_forces_register_spring_function(ibamr_forces* f, int force_fcn_index, int law)
+{
+    if (!f) return set_error(IBAMR_ERR_INVALID, "null force object");
+    if (law != IBAMR_SPRING_HOOKEAN && law != IBAMR_SPRING_LINEAR_STRAIN && law != IBAMR_SPRING_POLYNOMIAL)
+        return set_error(IBAMR_ERR_INVALID, "unknown spring law " + std::to_string(law));
+    f->registry[force_fcn_index] = law;
+    return IBAMR_OK;
+}
+
+int ibamr_forces_set_spring_parameters(ibamr_forces* f, const int* force_fcn_idx, const int* param_offsets,
+                                       const double* params)
+{
+    if (!f) return set_error(IBAMR_ERR_INVALID, "null force object");
+    const int ns = f->ns;
+    if (ns == 0) return IBAMR_OK;
+    if (!force_fcn_idx || !param_offsets || (!params && param_offsets[ns] > 0))
+        return set_error(IBAMR_ERR_INVALID, "null spring parameter array");
+    if (param_offsets[0] != 0) return set_error(IBAMR_ERR_INVALID, "spring parameter offsets must start at 0");
+    std::vector<int> law((size_t)ns);
+    for (int k = 0; k < ns; ++k)
+    {
+        const auto it = f->registry.find(force_fcn_idx[k]);
+        if (it == f->registry.end())
+            return set_error(IBAMR_ERR_INVALID, "spring " + std::to_string(k) + ": no spring force function is "
+                                                "registered under index " + std::to_string(force_fcn_idx[k]));
+        const int np = param_offsets[k + 1] - param_offsets[k];
+        if (np < 2)
+            return set_error(IBAMR_ERR_INVALID, "spring " + std::to_string(k) + ": " + std::to_string(np) +
+                                                    " parameters (at least stiffness and rest length)");
+        law[k] = it->second;
+    }
+    int* d_law = nullptr;
+    int* d_off = nullptr;
+    double* d_par = nullptr;
+    int rc = dev_copy(&d_law, law.data(), law.size());
+    if (rc == IBAMR_OK) rc = dev_copy(&d_off, param_offsets, (size_t)ns + 1);
+    if (rc == IBAMR_OK) rc = dev_copy(&d_par, params, (size_t)param_offsets[ns]);
+    if (rc != IBAMR_OK)
+    {
+        dfree(d_law);
+        dfree(d_off);
+        dfree(d_par);
+        return rc;
+    }
+    dfree(f->s_law);
+    dfree(f->s_off);
+    dfree(f->s_par);
+    f->s_law = d_law;
+    f->s_off = d_off;
+    f->s_par = d_par;
+    return IBAMR_OK;
+}
+
 int ibamr_forces_compute(ibamr_forces* f, void* F, int f_precision, const double* X, const void* U,
                          int u_precision, int mode, void* stream)
 {
@@ -278,7 +376,7 @@ int ibamr_forces_compute(ibamr_forces* f, void* F, int f_precision, const double
     if (f->ns > 0)
     {
         hipLaunchKernelGGL(k_spring_elem, dim3((f->ns + B - 1) / B), dim3(B), 0, s, f->ns, f->ndim, f->s_m, f->s_s,
-                           f->s_k, f->s_r, X, f->Fs, f->skip);
+                           f->s_law, f->s_off, f->s_par, X, f->Fs, f->skip);
         F_HIP(hipGetLastError());
     }
     if (f->nb > 0)

@@ -596,12 +596,19 @@ IBStandardForceGen::IBStandardForceGen(int ndim, int num_nodes, const std::vecto
 {
     std::vector<int> sm, ss, bm, bn, bp, tn;
     std::vector<double> sk, sr, bK, bc, tK, tE, tX;
+    d_par_off.push_back(0);
     for (const Spring& s : springs)
     {
         sm.push_back(s.master);
         ss.push_back(s.slave);
         sk.push_back(s.stiffness);
         sr.push_back(s.rest_length);
+        d_fcn_idx.push_back(s.force_fcn_idx);
+        d_par.push_back(s.stiffness);
+        d_par.push_back(s.rest_length);
+        d_par.insert(d_par.end(), s.extra_parameters.begin(), s.extra_parameters.end());
+        d_par_off.push_back((int)d_par.size());
+        if (s.force_fcn_idx != 0) d_resolve = true;
     }
     for (const Beam& b : beams)
     {
@@ -629,8 +636,23 @@ IBStandardForceGen::~IBStandardForceGen()
     ibamr_forces_destroy(d_forces);
 }
 
+void IBStandardForceGen::registerSpringForceFunction(int force_fcn_index, int law)
+{
+    abi_check(ibamr_forces_register_spring_function(d_forces, force_fcn_index, law),
+              "IBStandardForceGen::registerSpringForceFunction");
+    d_resolve = true;
+}
+
+void IBStandardForceGen::initializeLevelData()
+{
+    abi_check(ibamr_forces_set_spring_parameters(d_forces, d_fcn_idx.data(), d_par_off.data(), d_par.data()),
+              "IBStandardForceGen::initializeLevelData");
+    d_resolve = false;
+}
+
 void IBStandardForceGen::computeLagrangianForce(LData<double>& F, const LData<double>& X, const LData<double>& U)
 {
+    if (d_resolve) initializeLevelData();
     if (F.getLocalNodeCount() != d_n || X.getLocalNodeCount() != d_n || U.getLocalNodeCount() != d_n ||
         F.getDepth() != d_ndim || X.getDepth() != d_ndim || U.getDepth() != d_ndim)
         throw Error("IBStandardForceGen::computeLagrangianForce: F, X and U must hold NDIM values per node");

@@ -339,8 +339,9 @@ int ibamr_forward_euler_step_periodic(double* X_new, const double* X, const void
                                       const int* periodic, void* stream);
 
 /* Lagrangian force generation, IBStandardForceGen::computeLagrangianForce
- * (src/IB/IBStandardForceGen.cpp:254-305): Hookean springs (default_spring_force,
- * IBSpringForceFunctions.h:104: tension kappa (R - rest)), beams (three-point
+ * (src/IB/IBStandardForceGen.cpp:254-305): springs (created Hookean,
+ * default_spring_force, IBSpringForceFunctions.h:104: tension kappa (R - rest);
+ * other laws below), beams (three-point
  * bending with rigidity and reference curvature, :1038-1147) and target points
  * (kappa (X0 - X) - eta U, :1202-1299).  Element arrays are host arrays, node
  * indices 0-based into the n points; curvature and X0 are [element][ndim].
@@ -363,6 +364,38 @@ int ibamr_forces_create(ibamr_forces** forces, int ndim, int npoints, int nsprin
 int ibamr_forces_compute(ibamr_forces* forces, void* F, int f_precision, const double* X, const void* U,
                          int u_precision, int mode, void* stream);
 int ibamr_forces_destroy(ibamr_forces* forces);
+
+/* Spring force functions (IBStandardForceGen::registerSpringForceFunction,
+ * src/IB/IBStandardForceGen.cpp:133-140, and the per-spring force_fcn_idx and
+ * parameter lists of IBSpringForceSpec / the .spring files,
+ * IBStandardInitializer.cpp:397-434).  The reference registers host function
+ * pointers T(R, params); on the GPU the registry maps a force function index
+ * to one of these built-in laws (params p[0 .. n), n >= 2):
+ *   IBAMR_SPRING_HOOKEAN        p[0] (R - p[1])  (default_spring_force,
+ *                               IBSpringForceFunctions.h:104; index 0 holds it
+ *                               from creation, as in the reference's constructor)
+ *   IBAMR_SPRING_LINEAR_STRAIN  p[0] (R / p[1] - 1)  (the linear_spring_force
+ *                               of examples/IB/explicit/ex3/example.cpp:58-63)
+ *   IBAMR_SPRING_POLYNOMIAL     x (p[0] + x (p[2] + ... + x p[n-1])), x = R - p[1]
+ *                               (no reference counterpart: a nonlinear law in
+ *                               the extension; with n == 2 it is the Hookean law)
+ * ibamr_forces_set_spring_parameters gives spring k its force function index
+ * and parameters params[param_offsets[k] .. param_offsets[k+1]) (offsets:
+ * [nsprings + 1], from 0) and resolves every index through the registry then,
+ * as the reference resolves them when level data is initialised
+ * (IBStandardForceGen.cpp:728-790): register first, then set.  Unregistered
+ * indices and fewer than two parameters are errors.  Forces use
+ * F = (T(R) / R) D per spring as before, bitwise in FP64 with the reference
+ * loop evaluating the same law. */
+enum
+{
+    IBAMR_SPRING_HOOKEAN = 0,
+    IBAMR_SPRING_LINEAR_STRAIN = 1,
+    IBAMR_SPRING_POLYNOMIAL = 2
+};
+int ibamr_forces_register_spring_function(ibamr_forces* forces, int force_fcn_index, int law);
+int ibamr_forces_set_spring_parameters(ibamr_forces* forces, const int* force_fcn_idx, const int* param_offsets,
+                                       const double* params);
 
 /* Kernel timing for measurement.  With timing on, HIP events are recorded on
  * the launch stream right around each spread tile kernel and each

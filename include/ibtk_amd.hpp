@@ -257,14 +257,17 @@ private:
     std::unique_ptr<Impl> d_impl;
 };
 
-// IBStandardForceGen (src/IB/IBStandardForceGen.cpp:254-305): Hookean springs
-// (default_spring_force), beams and target points of one structure, given as
-// IBStandardInitializer reads them (node indices 0-based; beam curvature and
-// target positions [element][NDIM]).
+// IBStandardForceGen (src/IB/IBStandardForceGen.cpp:254-305): springs, beams
+// and target points of one structure, given as IBStandardInitializer reads them
+// (node indices 0-based; beam curvature and target positions [element][NDIM]).
+// A spring's parameters are {stiffness, rest_length, extra_parameters...} and
+// its force function index selects the law (IBStandardInitializer.cpp:397-434).
 struct Spring
 {
     int master, slave;
     double stiffness, rest_length;
+    int force_fcn_idx = 0;
+    std::vector<double> extra_parameters;
 };
 struct Beam
 {
@@ -290,10 +293,20 @@ public:
     // computeLagrangianForce: F += springs + beams + target points at X (U damps
     // the target points); IBMethod::computeLagrangianForce zeroes F first.
     void computeLagrangianForce(LData<double>& F, const LData<double>& X, const LData<double>& U);
+    // registerSpringForceFunction (IBStandardForceGen.cpp:133-140) with a
+    // built-in law (IBAMR_SPRING_HOOKEAN, _LINEAR_STRAIN, _POLYNOMIAL); index 0
+    // starts as the Hookean default.  Springs take the laws of their indices at
+    // initializeLevelData (:728-790), which computeLagrangianForce calls first
+    // when a registration or a non-default index is pending.
+    void registerSpringForceFunction(int force_fcn_index, int law);
+    void initializeLevelData();
 
 private:
     int d_ndim, d_n;
     ibamr_forces* d_forces = nullptr;
+    std::vector<int> d_fcn_idx, d_par_off;
+    std::vector<double> d_par;
+    bool d_resolve = false;
 };
 
 // The IBStrategy hooks of IBMethod on one level.

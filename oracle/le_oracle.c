@@ -795,10 +795,38 @@ double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const do
  * F_ghost = 0; springs (:814-930, default_spring_force kappa (R - rest),
  * IBSpringForceFunctions.h:104), beams (:1038-1147), target points (:1202-1299)
  * scattered into F_ghost in element order; then F += F_ghost. */
-void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
-                          const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,
-                          const int* bp, const double* bK, const double* bc, int nt, const int* tn, const double* tK,
-                          const double* tE, const double* tX0)
+/* Spring tension T(R) of the built-in spring laws, selected per spring:
+ *   law 0: default_spring_force, kappa (R - rest), params {kappa, rest}
+ *          (include/ibamr/IBSpringForceFunctions.h:104-107);
+ *   law 1: stiffness (R / rest - 1), params {stiffness, rest}: the
+ *          linear_spring_force examples/IB/explicit/ex3/example.cpp:58-63
+ *          registers (IBStandardForceGen::registerSpringForceFunction);
+ *   law 2: a polynomial in the extension x = R - rest, params {c1, rest, c2, ...,
+ *          cM}: T = x (c1 + x (c2 + ... + x cM)) (Horner, innermost last); with
+ *          params {c1, rest} it is law 0.  No counterpart among the reference's
+ *          functions: a built-in stand-in for a user's nonlinear law. */
+static double spring_tension(int law, const double* p, int np, double R)
+{
+    if (law == 1) return p[0] * (R / p[1] - 1.0);
+    if (law == 2)
+    {
+        const double x = R - p[1];
+        double h = np > 2 ? p[np - 1] : p[0];
+        for (int m = np - 2; m >= 2; --m) h = p[m] + x * h;
+        if (np > 2) h = p[0] + x * h;
+        return x * h;
+    }
+    return p[0] * (R - p[1]);
+}
+
+/* IBStandardForceGen::computeLagrangianForce with a spring law and parameter
+ * list per spring (the reference's force_fcns[k] and parameters[k],
+ * IBStandardForceGen.cpp:728-790, evaluated at :872 as force_fcn(R, params) / R).
+ * Spring k's parameters are par[poff[k] .. poff[k+1]). */
+void orc_lagrangian_force_laws(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
+                               const int* ss, const int* law, const int* poff, const double* par, int nb,
+                               const int* bm, const int* bn, const int* bp, const double* bK, const double* bc,
+                               int nt, const int* tn, const double* tK, const double* tE, const double* tX0)
 {
     double* G = (double*)calloc((size_t)(n > 0 ? n : 1) * (size_t)ndim, sizeof(double));
     for (int k = 0; k < ns; ++k)
@@ -810,7 +838,7 @@ void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const dou
         if (ndim == 3) R2 = R2 + D[2] * D[2];
         const double R = sqrt(R2);
         if (R < DBL_EPSILON) continue;
-        const double T_over_R = (sk[k] * (R - sr[k])) / R;
+        const double T_over_R = spring_tension(law[k], par + poff[k], poff[k + 1] - poff[k], R) / R;
         for (int d = 0; d < ndim; ++d)
         {
             const double f = T_over_R * D[d];
@@ -837,6 +865,29 @@ void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const dou
     }
     for (int i = 0; i < n * ndim; ++i) F[i] = F[i] + G[i];
     free(G);
+}
+
+/* The Hookean springs of IBStandardForceGen's default force function (index 0):
+ * law 0 with parameters {kappa[k], rest[k]}. */
+void orc_lagrangian_force(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
+                          const int* ss, const double* sk, const double* sr, int nb, const int* bm, const int* bn,
+                          const int* bp, const double* bK, const double* bc, int nt, const int* tn, const double* tK,
+                          const double* tE, const double* tX0)
+{
+    int* law = (int*)calloc((size_t)(ns > 0 ? ns : 1), sizeof(int));
+    int* poff = (int*)malloc(sizeof(int) * ((size_t)ns + 1));
+    double* par = (double*)malloc(sizeof(double) * (size_t)(ns > 0 ? 2 * ns : 1));
+    for (int k = 0; k <= ns; ++k) poff[k] = 2 * k;
+    for (int k = 0; k < ns; ++k)
+    {
+        par[2 * k] = sk[k];
+        par[2 * k + 1] = sr[k];
+    }
+    orc_lagrangian_force_laws(ndim, n, F, X, U, ns, sm, ss, law, poff, par, nb, bm, bn, bp, bK, bc, nt, tn, tK, tE,
+                              tX0);
+    free(law);
+    free(poff);
+    free(par);
 }
 
 /* ------------------------------------------------------------- redistribution */

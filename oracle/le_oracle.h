@@ -152,6 +152,12 @@ double orc_bruteforce_spread_at(int kernel, int ndim, const double* xg, const do
 
 /* IBStandardForceGen::computeLagrangianForce: F += spring, beam and target
  * point forces in the reference's scatter order (src/IB/IBStandardForceGen.cpp:254-305). */
+/* (orc_lagrangian_force below.)  The same with a spring law (0 Hookean, 1 linear strain, 2 polynomial in the
+ * extension) and a parameter list par[poff[k] .. poff[k+1]) per spring. */
+void orc_lagrangian_force_laws(int ndim, int n, double* F, const double* X, const double* U, int ns, const int* sm,
+                               const int* ss, const int* law, const int* poff, const double* par, int nb,
+                               const int* bm, const int* bn, const int* bp, const double* bK, const double* bc,
+                               int nt, const int* tn, const double* tK, const double* tE, const double* tX0);
 /* LDataManager::beginDataRedistribution / computeNodeDistribution for one
  * level: wrap/clamp, periodic offsets, owner patch, new order by (patch, cell
  * in box order, Lagrangian index).  0 ok, 1 escaped, 2 in no patch, 3 lost. */

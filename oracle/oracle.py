@@ -319,10 +319,13 @@ def spread_patch(kernel, centering_side, dx, x_lower, ilower, iupper, gcw, array
     return arrays
 
 
-def lagrangian_force(ndim, X, F=None, U=None, springs=None, beams=None, targets=None):
+def lagrangian_force(ndim, X, F=None, U=None, springs=None, beams=None, targets=None, spring_laws=None):
     """IBStandardForceGen forces, reference scatter order (orc_lagrangian_force).
     springs = (master, slave, kappa, rest); beams = (prev, master, next,
-    rigidity, curvature); targets = (node, kappa, eta, X0).  Returns F + forces."""
+    rigidity, curvature); targets = (node, kappa, eta, X0).  Returns F + forces.
+    spring_laws = (law, offsets, params): a law (0 Hookean, 1 linear strain,
+    2 polynomial) and params[offsets[k]:offsets[k+1]] per spring, in place of
+    the Hookean kappa and rest (orc_lagrangian_force_laws)."""
     X = _dv(X).reshape(-1, ndim)
     n = X.shape[0]
     out = np.zeros((n, ndim)) if F is None else _dv(F).reshape(n, ndim).copy()
@@ -343,10 +346,26 @@ def lagrangian_force(ndim, X, F=None, U=None, springs=None, beams=None, targets=
         tn = np.zeros(1, np.int32)
         tK = tE = tX0 = np.zeros(max(1, ndim))
     Ud = None if U is None else _dv(U).reshape(n, ndim)
-    lib().orc_lagrangian_force(ndim, n, _d(out), _d(X), _d(Ud) if Ud is not None else None,
-                               len(springs[0]) if springs is not None else 0, _i(sm), _i(ss), _d(sk), _d(sr),
-                               len(beams[0]) if beams is not None else 0, _i(bm), _i(bn), _i(bp), _d(bK), _d(bc),
-                               len(targets[0]) if targets is not None else 0, _i(tn), _d(tK), _d(tE), _d(tX0))
+    nsp = len(springs[0]) if springs is not None else 0
+    nbm = len(beams[0]) if beams is not None else 0
+    ntg = len(targets[0]) if targets is not None else 0
+    Up = _d(Ud) if Ud is not None else None
+    if spring_laws is None:
+        lib().orc_lagrangian_force(ndim, n, _d(out), _d(X), Up, nsp, _i(sm), _i(ss), _d(sk), _d(sr),
+                                   nbm, _i(bm), _i(bn), _i(bp), _d(bK), _d(bc), ntg, _i(tn), _d(tK), _d(tE), _d(tX0))
+        return out
+    law, poff, par = i32(spring_laws[0]), i32(spring_laws[1]), _dv(spring_laws[2])
+    assert len(law) == nsp and len(poff) == nsp + 1
+    L = lib()
+    if not getattr(L, "_lfl_types", False):
+        L.orc_lagrangian_force_laws.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 3 + \
+            [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 5 + \
+            [ctypes.c_int] + [ctypes.c_void_p] * 4
+        L._lfl_types = True
+    law = law if nsp else np.zeros(1, np.int32)
+    par = par if len(par) else np.zeros(1)
+    L.orc_lagrangian_force_laws(ndim, n, _d(out), _d(X), Up, nsp, _i(sm), _i(ss), _i(law), _i(poff), _d(par),
+                                nbm, _i(bm), _i(bn), _i(bp), _d(bK), _d(bc), ntg, _i(tn), _d(tK), _d(tE), _d(tX0))
     return out
 
 

@@ -10,7 +10,9 @@
 //   interpolate_01 interp <IB_4|IB_6|BSPLINE_3> <output>
 //   interpolate_01 side   <kernel> <output>     raw-array side interp + spread
 //   interpolate_01 level  <kernel> <output>     LDataManager spread/interp, forwardEulerStep
-//   interpolate_01 regrid <kernel> <output>     IBStandardForceGen, then LDataManager::redistribute
+//   interpolate_01 regrid <kernel> <output>     IBStandardForceGen (springs under force functions 0,
+//                                               1 and 2: Hookean, linear strain, polynomial), then
+//                                               LDataManager::redistribute
 #include "ibtk_amd.hpp"
 
 #include <cmath>
@@ -239,11 +241,16 @@ int run_regrid(const std::string& k, const std::string& out_name)
     std::vector<TargetPoint> targets;
     for (int i = 0; i < n; ++i)
     {
-        springs.push_back({i, (i + 1) % n, 1.0 + 0.01 * i, 0.05});
+        Spring s{i, (i + 1) % n, 1.0 + 0.01 * i, 0.05};
+        s.force_fcn_idx = i % 3;
+        if (i % 3 == 2) s.extra_parameters = {30.0, -40.0};
+        springs.push_back(s);
         beams.push_back({(i + n - 1) % n, i, (i + 1) % n, 0.5, {{0.001, -0.002, 0.0}}});
         if (i % 7 == 0) targets.push_back({i, 3.0, 0.5, {{0.5, 0.5, 0.5}}});
     }
     IBStandardForceGen gen(NDIM, n, springs, beams, targets);
+    gen.registerSpringForceFunction(1, IBAMR_SPRING_LINEAR_STRAIN);
+    gen.registerSpringForceFunction(2, IBAMR_SPRING_POLYNOMIAL);
     LData<double> X(n, NDIM), F(n, NDIM), U(n, NDIM);
     LData<int> lag(n, 1);
     std::vector<int> lh(n);

@@ -33,3 +33,32 @@ def ring3d(n=500, seed=9):
     tn = np.arange(0, n, 7)
     targets = (tn, rng.uniform(100, 1000, len(tn)), rng.uniform(0.0, 2.0, len(tn)), X0[tn])
     return 3, X, springs, beams, targets
+
+
+def ring3d_laws(seed=10):
+    """ring3d with the springs spread over three force function indices:
+    0 Hookean {kappa, rest}, 1 linear strain {stiffness, rest > 0}, 7 a
+    polynomial in the extension with 2 to 5 parameters."""
+    ndim, X, springs, beams, targets = ring3d()
+    rng = np.random.default_rng(seed)
+    ns = len(springs[0])
+    idx = rng.choice([0, 1, 7], ns).astype(np.int32)
+    par = []
+    for k in range(ns):
+        rest = float(springs[3][k])
+        if idx[k] == 0:
+            par.append([float(springs[2][k]), rest])
+        elif idx[k] == 1:
+            par.append([float(springs[2][k]), rest if rest > 0 else 0.01])
+        else:
+            par.append([float(springs[2][k]), rest] + list(rng.uniform(-50.0, 50.0, rng.integers(0, 4))))
+    off = np.zeros(ns + 1, dtype=np.int32)
+    off[1:] = np.cumsum([len(p) for p in par])
+    registry = {0: 0, 1: 1, 7: 2}
+    return ndim, X, springs, beams, targets, registry, (idx, off, np.array([v for p in par for v in p]))
+
+
+def laws_of(registry, functions):
+    """(law per spring, offsets, params) from a registry and the per-spring indices."""
+    idx, off, par = functions
+    return np.array([registry[int(i)] for i in idx], dtype=np.int32), off, par

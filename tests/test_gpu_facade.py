@@ -225,7 +225,12 @@ def test_cpp_forcegen_and_redistribute(tmp_path, orc):
     beams = ((i - 1) % n, i, (i + 1) % n, np.full(n, 0.5), np.tile([0.001, -0.002, 0.0], (n, 1)))
     tn = np.arange(0, n, 7)
     targets = (tn, np.full(len(tn), 3.0), np.full(len(tn), 0.5), np.full((len(tn), 3), 0.5))
-    F = orc.lagrangian_force(3, Xb, U=Ub, springs=springs, beams=beams, targets=targets)
+    # spring i under force function i % 3, registered as Hookean, linear strain, polynomial {k, 0.05, 30, -40}
+    law = (i % 3).astype(np.int32)
+    par = [[1.0 + 0.01 * k, 0.05] + ([30.0, -40.0] if k % 3 == 2 else []) for k in range(n)]
+    off = np.concatenate([[0], np.cumsum([len(p) for p in par])]).astype(np.int32)
+    F = orc.lagrangian_force(3, Xb, U=Ub, springs=springs, beams=beams, targets=targets,
+                             spring_laws=(law, off, np.array([v for p in par for v in p])))
     lag = n - 1 - i
     boxes = [([0, 0, 0], [7, 15, 15]), ([8, 0, 0], [15, 15, 15])]
     rc, Xw, order, poff, _ = orc.redistribute(Xb, [0.0] * 3, [1.0] * 3, [1.0 / 16] * 3, [0] * 3, [15] * 3,

@@ -74,3 +74,63 @@ def test_forces_no_velocity_means_no_damping(orc):
     f.compute(Fd, torch.from_numpy(X).cuda())
     assert np.array_equal(Fd.cpu().numpy(), want)
     f.close()
+
+
+def _big_laws(seed=32):
+    """_big with every spring given index 0, 1 or 7 (Hookean, linear strain,
+    polynomial with 2 to 5 parameters)."""
+    ndim, X, springs, beams, targets = _big()
+    rng = np.random.default_rng(seed)
+    ns = len(springs[0])
+    idx = rng.choice([0, 1, 7], ns).astype(np.int32)
+    npar = np.where(idx == 7, rng.integers(2, 6, ns), 2)
+    off = np.zeros(ns + 1, dtype=np.int32)
+    off[1:] = np.cumsum(npar)
+    par = rng.uniform(-20.0, 20.0, int(off[-1]))
+    par[off[:-1]] = springs[2]
+    par[off[:-1] + 1] = np.where(idx == 1, springs[3] + 1e-3, springs[3])   # rest > 0 for the strain law
+    return ndim, X, springs, beams, targets, {0: 0, 1: 1, 7: 2}, (idx, off, par)
+
+
+@pytest.mark.parametrize("case", ["ring3d_laws", "big_laws"])
+def test_spring_laws_fp64_bitwise(orc, case):
+    """Springs under registered force functions (IBStandardForceGen::
+    registerSpringForceFunction): bitwise against the oracle evaluating the
+    same law per spring, with beams and target points beside them."""
+    ndim, X, springs, beams, targets, reg, fn = fc.ring3d_laws() if case == "ring3d_laws" else _big_laws()
+    print("spring laws", case, "n", len(X), flush=True)
+    rng = np.random.default_rng(7)
+    U = rng.standard_normal(X.shape)
+    want = orc.lagrangian_force(ndim, X, U=U, springs=springs, beams=beams, targets=targets,
+                                spring_laws=fc.laws_of(reg, fn))
+    f = structure.Forces(ndim, len(X), springs, beams, targets, registry=reg, spring_functions=fn)
+    Fd = torch.full(X.shape, float("nan"), dtype=torch.float64, device="cuda")
+    f.compute(Fd, torch.from_numpy(X).cuda(), torch.from_numpy(U).cuda())
+    got = Fd.cpu().numpy()
+    assert np.array_equal(got, want), np.abs(got - want).max()
+    f.close()
+
+
+def test_spring_registry_errors_and_default():
+    ndim, X, springs, beams, targets = fc.ring3d()
+    import ibamr_amd as ia
+    f = structure.Forces(ndim, len(X), springs)
+    ns = len(springs[0])
+    off = np.arange(0, 2 * ns + 1, 2)
+    par = np.stack([springs[2], springs[3]], axis=1).reshape(-1)
+    with pytest.raises(ia.IBAMRError, match="no spring force function is registered under index 3"):
+        f.set_spring_parameters(np.full(ns, 3), off, par)
+    with pytest.raises(ia.IBAMRError, match="unknown spring law"):
+        f.register_spring_function(3, 9)
+    short = np.arange(ns + 1)
+    with pytest.raises(ia.IBAMRError, match="at least stiffness and rest length"):
+        f.set_spring_parameters(np.zeros(ns), short, par[:ns])
+    # index 0 is the Hookean default: the same parameters through the registry give the created forces
+    Xd = torch.from_numpy(X).cuda()
+    F1 = torch.zeros(X.shape, dtype=torch.float64, device="cuda")
+    f.compute(F1, Xd)
+    f.set_spring_parameters(np.zeros(ns), off, par)
+    F2 = torch.zeros(X.shape, dtype=torch.float64, device="cuda")
+    f.compute(F2, Xd)
+    assert torch.equal(F1, F2)
+    f.close()
