@@ -49,6 +49,8 @@ C_API = [
     "ibamr_plan_set_physical_boundary", "ibamr_plan_set_physical_boundary_coefs",
     "ibamr_plan_accumulate_physical_boundary", "ibamr_plan_fill_physical_boundary", "ibamr_fe_quadrature_points",
     "ibamr_fe_mesh_create", "ibamr_fe_integrate_rhs", "ibamr_fe_mesh_destroy",
+    "ibamr_fe_adaptive_quadrature_orders", "ibamr_fe_quadrature_points_rules", "ibamr_fe_integrate_rhs_rules",
+    "ibamr_fe_nodal_mass", "ibamr_fe_nodal_weight",
     "ibamr_last_error", "ibamr_forces_create", "ibamr_forces_compute", "ibamr_forces_destroy",
     "ibamr_forces_register_spring_function", "ibamr_forces_set_spring_parameters",
     "ibamr_s0_interp", "ibamr_s0_spread", "ibamr_s0_interp_noalloc",
@@ -144,6 +146,15 @@ def lib():
                                        ctypes.POINTER(ctypes.c_int)]
     L.ibamr_fe_integrate_rhs.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp]
     L.ibamr_fe_mesh_destroy.argtypes = [vp]
+    L.ibamr_fe_adaptive_quadrature_orders.argtypes = [ctypes.c_int] * 5 + [vp, vp, ctypes.c_int, ctypes.c_double,
+                                                                          ctypes.c_double, vp, vp]
+    L.ibamr_fe_quadrature_points_rules.argtypes = [ctypes.c_int] * 5 + [vp, ctypes.c_int] + [vp] * 5 + \
+        [ctypes.c_long] + [vp] * 5 + [ctypes.c_int, vp]
+    L.ibamr_fe_integrate_rhs_rules.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [vp] * 7 + [ctypes.c_int, vp,
+                                                                                         ctypes.c_int, vp]
+    L.ibamr_fe_nodal_mass.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+    L.ibamr_fe_nodal_weight.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int,
+                                        ctypes.c_int, vp]
     L.ibamr_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_int)]
     L.ibamr_plan_set_tile_height.argtypes = [vp, ctypes.c_int]
@@ -556,9 +567,92 @@ def fe_quadrature_points(elem_nodes, phi, JxW, X_nodes, F_nodes=None, f_dtype=No
     return Xq[:nelem * nqp], (Fq[:nelem * nqp] if Fq is not None else None)
 
 
+FE_ELEM_TYPES = ["EDGE2", "EDGE3", "TRI3", "TRI6", "QUAD4", "QUAD8", "QUAD9", "TET4", "TET10", "HEX8", "HEX20",
+                 "HEX27"]
+FE_QGAUSS, FE_QGRID = 0, 1
+
+
+def fe_adaptive_quadrature_orders(elem_type, elem_nodes, X_nodes, quad_type, point_density, dx_min, stream=None):
+    """getQuadratureKey's adaptive order of every element (device int32 [nelem]):
+    elem_type a name of FE_ELEM_TYPES, elem_nodes [nelem][nbasis] int32 and
+    X_nodes [nnodes][ndim] float64 device tensors."""
+    import torch
+    nelem, nbasis = elem_nodes.shape
+    out = torch.empty(max(nelem, 1), dtype=torch.int32, device=X_nodes.device)
+    _check(lib().ibamr_fe_adaptive_quadrature_orders(FE_ELEM_TYPES.index(elem_type), X_nodes.shape[1], nelem, nbasis,
+                                                     X_nodes.shape[0], _ptr(elem_nodes), _ptr(X_nodes), quad_type,
+                                                     float(point_density), float(dx_min), _ptr(out),
+                                                     _stream_ptr(stream)))
+    return out[:nelem]
+
+
+class FERules:
+    """Quadrature rules per element on the device: phis[r] the shape function
+    table [nbasis][nqp_r] of rule r, elem_rule[e] the rule of element e.  Holds
+    rule_nqp, rule_phi_off, the concatenated tables and qp_off (element e's
+    points start at qp_off[e]; nqp_total points)."""
+
+    def __init__(self, phis, elem_rule, device="cuda"):
+        import numpy as np
+        import torch
+        er = np.ascontiguousarray(elem_rule, dtype=np.int32)
+        nqp = np.array([np.shape(p)[1] for p in phis], dtype=np.int32)
+        off = np.zeros(len(phis), dtype=np.int64)
+        off[1:] = np.cumsum([np.size(p) for p in phis])[:-1]
+        qp_off = np.zeros(len(er) + 1, dtype=np.int64)
+        qp_off[1:] = np.cumsum(nqp[er])
+        self.nrules = len(phis)
+        self.nqp_total = int(qp_off[-1])
+        self.qp_off_host = qp_off
+        self.elem_rule = torch.from_numpy(er).to(device)
+        self.rule_nqp = torch.from_numpy(nqp).to(device)
+        self.rule_phi_off = torch.from_numpy(off).to(device)
+        self.phi = torch.from_numpy(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1) for p in phis])).to(device)
+        self.qp_off = torch.from_numpy(qp_off).to(device)
+
+
+def fe_quadrature_points_rules(elem_nodes, rules, JxW, X_nodes, F_nodes=None, f_dtype=None, stream=None):
+    """fe_quadrature_points with a rule per element (FERules); JxW flat
+    [nqp_total] (element e's from qp_off[e]) or None.  Returns (X_qp, F_qp)."""
+    import torch
+    nelem, nbasis = elem_nodes.shape
+    ndim = X_nodes.shape[1]
+    tot = rules.nqp_total
+    if rules.elem_rule.numel() != nelem or (JxW is not None and JxW.numel() != tot):
+        raise ValueError("fe_quadrature_points_rules: one rule per element, JxW [nqp_total]")
+    dev = X_nodes.device
+    Xq = torch.empty((max(tot, 1), ndim), dtype=torch.float64, device=dev)
+    Fq = None
+    nvars = 0
+    if F_nodes is not None:
+        nvars = F_nodes.shape[1]
+        Fq = torch.empty((max(tot, 1), nvars), dtype=f_dtype or torch.float64, device=dev)
+    prec = FP32C if Fq is not None and Fq.dtype == torch.float32 else FP64
+    _check(lib().ibamr_fe_quadrature_points_rules(ndim, nvars, nelem, nbasis, X_nodes.shape[0], _ptr(elem_nodes),
+                                                  rules.nrules, _ptr(rules.elem_rule), _ptr(rules.rule_nqp),
+                                                  _ptr(rules.rule_phi_off), _ptr(rules.phi), _ptr(rules.qp_off), tot,
+                                                  _ptr(JxW), _ptr(X_nodes), _ptr(F_nodes), _ptr(Xq), _ptr(Fq), prec,
+                                                  _stream_ptr(stream)))
+    return Xq[:tot], (Fq[:tot] if Fq is not None else None)
+
+
+def fe_nodal_weight(V, M_diag, out=None, accumulate=False, stream=None):
+    """out = (or +=) V * M_diag[:, None] (nodal quadrature's F (x) dX and U * dX);
+    V and out float64 or float32 [nnodes][nvars] device tensors."""
+    import torch
+    n, nvars = V.shape
+    if out is None:
+        out = torch.empty_like(V)
+    _check(lib().ibamr_fe_nodal_weight(n, nvars, _ptr(V), FP32C if V.dtype == torch.float32 else FP64, _ptr(M_diag),
+                                       _ptr(out), FP32C if out.dtype == torch.float32 else FP64,
+                                       ACCUMULATE if accumulate else OVERWRITE, _stream_ptr(stream)))
+    return out
+
+
 class FEMesh:
     """Element connectivity on the GPU for FEDataManager::interpWeighted's
-    right-hand side (ibamr_fe_mesh_*).  elem_nodes: host int array [nelem][nbasis]."""
+    right-hand side and nodal quadrature's diagonal mass (ibamr_fe_mesh_*).
+    elem_nodes: host int array [nelem][nbasis]."""
 
     def __init__(self, elem_nodes, nnodes):
         import numpy as np
@@ -585,6 +679,29 @@ class FEMesh:
         _check(lib().ibamr_fe_integrate_rhs(self._h, nvars, nqp, _ptr(phi), _ptr(JxW), _ptr(U_qp), prec,
                                             _ptr(F_nodes), ACCUMULATE if accumulate else OVERWRITE,
                                             _stream_ptr(stream)))
+
+    def integrate_rhs_rules(self, F_nodes, U_qp, rules, JxW, accumulate=False, stream=None):
+        """integrate_rhs with a rule per element (FERules); JxW flat [nqp_total]."""
+        import torch
+        nvars = F_nodes.shape[1]
+        if tuple(U_qp.shape) != (rules.nqp_total, nvars) or JxW.numel() != rules.nqp_total or \
+                rules.elem_rule.numel() != self.nelem or F_nodes.dtype != torch.float64:
+            raise ValueError("integrate_rhs_rules: U_qp [nqp_total][nvars], JxW [nqp_total], F_nodes float64")
+        _check(lib().ibamr_fe_integrate_rhs_rules(self._h, nvars, rules.nrules, _ptr(rules.elem_rule),
+                                                  _ptr(rules.rule_nqp), _ptr(rules.rule_phi_off), _ptr(rules.phi),
+                                                  _ptr(rules.qp_off), _ptr(JxW), _ptr(U_qp),
+                                                  FP32C if U_qp.dtype == torch.float32 else FP64, _ptr(F_nodes),
+                                                  ACCUMULATE if accumulate else OVERWRITE, _stream_ptr(stream)))
+
+    def nodal_mass(self, phi, JxW, stream=None):
+        """Diagonal mass of a rule (the nodal rule for nodal quadrature): float64 [nnodes]."""
+        import torch
+        nqp = phi.shape[1]
+        if tuple(phi.shape) != (self.nbasis, nqp) or tuple(JxW.shape) != (self.nelem, nqp):
+            raise ValueError("nodal_mass: phi [nbasis][nqp], JxW [nelem][nqp]")
+        M = torch.empty(max(self.nnodes, 1), dtype=torch.float64, device=phi.device)
+        _check(lib().ibamr_fe_nodal_mass(self._h, nqp, _ptr(phi), _ptr(JxW), _ptr(M), _stream_ptr(stream)))
+        return M[:self.nnodes]
 
     def close(self):
         if getattr(self, "_h", None):

@@ -2,7 +2,8 @@
 (ibamr_fe_quadrature_points): what libMesh supplies to FEDataManager::spread in
 the reference (shape function values at the quadrature points of a rule, and
 JxW per element), for the first-order Lagrange elements QUAD4 and HEX8 with
-tensor-product Gauss-Legendre rules, plus structured box meshes.
+tensor-product Gauss-Legendre rules and the nodal (trapezoid) rule, plus
+structured box meshes.
 
 Node numbering follows libMesh's QUAD4 / HEX8 convention: reference
 coordinates in [-1, 1]^d, nodes counter-clockwise in the bottom face first
@@ -26,6 +27,20 @@ def gauss_rule(ndim, npts):
         pts.append([x[k] for k in i])
         wts.append(np.prod([w[k] for k in i]))
     return np.array(pts), np.array(wts)
+
+
+def nodal_rule(ndim):
+    """The nodal (trapezoid) rule of first-order Lagrange elements, libMesh's
+    build_nodal_qrule for QUAD4 / HEX8: the 2^ndim corners of [-1, 1]^ndim,
+    first coordinate fastest, weight 1 each."""
+    pts = np.array([[(-1.0, 1.0)[(i >> d) & 1] for d in range(ndim)] for i in range(2 ** ndim)])
+    return pts, np.ones(2 ** ndim)
+
+
+def gauss_rule_for_order(ndim, order):
+    """The tensor Gauss rule libMesh's QGAUSS builds for a given order: order // 2 + 1
+    points per direction (exact for polynomials of that degree per direction)."""
+    return gauss_rule(ndim, order // 2 + 1)
 
 
 def q1_shape(ndim, xi):

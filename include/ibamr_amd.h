@@ -270,6 +270,68 @@ int ibamr_fe_integrate_rhs(const ibamr_fe_mesh* mesh, int nvars, int nqp, const 
                            const void* U_qp, int u_precision, double* F_nodes, int mode, void* stream);
 int ibamr_fe_mesh_destroy(ibamr_fe_mesh* mesh);
 
+/* A quadrature rule per element.  The reference keys each element's rule
+ * separately (FEDataManager.cpp:1153-1165 spread, :1951-1963 interp); with
+ * adaptive quadrature the order follows the deformed element, getQuadratureKey
+ * (ibtk/src/utilities/libmesh_utilities.cpp:187-225):
+ *   hmax  = get_max_edge_length (ibtk/include/ibtk/libmesh_utilities.h:1373-1511):
+ *           TET4, HEX8: longest vertex-vertex edge; TET10, HEX27: longest edge as
+ *           the sum of its two segments through the mid-edge node; other types:
+ *           longest vertex pair on one edge (HEX20: the hexahedron's edges)
+ *   npts  = int(ceil(point_density * hmax / dx_min)), at least 2 (first-order
+ *           elements) or 3 (second-order)
+ *   order = min(2 npts - 1, 43) for IBAMR_FE_QGAUSS, npts for IBAMR_FE_QGRID.
+ * ibamr_fe_adaptive_quadrature_orders writes each element's order (device
+ * int[nelem]; -1 for an element with a node index outside [0, nnodes)); kernel
+ * only.  The caller's FE library gives the shape function table of each rule
+ * (libMesh builds the rule from the key, as in the reference); the rule-per-
+ * element forms of the point source and the weighted right-hand side then take
+ * rule r = elem_rule[e] with rule_nqp[r] points, its table at phi +
+ * rule_phi_off[r] ([nbasis][rule_nqp[r]]), element e's points and JxW values
+ * from qp_off[e] (qp_off [nelem + 1], qp_off[nelem] = nqp_total), all device
+ * arrays.  Arithmetic per point and per node as ibamr_fe_quadrature_points /
+ * ibamr_fe_integrate_rhs; an element whose rule index or point count does not
+ * fit gives NaN.  Kernels only. */
+enum
+{
+    IBAMR_FE_EDGE2 = 0, IBAMR_FE_EDGE3, IBAMR_FE_TRI3, IBAMR_FE_TRI6, IBAMR_FE_QUAD4, IBAMR_FE_QUAD8,
+    IBAMR_FE_QUAD9, IBAMR_FE_TET4, IBAMR_FE_TET10, IBAMR_FE_HEX8, IBAMR_FE_HEX20, IBAMR_FE_HEX27
+};
+enum
+{
+    IBAMR_FE_QGAUSS = 0,
+    IBAMR_FE_QGRID = 1
+};
+int ibamr_fe_adaptive_quadrature_orders(int elem_type, int ndim, int nelem, int nbasis, int nnodes,
+                                        const int* elem_nodes, const double* X_nodes, int quad_type,
+                                        double point_density, double dx_min, int* order, void* stream);
+int ibamr_fe_quadrature_points_rules(int ndim, int nvars, int nelem, int nbasis, int nnodes, const int* elem_nodes,
+                                     int nrules, const int* elem_rule, const int* rule_nqp, const long* rule_phi_off,
+                                     const double* phi, const long* qp_off, long nqp_total, const double* JxW,
+                                     const double* X_nodes, const double* F_nodes, double* X_qp, void* F_qp,
+                                     int f_precision, void* stream);
+int ibamr_fe_integrate_rhs_rules(const ibamr_fe_mesh* mesh, int nvars, int nrules, const int* elem_rule,
+                                 const int* rule_nqp, const long* rule_phi_off, const double* phi, const long* qp_off,
+                                 const double* JxW, const void* U_qp, int u_precision, double* F_nodes, int mode,
+                                 void* stream);
+
+/* Nodal quadrature (spread_spec / interp_spec use_nodal_quadrature,
+ * FEDataManager.cpp:996-1075 spread, :1761-1905 interp): the mesh nodes are the
+ * points.  ibamr_fe_nodal_mass forms the diagonal mass of a rule,
+ * FEProjector::buildDiagonalL2MassMatrix (FEProjector.cpp:587-672): M[v] = sum
+ * over v's elements in element order of sum_q (phi[k][q] * phi[k][q]) * JxW[e][q]
+ * (phi [nbasis][nqp] and JxW [nelem][nqp] of the nodal rule, build_nodal_qrule).
+ * ibamr_fe_nodal_weight forms out[n][i] = (IBAMR_SPREAD_OVERWRITE) or +=
+ * (IBAMR_SPREAD_ACCUMULATE) V[n][i] * M[n] over nnodes rows of nvars: the
+ * spread's F (x) dX (:1014), whose rows are spread from X_nodes by a level plan,
+ * and the interpolation's U * dX added into the right-hand side (:1883-1899)
+ * after a level plan's interp to X_nodes.  V and out are double (IBAMR_FP64) or
+ * float (IBAMR_FP32C).  Kernels only. */
+int ibamr_fe_nodal_mass(const ibamr_fe_mesh* mesh, int nqp, const double* phi, const double* JxW, double* M_diag,
+                        void* stream);
+int ibamr_fe_nodal_weight(int nnodes, int nvars, const void* V, int v_precision, const double* M_diag, void* out,
+                          int out_precision, int mode, void* stream);
+
 /* Physical boundary operators after the spread / before the interpolation on
  * non-periodic domains: CartCellRobinPhysBdryOp / CartSideRobinPhysBdryOp
  * with LINEAR interpolation (ibtk/src/boundary/physical_boundary/

@@ -1067,6 +1067,161 @@ void orc_fe_integrate_rhs(int nvars, int nelem, int nbasis, int nqp, const int* 
     free(rhs);
 }
 
+/* FEProjector::buildDiagonalL2MassMatrix (ibtk/src/lagrangian/FEProjector.cpp:587-672),
+ * the dX of nodal quadrature (FEDataManager.cpp:1008-1014, 1797-1800): per
+ * element M_e[k] = sum_q (phi[k][q] * phi[k][q]) * JxW[e][q] from 0.0, added to
+ * the nodes element by element into M (zeroed first).  phi and JxW are those of
+ * the nodal rule (build_nodal_qrule) the caller's FE library gives. */
+void orc_fe_nodal_mass(int nelem, int nbasis, int nnodes, const int* elem_nodes, int nqp, const double* phi,
+                       const double* JxW, double* M)
+{
+    for (int v = 0; v < nnodes; ++v) M[v] = 0.0;
+    for (int e = 0; e < nelem; ++e)
+        for (int k = 0; k < nbasis; ++k)
+        {
+            double m = 0.0;
+            for (int q = 0; q < nqp; ++q)
+                m += (phi[(long)k * nqp + q] * phi[(long)k * nqp + q]) * JxW[(long)e * nqp + q];
+            M[elem_nodes[(long)e * nbasis + k]] += m;
+        }
+}
+
+/* Element types of the adaptive quadrature rule (libMesh ElemType names). */
+enum
+{
+    ORC_EDGE2 = 0, ORC_EDGE3, ORC_TRI3, ORC_TRI6, ORC_QUAD4, ORC_QUAD8, ORC_QUAD9,
+    ORC_TET4, ORC_TET10, ORC_HEX8, ORC_HEX20, ORC_HEX27
+};
+
+static double orc_dist2(int ndim, const double* X, long a, long b)
+{
+    double s = 0.0;
+    for (int d = 0; d < ndim; ++d) s += (X[a * ndim + d] - X[b * ndim + d]) * (X[a * ndim + d] - X[b * ndim + d]);
+    return s;
+}
+
+/* get_max_edge_length (ibtk/include/ibtk/libmesh_utilities.h:1373-1511) of one
+ * element, nodes n[] in libMesh order: TET4 and HEX8 the longest vertex-vertex
+ * edge; TET10 and HEX27 the longest edge as the sum of its two segments through
+ * the mid-edge node; the other types (the reference's general branch) the
+ * longest vertex pair on one edge (1-D: the two vertices; 2-D: the sides; HEX20:
+ * the hexahedron's edges).  Squared lengths sum over the ndim coordinates. */
+double orc_fe_max_edge_length(int type, int ndim, const int* n, const double* X)
+{
+    static const int tet4[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    static const int hex8[12][2] = {{0, 1}, {0, 3}, {0, 4}, {1, 2}, {1, 5}, {2, 3},
+                                    {2, 6}, {3, 7}, {4, 5}, {4, 7}, {5, 6}, {6, 7}};
+    static const int tet10[6][3] = {{0, 4, 1}, {0, 6, 2}, {0, 7, 3}, {1, 5, 2}, {1, 8, 3}, {2, 9, 3}};
+    static const int hex27[12][3] = {{0, 8, 1},  {0, 11, 3}, {0, 12, 4}, {1, 9, 2},  {1, 13, 5}, {2, 10, 3},
+                                     {2, 14, 6}, {3, 15, 7}, {4, 16, 5}, {4, 19, 7}, {5, 17, 6}, {6, 18, 7}};
+    static const int tri[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+    static const int quad[4][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}};
+    double m2 = 0.0, m = 0.0;
+    int i;
+    switch (type)
+    {
+    case ORC_TET10:
+        for (i = 0; i < 6; ++i)
+        {
+            const double l = sqrt(orc_dist2(ndim, X, n[tet10[i][0]], n[tet10[i][1]])) +
+                             sqrt(orc_dist2(ndim, X, n[tet10[i][1]], n[tet10[i][2]]));
+            m = l > m ? l : m;
+        }
+        return m;
+    case ORC_HEX27:
+        for (i = 0; i < 12; ++i)
+        {
+            const double l = sqrt(orc_dist2(ndim, X, n[hex27[i][0]], n[hex27[i][1]])) +
+                             sqrt(orc_dist2(ndim, X, n[hex27[i][1]], n[hex27[i][2]]));
+            m = l > m ? l : m;
+        }
+        return m;
+    case ORC_TET4:
+        for (i = 0; i < 6; ++i)
+        {
+            const double l = orc_dist2(ndim, X, n[tet4[i][0]], n[tet4[i][1]]);
+            m2 = l > m2 ? l : m2;
+        }
+        return sqrt(m2);
+    case ORC_HEX8:
+    case ORC_HEX20:
+        for (i = 0; i < 12; ++i)
+        {
+            const double l = orc_dist2(ndim, X, n[hex8[i][0]], n[hex8[i][1]]);
+            m2 = l > m2 ? l : m2;
+        }
+        return sqrt(m2);
+    case ORC_TRI3:
+    case ORC_TRI6:
+        for (i = 0; i < 3; ++i)
+        {
+            const double l = orc_dist2(ndim, X, n[tri[i][0]], n[tri[i][1]]);
+            m2 = l > m2 ? l : m2;
+        }
+        return sqrt(m2);
+    case ORC_QUAD4:
+    case ORC_QUAD8:
+    case ORC_QUAD9:
+        for (i = 0; i < 4; ++i)
+        {
+            const double l = orc_dist2(ndim, X, n[quad[i][0]], n[quad[i][1]]);
+            m2 = l > m2 ? l : m2;
+        }
+        return sqrt(m2);
+    default: /* EDGE2, EDGE3 */
+        return sqrt(orc_dist2(ndim, X, n[0], n[1]));
+    }
+}
+
+/* getQuadratureKey's adaptive order (ibtk/src/utilities/libmesh_utilities.cpp:187-225)
+ * per element: npts = int(ceil(point_density * hmax / dx_min)); below 3 it is 2
+ * for first-order elements, 3 otherwise; QGAUSS (quad_type 0) order
+ * min(2 npts - 1, 43) (FORTYTHIRD), QGRID (1) order npts. */
+void orc_fe_adaptive_orders(int type, int ndim, int nelem, int nbasis, const int* elem_nodes, const double* X,
+                            int quad_type, double point_density, double dx_min, int* order)
+{
+    const int first = type == ORC_EDGE2 || type == ORC_TRI3 || type == ORC_QUAD4 || type == ORC_TET4 ||
+                      type == ORC_HEX8;
+    for (int e = 0; e < nelem; ++e)
+    {
+        const double hmax = orc_fe_max_edge_length(type, ndim, elem_nodes + (long)e * nbasis, X);
+        int npts = (int)ceil(point_density * hmax / dx_min);
+        if (npts < 3) npts = first ? 2 : 3;
+        order[e] = quad_type == 0 ? (2 * npts - 1 < 43 ? 2 * npts - 1 : 43) : npts;
+    }
+}
+
+/* The quadrature point source and the weighted right-hand side with a rule per
+ * element (the reference keys each element's rule separately, FEDataManager.cpp:
+ * 1153-1165, 1951-1963): element e uses rule r = elem_rule[e] with rule_nqp[r]
+ * points and table phi + rule_phi_off[r] ([nbasis][rule_nqp[r]]); its points and
+ * JxW values start at qp_off[e].  Arithmetic per point as in the one-rule forms. */
+void orc_fe_quadrature_points_rules(int ndim, int nvars, int nelem, int nbasis, const int* elem_nodes,
+                                    const int* elem_rule, const int* rule_nqp, const long* rule_phi_off,
+                                    const double* phi, const long* qp_off, const double* JxW,
+                                    const double* X_nodes, const double* F_nodes, double* X_qp, double* F_qp)
+{
+    for (int e = 0; e < nelem; ++e)
+    {
+        const int r = elem_rule[e];
+        orc_fe_quadrature_points(ndim, nvars, 1, nbasis, rule_nqp[r], elem_nodes + (long)e * nbasis,
+                                 phi + rule_phi_off[r], JxW ? JxW + qp_off[e] : NULL, X_nodes, F_nodes,
+                                 X_qp + qp_off[e] * ndim, F_nodes ? F_qp + qp_off[e] * nvars : NULL);
+    }
+}
+
+void orc_fe_integrate_rhs_rules(int nvars, int nelem, int nbasis, const int* elem_nodes, const int* elem_rule,
+                                const int* rule_nqp, const long* rule_phi_off, const double* phi, const long* qp_off,
+                                const double* JxW, const double* U_qp, double* F)
+{
+    for (int e = 0; e < nelem; ++e)
+    {
+        const int r = elem_rule[e];
+        orc_fe_integrate_rhs(nvars, 1, nbasis, rule_nqp[r], elem_nodes + (long)e * nbasis, phi + rule_phi_off[r],
+                             JxW + qp_off[e], U_qp + qp_off[e] * nvars, F);
+    }
+}
+
 /* ------------------------------------------------- physical boundary (Robin) ops */
 
 /* The LINEAR Robin ghost-cell operators of CartCellRobinPhysBdryOp and

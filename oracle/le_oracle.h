@@ -175,6 +175,24 @@ void orc_fe_quadrature_points(int ndim, int nvars, int nelem, int nbasis, int nq
 void orc_fe_integrate_rhs(int nvars, int nelem, int nbasis, int nqp, const int* elem_nodes, const double* phi,
                           const double* JxW, const double* U_qp, double* F);
 
+/* Nodal quadrature's diagonal mass (FEProjector.cpp:587-672), the adaptive
+ * quadrature order per element (libmesh_utilities.cpp:187-225 with
+ * get_max_edge_length, libmesh_utilities.h:1373-1511; element types 0..11 =
+ * EDGE2 EDGE3 TRI3 TRI6 QUAD4 QUAD8 QUAD9 TET4 TET10 HEX8 HEX20 HEX27) and the
+ * quadrature point source / right-hand side with a rule per element. */
+void orc_fe_nodal_mass(int nelem, int nbasis, int nnodes, const int* elem_nodes, int nqp, const double* phi,
+                       const double* JxW, double* M);
+double orc_fe_max_edge_length(int type, int ndim, const int* n, const double* X);
+void orc_fe_adaptive_orders(int type, int ndim, int nelem, int nbasis, const int* elem_nodes, const double* X,
+                            int quad_type, double point_density, double dx_min, int* order);
+void orc_fe_quadrature_points_rules(int ndim, int nvars, int nelem, int nbasis, const int* elem_nodes,
+                                    const int* elem_rule, const int* rule_nqp, const long* rule_phi_off,
+                                    const double* phi, const long* qp_off, const double* JxW,
+                                    const double* X_nodes, const double* F_nodes, double* X_qp, double* F_qp);
+void orc_fe_integrate_rhs_rules(int nvars, int nelem, int nbasis, const int* elem_nodes, const int* elem_rule,
+                                const int* rule_nqp, const long* rule_phi_off, const double* phi, const long* qp_off,
+                                const double* JxW, const double* U_qp, double* F);
+
 /* CartCellRobinPhysBdryOp / CartSideRobinPhysBdryOp LINEAR ghost operators on one
  * patch: adjoint = accumulateFromPhysicalBoundaryData, else setPhysicalBoundaryConditions.
  * boxes: nboxes x {codim, location, lo[3], hi[3]}; abg: [nboxes][arrays][3], one

@@ -438,6 +438,105 @@ def fe_integrate_rhs(elem_nodes, phi, JxW, U_qp, nnodes, F=None):
     return out
 
 
+FE_ELEM_TYPES = ["EDGE2", "EDGE3", "TRI3", "TRI6", "QUAD4", "QUAD8", "QUAD9", "TET4", "TET10", "HEX8", "HEX20",
+                 "HEX27"]
+
+
+def fe_nodal_mass(elem_nodes, phi, JxW, nnodes):
+    """Nodal quadrature's diagonal mass (orc_fe_nodal_mass): phi [nbasis][nqp]
+    and JxW [nelem][nqp] of the nodal rule."""
+    en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+    nelem, nbasis = en.shape
+    phi = _dv(phi)
+    M = np.zeros(max(nnodes, 1))
+    L = lib()
+    L.orc_fe_nodal_mass.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int), ctypes.c_int] + \
+        [ctypes.POINTER(ctypes.c_double)] * 3
+    L.orc_fe_nodal_mass(nelem, nbasis, nnodes, _i(en), phi.shape[1], _d(phi), _d(_dv(JxW)), _d(M))
+    return M[:nnodes]
+
+
+def fe_adaptive_orders(elem_type, elem_nodes, X_nodes, quad_type, point_density, dx_min):
+    """getQuadratureKey's adaptive order per element (orc_fe_adaptive_orders);
+    quad_type 0 = QGAUSS, 1 = QGRID."""
+    en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+    nelem, nbasis = en.shape
+    X = _dv(X_nodes)
+    out = np.zeros(max(nelem, 1), dtype=np.int32)
+    L = lib()
+    L.orc_fe_adaptive_orders.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int),
+                                                              ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                                              ctypes.c_double, ctypes.c_double,
+                                                              ctypes.POINTER(ctypes.c_int)]
+    L.orc_fe_adaptive_orders(FE_ELEM_TYPES.index(elem_type), X.shape[1], nelem, nbasis, _i(en), _d(X), quad_type,
+                             point_density, dx_min, _i(out))
+    return out[:nelem]
+
+
+def fe_max_edge_length(elem_type, nodes, X_nodes):
+    X = _dv(X_nodes)
+    n = np.ascontiguousarray(nodes, dtype=np.int32)
+    L = lib()
+    L.orc_fe_max_edge_length.restype = ctypes.c_double
+    L.orc_fe_max_edge_length.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_double)]
+    return L.orc_fe_max_edge_length(FE_ELEM_TYPES.index(elem_type), X.shape[1], _i(n), _d(X))
+
+
+def fe_rules_layout(phis, elem_rule):
+    """(rule_nqp, rule_phi_off, concatenated phi, qp_off) for tables phis[r]
+    ([nbasis][nqp_r]) and a rule index per element."""
+    nqp = np.array([np.shape(p)[1] for p in phis], dtype=np.int32)
+    phi_off = np.zeros(len(phis), dtype=np.int64)
+    phi_off[1:] = np.cumsum([np.size(p) for p in phis])[:-1]
+    phi = np.concatenate([_dv(p).reshape(-1) for p in phis])
+    er = np.ascontiguousarray(elem_rule, dtype=np.int32)
+    qp_off = np.zeros(len(er) + 1, dtype=np.int64)
+    qp_off[1:] = np.cumsum(nqp[er])
+    return nqp, phi_off, phi, qp_off
+
+
+def fe_quadrature_points_rules(elem_nodes, phis, elem_rule, JxW, X_nodes, F_nodes=None):
+    """The point source with a rule per element (orc_fe_quadrature_points_rules);
+    JxW flat, qp_off[e] onwards for element e."""
+    en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+    nelem, nbasis = en.shape
+    nqp, phi_off, phi, qp_off = fe_rules_layout(phis, elem_rule)
+    er = np.ascontiguousarray(elem_rule, dtype=np.int32)
+    Xn = _dv(X_nodes)
+    ndim = Xn.shape[1]
+    tot = int(qp_off[-1])
+    Xq = np.zeros((max(tot, 1), ndim))
+    Fn = None if F_nodes is None else _dv(F_nodes)
+    nvars = 0 if Fn is None else Fn.shape[1]
+    Fq = np.zeros((max(tot, 1), max(nvars, 1)))
+    Jw = None if JxW is None else _dv(JxW)
+    L = lib()
+    vp = ctypes.c_void_p
+    L.orc_fe_quadrature_points_rules.argtypes = [ctypes.c_int] * 4 + [vp] * 10
+    L.orc_fe_quadrature_points_rules(ndim, nvars, nelem, nbasis, _i(en), _i(er), _i(nqp), phi_off.ctypes.data,
+                                     _d(phi), qp_off.ctypes.data, _d(Jw) if Jw is not None else None, _d(Xn),
+                                     _d(Fn) if Fn is not None else None, _d(Xq), _d(Fq))
+    return Xq[:tot], (Fq[:tot, :nvars] if Fn is not None else None)
+
+
+def fe_integrate_rhs_rules(elem_nodes, phis, elem_rule, JxW, U_qp, nnodes, F=None):
+    """The weighted right-hand side with a rule per element (orc_fe_integrate_rhs_rules)."""
+    en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+    nelem, nbasis = en.shape
+    nqp, phi_off, phi, qp_off = fe_rules_layout(phis, elem_rule)
+    er = np.ascontiguousarray(elem_rule, dtype=np.int32)
+    U = _dv(U_qp)
+    nvars = U.shape[1]
+    out = np.zeros((nnodes, nvars)) if F is None else _dv(F).copy()
+    L = lib()
+    vp = ctypes.c_void_p
+    L.orc_fe_integrate_rhs_rules.argtypes = [ctypes.c_int] * 3 + [vp] * 9
+    L.orc_fe_integrate_rhs_rules(nvars, nelem, nbasis, _i(en), _i(er), _i(nqp), phi_off.ctypes.data, _d(phi),
+                                 qp_off.ctypes.data, _d(_dv(JxW)), _d(U), _d(out))
+    return out
+
+
 def phys_bdry(arrays, ilower, iupper, gcw, dx, boxes, abg, side, adjoint, depth=1):
     """CartCell/CartSideRobinPhysBdryOp LINEAR ops on one patch (orc_phys_bdry),
     in place on float64 C-contiguous numpy arrays (cell: [arr]; side: one per axis).

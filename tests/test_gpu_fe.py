@@ -138,3 +138,115 @@ def test_fe_interp_end_to_end(orc):
     assert np.max(np.abs(Fd.cpu().numpy() - want)) <= 1e-13 * np.abs(want).max()
     mesh.close()
     plan.close()
+
+
+def _c(a, dt=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+
+
+@pytest.mark.parametrize("ndim,n", [(2, (40, 30)), (3, (20, 16, 24))])
+def test_nodal_quadrature_mass_and_weights_bitwise(orc, ndim, n):
+    """Nodal quadrature: the diagonal mass of the nodal rule (and of a Gauss rule)
+    bitwise against the oracle; F (x) dX and U * dX into a right-hand side."""
+    print("fe nodal", ndim, n, flush=True)
+    X, E = _mesh(ndim, n, 21)
+    m = ia.FEMesh(E, len(X))
+    for xi, w in (fe.nodal_rule(ndim), fe.gauss_rule(ndim, 2)):
+        phi = fe.q1_shape(ndim, xi)
+        J = fe.q1_jxw(X, E, xi, w)
+        M = m.nodal_mass(_c(phi), _c(J))
+        want = orc.fe_nodal_mass(E, phi, J, len(X))
+        assert np.array_equal(M.cpu().numpy(), want)
+    rng = np.random.default_rng(22)
+    F = rng.standard_normal((len(X), ndim))
+    FdX = ia.fe_nodal_weight(_c(F), M)
+    assert np.array_equal(FdX.cpu().numpy(), F * want[:, None])
+    F32 = ia.fe_nodal_weight(_c(F), M, out=torch.empty(F.shape, dtype=torch.float32, device="cuda"))
+    assert np.array_equal(F32.cpu().numpy(), (F * want[:, None]).astype(np.float32))
+    R0 = rng.standard_normal((len(X), ndim))
+    R = _c(R0)
+    ia.fe_nodal_weight(_c(F.astype(np.float32), torch.float32), M, out=R, accumulate=True)
+    assert np.array_equal(R.cpu().numpy(), R0 + F.astype(np.float32).astype(np.float64) * want[:, None])
+    m.close()
+
+
+def test_nodal_quadrature_spread_keeps_the_force(orc):
+    """Nodes as points with F (x) dX through a periodic level plan: the spread
+    force integrates to sum_n F_n dX_n."""
+    N = 48
+    h = 1.0 / N
+    X, E = _mesh(3, (12, 10, 14), 23)
+    xi, w = fe.nodal_rule(3)
+    m = ia.FEMesh(E, len(X))
+    M = m.nodal_mass(_c(fe.q1_shape(3, xi)), _c(fe.q1_jxw(X, E, xi, w)))
+    F = np.random.default_rng(24).standard_normal((len(X), 3))
+    P = ia.fe_nodal_weight(_c(F), M)
+    level = ia.Level.uniform_periodic(N, 3)
+    plan = ia.LevelPlan(level, "IB_4", ia.SIDE, ia.FP64, max_points=len(X))
+    f = [torch.zeros(ia.array_shape(level, 0, plan.data_gcw, ia.SIDE, axis=a), dtype=torch.float64, device="cuda")
+         for a in range(3)]
+    plan.bind(0, f)
+    Xd = _c(X)
+    plan.bin(Xd)
+    plan.spread(0, P, Xd, mode=ia.OVERWRITE)
+    torch.cuda.synchronize()
+    g = plan.data_gcw[0]
+    FM = F * M.cpu().numpy()[:, None]
+    for a in range(3):   # the periodic side array holds face N == face 0: count the faces 0..N-1
+        tot = float(f[a][g:g + N, g:g + N, g:g + N].sum()) * h ** 3
+        assert abs(tot - FM[:, a].sum()) <= 1e-12 * np.abs(FM[:, a]).sum()
+    m.close()
+
+
+TYPES = {"EDGE2": 2, "EDGE3": 3, "TRI3": 3, "TRI6": 6, "QUAD4": 4, "QUAD8": 8, "QUAD9": 9, "TET4": 4, "TET10": 10,
+         "HEX8": 8, "HEX20": 20, "HEX27": 27}
+
+
+@pytest.mark.parametrize("etype", sorted(TYPES))
+def test_adaptive_orders_bitwise(orc, etype):
+    """getQuadratureKey's adaptive order of 20000 random elements of each type
+    (node coordinates spread over several cell widths, both rule families,
+    including densities where npts falls below its floor and above the QGAUSS
+    cap)."""
+    rng = np.random.default_rng(31 + len(etype))
+    nb = TYPES[etype]
+    ndim = 3
+    X = rng.uniform(0.0, 1.0, (5000, ndim)) * rng.choice([0.01, 0.1, 1.0, 10.0], (5000, 1))
+    E = rng.integers(0, len(X), (20000, nb)).astype(np.int32)
+    for qt in (ia.FE_QGAUSS, ia.FE_QGRID):
+        for dens in (0.3, 2.0):
+            got = ia.fe_adaptive_quadrature_orders(etype, _c(E, torch.int32), _c(X), qt, dens, 0.05)
+            want = orc.fe_adaptive_orders(etype, E, X, qt, dens, 0.05)
+            assert np.array_equal(got.cpu().numpy(), want), (qt, dens)
+    with pytest.raises(ia.IBAMRError, match="only for QGAUSS or QGRID"):
+        ia.fe_adaptive_quadrature_orders(etype, _c(E, torch.int32), _c(X), 5, 1.0, 0.05)
+
+
+@pytest.mark.parametrize("ndim,n", [(2, (30, 20)), (3, (12, 10, 8))])
+def test_rules_per_element_bitwise(orc, ndim, n):
+    """Adaptive orders of a distorted mesh pick a Gauss rule per element; the
+    quadrature points, F JxW and the weighted right-hand side with those rules
+    are bitwise the oracle's."""
+    print("fe rules", ndim, n, flush=True)
+    X, E = _mesh(ndim, n, 41)
+    etype = "QUAD4" if ndim == 2 else "HEX8"
+    h = (0.3 / np.array(n)).min()
+    orders = ia.fe_adaptive_quadrature_orders(etype, _c(E, torch.int32), _c(X), ia.FE_QGAUSS, 2.0, h).cpu().numpy()
+    assert np.array_equal(orders, orc.fe_adaptive_orders(etype, E, X, 0, 2.0, h))
+    uniq = sorted(set(orders.tolist()))
+    assert len(uniq) >= 2, uniq              # the distortion gives several rules
+    rules = [fe.gauss_rule_for_order(ndim, o) for o in uniq]
+    phis = [fe.q1_shape(ndim, xi) for xi, _ in rules]
+    er = np.array([uniq.index(o) for o in orders], dtype=np.int32)
+    J = np.concatenate([fe.q1_jxw(X, E[e:e + 1], *rules[r]).reshape(-1) for e, r in enumerate(er)])
+    F = np.random.default_rng(42).standard_normal((len(X), 2))
+    R = ia.FERules(phis, er)
+    gX, gF = ia.fe_quadrature_points_rules(_c(E, torch.int32), R, _c(J), _c(X), _c(F))
+    Xq, Fq = orc.fe_quadrature_points_rules(E, phis, er, J, X, F)
+    assert np.array_equal(gX.cpu().numpy(), Xq) and np.array_equal(gF.cpu().numpy(), Fq)
+    U = np.random.default_rng(43).standard_normal((R.nqp_total, 2))
+    m = ia.FEMesh(E, len(X))
+    G = torch.zeros((len(X), 2), dtype=torch.float64, device="cuda")
+    m.integrate_rhs_rules(G, _c(U), R, _c(J))
+    assert np.array_equal(G.cpu().numpy(), orc.fe_integrate_rhs_rules(E, phis, er, J, U, len(X)))
+    m.close()
