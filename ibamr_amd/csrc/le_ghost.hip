@@ -178,8 +178,9 @@ struct BdryArgs
 {
     const int4* patches_list;
     const int* boxes;
-    // Robin coefficients of codim-1 boxes: for box slot k and array j, a, b and g
-    // over the box's columns start at coef + coef_off[k * narr + j]
+    // Robin coefficients of codim-1 boxes: for box slot k and coefficient array j
+    // (an axis of side data, a component of cell data), a, b and g over the box's
+    // columns start at coef + coef_off[k * m + j], m = ndim (side) or depth (cell)
     const double* coef;
     const long* coef_off;
     const PatchDev* patches;
