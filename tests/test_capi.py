@@ -156,3 +156,36 @@ def test_forces_create_rejects_bad_elements(L):
     rc = ia.lib().ibamr_forces_create(ctypes.byref(h), 4, 3, 0, None, None, None, None, 0, None, None, None, None,
                                       None, 0, None, None, None, None)
     assert rc != 0 and "ndim" in ia.lib().ibamr_last_error().decode()
+
+
+def test_new_entry_points_reject_bad_arguments_before_device_work(L):
+    """Argument errors of the FE rule / nodal quadrature, spring law and varying
+    Robin coefficient entry points come back as IBAMR_ERR_INVALID with a message,
+    before any device call (so this runs without a GPU)."""
+    import ibamr_amd as ia
+    cases = [
+        (lambda: L.ibamr_fe_adaptive_quadrature_orders(99, 3, 1, 8, 8, None, None, 0, 1.0, 0.1, None, None),
+         "unknown element type"),
+        (lambda: L.ibamr_fe_adaptive_quadrature_orders(ia.FE_ELEM_TYPES.index("HEX8"), 3, 1, 8, 8, None, None, 5, 1.0,
+                                                       0.1, None, None), "only for QGAUSS or QGRID"),
+        (lambda: L.ibamr_fe_adaptive_quadrature_orders(ia.FE_ELEM_TYPES.index("HEX27"), 3, 1, 8, 8, None, None, 0,
+                                                       1.0, 0.1, None, None), "below the element type"),
+        (lambda: L.ibamr_fe_adaptive_quadrature_orders(ia.FE_ELEM_TYPES.index("TET4"), 3, 1, 4, 4, None, None, 1, 1.0,
+                                                       0.0, None, None), "dx_min must be positive"),
+        (lambda: L.ibamr_fe_nodal_weight(4, 3, None, 0, None, None, 0, 7, None), "bad mode"),
+        (lambda: L.ibamr_fe_nodal_weight(4, 3, None, 3, None, None, 0, 0, None), "bad precision"),
+        (lambda: L.ibamr_fe_nodal_mass(None, 8, None, None, None, None), "null mesh"),
+        (lambda: L.ibamr_fe_quadrature_points_rules(4, 1, 1, 8, 8, None, 1, None, None, None, None, None, 8, None,
+                                                    None, None, None, None, 0, None), "ndim must be 2 or 3"),
+        (lambda: L.ibamr_fe_quadrature_points_rules(3, 1, 1, 8, 8, None, 0, None, None, None, None, None, 8, None,
+                                                    None, None, None, None, 0, None), "bad element or rule sizes"),
+        (lambda: L.ibamr_fe_integrate_rhs_rules(None, 3, 1, None, None, None, None, None, None, None, 0, None, 0,
+                                                None), "null mesh"),
+        (lambda: L.ibamr_forces_register_spring_function(None, 0, 1), "null force object"),
+        (lambda: L.ibamr_forces_set_spring_parameters(None, None, None, None), "null force object"),
+        (lambda: L.ibamr_plan_set_physical_boundary_coefs(None, None, 0, None, 0), "null plan"),
+    ]
+    for call, msg in cases:
+        with pytest.raises(ia.IBAMRError, match=msg) as e:
+            ia._check(call())
+        assert e.value.code == ia.IBAMR_ERR_INVALID, msg
